@@ -1,0 +1,203 @@
+"""Benchmark of the D2Q9 BGK hot path on one MI355X.
+
+Default workload: periodic Taylor-Green vortex, 4096 x 4096, FP32, tau = 0.8,
+u0 = 0.05, wavenumber 2 pi / 4096; K timed steps after W warm-up steps.  A
+step is one fused pull launch (stream + bounce-back + BGK collide) over the
+whole lattice, with the populations resident in HBM.
+
+Prints ONE JSON line.  `value` is lattice updates per second (GLUPS) over the
+timed steps (wall clock around a synchronised region); the roofline object
+prices the step kernel's average duration (HIP events on the lattice's own
+stream) against 72 algorithmic bytes per cell update.  The CPU baseline is
+the oracle's fused FP32 step (OpenMP over rows) on a bounded sample of the
+same lattice.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "D2Q9 GLUPS & % HBM roofline; time-to-steady-state, warm-start vs cold"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
+BYTES_PER_CELL = {"f32": 72, "f64": 144, "bf16": 36, "f16": 36}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def tgv_lattice(ctx, n, storage, tau, u0):
+    from chemsim_amd import Lattice, Params, capi
+    from chemsim_amd import workloads as W
+    rho, ux, uy = W.taylor_green(n, u0, np.float64 if storage == "f64" else np.float32)
+    lat = Lattice(ctx, n, n, storage, Params(tau=tau, edge=capi.CS_EDGE_PERIODIC))
+    lat.init_equilibrium(rho, ux, uy)
+    return lat
+
+
+def time_steps(lat, steps):
+    lat.sync()
+    t0 = time.perf_counter()
+    ms = lat.step_timed(steps)          # HIP events on the lattice's stream; synchronises
+    lat.sync()
+    return time.perf_counter() - t0, ms
+
+
+def cpu_baseline(n, tau, u0, budget_s=12.0):
+    """Oracle fused FP32 step (OpenMP over rows) on the same TGV lattice, a
+    bounded number of steps; returns (GLUPS, threads, steps)."""
+    from chemsim_amd import workloads as W
+    from oracle import pyoracle as O
+    threads = min(16, os.cpu_count() or 1)
+    p = O.params(n, n, tau=tau, edge=O.EDGE_PERIODIC)
+    rho, ux, uy = W.taylor_green(n, u0, np.float32)
+    f = O.equilibrium(p, rho, ux, uy)
+    t0 = time.perf_counter()
+    f, used = O.run_omp(p, f, 1, threads=threads)
+    one = time.perf_counter() - t0
+    steps = int(max(1, min(200, budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    O.run_omp(p, f, steps, threads=threads)
+    dt = time.perf_counter() - t0
+    return n * n * steps / dt / 1e9, used, steps
+
+
+def load_profile_traffic(kernel_key: str):
+    """HBM bytes per launch of the step kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE) and scaled by a copy kernel of known size; the
+    summary lives in profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data.get(kernel_key)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f64", "bf16", "f16"])
+    ap.add_argument("--tau", type=float, default=0.8)
+    ap.add_argument("--u0", type=float, default=0.05)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the FP64 / 16-bit comparison runs")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from chemsim_amd import Context
+    ctx = Context(0)
+    n, storage = args.n, args.storage
+    cells = n * n
+    log(f"[bench] device {ctx.device_name()}; TGV {n}^2 {storage}; warmup {args.warmup}, steps {args.steps}")
+
+    lat = tgv_lattice(ctx, n, storage, args.tau, args.u0)
+    if args.warmup:
+        lat.step(args.warmup)
+    wall, ms = time_steps(lat, args.steps)
+    s_final = lat.sums_f64()
+    glups = cells * args.steps / wall / 1e9
+    kernel_ms = ms / args.steps
+    bpc = BYTES_PER_CELL[storage]
+    achieved = bpc * cells / (kernel_ms * 1e-3) / 1e9
+    log(f"[bench] {glups:.2f} GLUPS wall, {kernel_ms * 1e3:.1f} us/step (events), {achieved:.0f} GB/s algorithmic")
+
+    copy_bytes = 1 << 30
+    copy_ms = ctx.bench_copy(copy_bytes, 20)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    log(f"[bench] copy kernel {copy_gbs:.0f} GB/s (read + write)")
+
+    extras = {"copy_kernel_GBs": round(copy_gbs, 1),
+              "frac_of_copy": round(achieved / copy_gbs, 4),
+              "frac_at_144B_per_cell": round(144 * cells / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+              if storage == "f32" else None,
+              "mass_final": float(s_final[0])}
+
+    if not args.no_extras and storage == "f32":
+        # FP64 oracle run and 16-bit storage on the same workload, same steps
+        from chemsim_amd import workloads as W
+        lat.close()
+        ref = tgv_lattice(ctx, n, "f64", args.tau, args.u0)
+        m0 = ref.sums_f64()[0]
+        w64, ms64 = time_steps(ref, args.steps)
+        r64, x64, y64 = ref.moments_f64()
+        m1 = ref.sums_f64()[0]
+        ref.close()
+        extras["f64"] = {"GLUPS": round(cells * args.steps / w64 / 1e9, 3),
+                         "mass_drift_rel": abs(m1 - m0) / m0}
+        amp = W.taylor_green_decay(n, args.tau, args.steps)
+        rho0, ux0, uy0 = W.taylor_green(n, args.u0)
+        a64 = float((x64 * ux0 + y64 * uy0).sum() / (ux0 * ux0 + uy0 * uy0).sum())
+        extras["f64"]["decay_vs_analytic_rel"] = abs(a64 / amp - 1)
+        del rho0, ux0, uy0
+        den = np.sqrt((x64 ** 2 + y64 ** 2).sum())
+        rden = np.sqrt(((r64 - 1.0) ** 2).sum())
+        for st in ("f32", "bf16", "f16"):
+            l2 = tgv_lattice(ctx, n, st, args.tau, args.u0)
+            m0 = l2.sums_f64()[0]
+            if args.warmup:
+                pass
+            w2, ms2 = time_steps(l2, args.steps)
+            r, x, y = l2.moments_f64()
+            m1 = l2.sums_f64()[0]
+            l2.close()
+            extras[st] = {"GLUPS": round(cells * args.steps / w2 / 1e9, 3),
+                          "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
+                          "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden),
+                          "mass_drift_rel": abs(m1 - m0) / m0}
+            log(f"[bench] {st}: {extras[st]}")
+        del x64, y64, r64
+
+    cpu = None
+    if not args.no_cpu:
+        cg, threads, csteps = cpu_baseline(n, args.tau, args.u0)
+        cpu = {"value": round(cg, 5), "unit": "GLUPS", "cores": threads, "kind": "port",
+               "sample": f"oracle fused FP32 step, OpenMP over rows, TGV {n}x{n}, {csteps} steps"}
+        log(f"[bench] cpu baseline {cg:.4f} GLUPS on {threads} threads ({csteps} steps)")
+
+    traffic = load_profile_traffic(f"k_step_{storage}_periodic_fluid_{n}")
+    line = {
+        "metric": METRIC,
+        "value": round(glups, 3),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "f32", "f64": "f64", "bf16": "bf16-storage/f32-math", "f16": "f16-storage/f32-math"}[storage],
+        "data": "synthetic (analytic Taylor-Green initial state)",
+        "config": {"workload": f"D2Q9 BGK periodic Taylor-Green {n}x{n}, tau={args.tau}, u0={args.u0}, "
+                               f"k=2pi/{n}, {storage} storage",
+                   "lattice": [n, n], "cells": cells, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "bytes_per_cell": bpc, "kernel_us": round(kernel_ms * 1e3, 2)},
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,117 @@
+// Internal definitions shared by the translation units of libchemsim_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/chemsim.h"
+
+namespace cs {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const std::string& msg);
+void clear_error();
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define CS_HIP(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::cs::Error{CS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define CS_REQUIRE(cond, code, msg)                        \
+    do {                                                   \
+        if (!(cond)) throw ::cs::Error{(code), (msg)};     \
+    } while (0)
+
+// Runs f, converting exceptions into a status code + thread-local message.
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        clear_error();
+        f();
+        return CS_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return CS_ERR_NOMEM;
+    } catch (...) {
+        set_error("unknown error");
+        return CS_ERR_ARG;
+    }
+}
+
+// ------------------------------------------------------------- constants --
+// D2Q9 directions of lbm.rs:221-231 and the opposite map of lbm.rs:298-309.
+constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+
+// Per-run constants in the working precision T (float or double):
+// K = -dt/tau (lbm.rs:357); A, B, C of compute_equilibrium (lbm.rs:54-66);
+// weights (lbm.rs:209-219); Ladd lid terms 6 w_i (c_i . u_lid); 16-bit
+// storage bases w_i * rho0; Zou-He boundary values.
+template <class T>
+struct Consts {
+    T K, A, B, C, cs2;
+    T w[9];
+    T lid[9];
+    T base[9];
+    T in_ux, in_uy, out_rho;
+};
+
+Consts<float> make_consts_f32(const cs_params& p);
+Consts<double> make_consts_f64(const cs_params& p);
+
+// Node info word (one per cell, built on the device from the flags):
+//   bits 0..8   source cell of direction i is solid (wall or lid)
+//   bits 9..17  source cell of direction i is a moving lid
+//   bits 24..27 the cell's own flag
+constexpr uint32_t NODE_TYPE_SHIFT = 24;
+
+inline bool is_solid_flag(int t) { return t == CS_WALL || t == CS_LID; }
+
+size_t storage_bytes(int storage);
+
+}  // namespace cs
+
+// ----------------------------------------------------------------- handles --
+struct cs_ctx {
+    int device = 0;
+    hipDeviceProp_t prop{};
+};
+
+struct cs_lattice {
+    cs_ctx* ctx = nullptr;
+    int w = 0, h = 0, storage = CS_F32;
+    int pitch = 0;          // row pitch, elements
+    int64_t plane = 0;      // plane stride, elements
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;            // buf[cur] holds the current populations
+    uint8_t* flags = nullptr;   // h*pitch flags (device) or null
+    uint32_t* node = nullptr;   // h*pitch node info (device) or null
+    bool has_flags = false;
+    cs_params params{};
+    cs::Consts<float> k32{};
+    cs::Consts<double> k64{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float* snap = nullptr;      // 2 planes (ux, uy) of h*pitch
+    bool has_snap = false;
+    double* partials = nullptr; // reduction partials
+    int npartials = 0;
+    double* dscratch = nullptr; // small device scratch for reduction results
+    int* iscratch = nullptr;
+    double time = 0.0;
+};

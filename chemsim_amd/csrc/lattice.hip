@@ -1,0 +1,1209 @@
+// C ABI of the single-lattice path: handles, transfers, the fused step,
+// the reference's unfused phases, moments and FP64 reductions.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "cs_internal.hpp"
+#include "lbm_device.hpp"
+
+namespace cs {
+
+// ------------------------------------------------------------------ errors --
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+void clear_error() { g_error.clear(); }
+
+size_t storage_bytes(int storage)
+{
+    switch (storage) {
+    case CS_F32: return 4;
+    case CS_F64: return 8;
+    case CS_BF16DEV:
+    case CS_F16DEV: return 2;
+    default: return 0;
+    }
+}
+
+// ----------------------------------------------------------- constants ----
+// Evaluated exactly as lbm.rs evaluates them in Scalar = f32:
+// cs = dx / (sqrt(3) dt) (lbm.rs:84), cs2, cs4 (lbm.rs:55-56), A = 1/cs2,
+// B = 1/(2 cs4), C = -1/(2 cs2) (lbm.rs:64-66), K = -dt/tau (lbm.rs:357),
+// weights 16/36, 4/36, 1/36 (lbm.rs:209-219).
+static const double W_EXACT[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9,
+                                  1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+
+template <class T>
+static Consts<T> make_consts(const cs_params& p)
+{
+    Consts<T> k{};
+    volatile T dx = (T)p.delta_x, dt = (T)p.delta_t, tau = (T)p.tau, three = (T)3;
+    T cs = dx / (std::sqrt((T)three) * dt);
+    T cs2 = cs * cs;
+    T cs4 = cs2 * cs2;
+    k.cs2 = cs2;
+    k.A = (T)1 / cs2;
+    k.B = (T)1 / ((T)2 * cs4);
+    k.C = (T)-1 / ((T)2 * cs2);
+    k.K = -dt / tau;
+    volatile T n0 = 16, n1 = 4, n5 = 1, d = 36;
+    const T w0 = n0 / d, w1 = n1 / d, w5 = n5 / d;
+    const T ws[9] = {w0, w1, w1, w1, w1, w5, w5, w5, w5};
+    for (int i = 0; i < 9; ++i) {
+        k.w[i] = ws[i];
+        // Ladd moving wall: f_i = f*_opp(i) + 2 w_i rho_w (c_i . u_w) / cs^2,
+        // rho_w = 1, cs^2 = 1/3 (extension).
+        k.lid[i] = (T)(6.0 * W_EXACT[i] * (CX[i] * p.lid_ux + CY[i] * p.lid_uy));
+        k.base[i] = ws[i] * (T)p.rho0;
+    }
+    k.in_ux = (T)p.inlet_ux;
+    k.in_uy = (T)p.inlet_uy;
+    k.out_rho = (T)p.outlet_rho;
+    return k;
+}
+
+Consts<float> make_consts_f32(const cs_params& p) { return make_consts<float>(p); }
+Consts<double> make_consts_f64(const cs_params& p) { return make_consts<double>(p); }
+
+// ------------------------------------------------------------- kernels ----
+
+// Node info from flags (layout in cs_internal.hpp).
+__global__ void k_node_info(const uint8_t* __restrict__ flags, uint32_t* __restrict__ node, int w, int h,
+                            int pitch, int edge)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w || r >= h) return;
+    const int t = flags[(int64_t)r * pitch + c];
+    uint32_t info = (uint32_t)t << NODE_TYPE_SHIFT;
+    for (int i = 1; i < 9; ++i) {
+        int sr = r + CX[i], sc = c - CY[i];
+        if (edge == CS_EDGE_PERIODIC) {
+            sr = (sr + h) % h;
+            sc = (sc + w) % w;
+        } else if (sr < 0 || sr >= h || sc < 0 || sc >= w) {
+            continue;
+        }
+        const int st = flags[(int64_t)sr * pitch + sc];
+        if (st == CS_WALL || st == CS_LID) info |= 1u << i;
+        if (st == CS_LID) info |= 1u << (9 + i);
+    }
+    node[(int64_t)r * pitch + c] = info;
+}
+
+template <class T>
+__device__ __forceinline__ void equilibrium(const Consts<T>& k, T rho, T ux, T uy, T (&F)[9])
+{
+#pragma clang fp contract(off)
+    const T v2 = ux * ux + uy * uy;
+    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T s = ((T(1) + vc[i] * k.A) + (vc[i] * vc[i]) * k.B) + v2 * k.C;
+        F[i] = (rho * k.w[i]) * s;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void moments(const T (&B)[9], T& rho, T& mx, T& my, T& ux, T& uy)
+{
+#pragma clang fp contract(off)
+    rho = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
+    mx = ((((B[1] - B[3]) + B[5]) - B[6]) - B[7]) + B[8];
+    my = ((((B[2] - B[4]) + B[5]) + B[6]) - B[7]) - B[8];
+    const T inv = T(1) / rho;
+    ux = inv * mx;
+    uy = inv * my;
+}
+
+template <class St>
+__device__ __forceinline__ void load_own(const typename St::S* f, int64_t P, int64_t own,
+                                         const Consts<typename St::T>& k, typename St::T (&B)[9])
+{
+#pragma unroll
+    for (int i = 0; i < 9; ++i) B[i] = St::ld(f, (int64_t)i * P + own, k.base[i]);
+}
+
+// Dense f32 (pitched) <-> storage.
+template <class St>
+__global__ void k_from_f32(const float* __restrict__ src, typename St::S* __restrict__ dst, int w, int h, int pitch,
+                           int64_t P, Consts<typename St::T> k)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t own = (int64_t)r * pitch + c;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) St::st(dst, i * P + own, (typename St::T)src[i * P + own], k.base[i]);
+}
+
+template <class St>
+__global__ void k_to_f32(const typename St::S* __restrict__ src, float* __restrict__ dst, int w, int h, int pitch,
+                         int64_t P, Consts<typename St::T> k)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t own = (int64_t)r * pitch + c;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dst[i * P + own] = (float)St::ld(src, i * P + own, k.base[i]);
+}
+
+// f = compute_equilibrium(rho, u) (lbm.rs:43-71); inputs dense h*w planes in T.
+template <class St>
+__global__ void k_init_eq(const typename St::T* __restrict__ rho, const typename St::T* __restrict__ ux,
+                          const typename St::T* __restrict__ uy, typename St::S* __restrict__ f, int w, int h,
+                          int pitch, int64_t P, Consts<typename St::T> k)
+{
+    using T = typename St::T;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t d = (int64_t)r * w + c, own = (int64_t)r * pitch + c;
+    T F[9];
+    equilibrium<T>(k, rho[d], ux[d], uy[d], F);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) St::st(f, i * P + own, F[i], k.base[i]);
+}
+
+enum : unsigned {
+    M_RHO = 1u, M_MX = 2u, M_MY = 4u, M_UX = 8u, M_UY = 16u, M_SPEED = 32u, M_P = 64u
+};
+
+// Moments of the stored populations into dense double planes.
+template <class St>
+__global__ void k_moments(const typename St::S* __restrict__ f, int w, int h, int pitch, int64_t P,
+                          Consts<typename St::T> k, unsigned mask, double* __restrict__ out)
+{
+    using T = typename St::T;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t own = (int64_t)r * pitch + c, d = (int64_t)r * w + c, n = (int64_t)w * h;
+    T B[9];
+    load_own<St>(f, P, own, k, B);
+    T rho, mx, my, ux, uy;
+    moments<T>(B, rho, mx, my, ux, uy);
+    int slot = 0;
+    if (mask & M_RHO) out[(slot++) * n + d] = (double)rho;
+    if (mask & M_MX) out[(slot++) * n + d] = (double)mx;
+    if (mask & M_MY) out[(slot++) * n + d] = (double)my;
+    if (mask & M_UX) out[(slot++) * n + d] = (double)ux;
+    if (mask & M_UY) out[(slot++) * n + d] = (double)uy;
+    if (mask & M_SPEED) out[(slot++) * n + d] = (double)sqrt(ux * ux + uy * uy);  // lbm.rs:153
+    if (mask & M_P) out[(slot++) * n + d] = (double)(rho * k.cs2);                // lbm.rs:786
+}
+
+// Lattice::equilibrium (lbm.rs:156-160) or non_equilibrium (lbm.rs:162-173).
+template <class St>
+__global__ void k_equilibrium(const typename St::S* __restrict__ f, int w, int h, int pitch, int64_t P,
+                              Consts<typename St::T> k, int non_eq, double* __restrict__ out)
+{
+    using T = typename St::T;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t own = (int64_t)r * pitch + c, d = (int64_t)r * w + c, n = (int64_t)w * h;
+    T B[9], F[9];
+    load_own<St>(f, P, own, k, B);
+    T rho, mx, my, ux, uy;
+    moments<T>(B, rho, mx, my, ux, uy);
+    equilibrium<T>(k, rho, ux, uy, F);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i * n + d] = (double)(non_eq ? B[i] - F[i] : F[i]);
+}
+
+// State::is_unstable (lbm.rs:815-818): any f_eq[0] < 0.
+template <class St>
+__global__ void k_unstable(const typename St::S* __restrict__ f, int w, int h, int pitch, int64_t P,
+                           Consts<typename St::T> k, int* __restrict__ flag)
+{
+    using T = typename St::T;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t own = (int64_t)r * pitch + c;
+    T B[9], F[9];
+    load_own<St>(f, P, own, k, B);
+    T rho, mx, my, ux, uy;
+    moments<T>(B, rho, mx, my, ux, uy);
+    equilibrium<T>(k, rho, ux, uy, F);
+    if (F[0] < T(0)) atomicOr(flag, 1);
+}
+
+// ---- the reference's phases alone (REFERENCE mode, F32/F64 storage) ----
+// State::stream (lbm.rs:716-729): out_i[r][c] = in_i[r + cx_i][c - cy_i], zero
+// outside (or wrapped).
+template <class St>
+__global__ void k_stream(const typename St::S* __restrict__ fin, typename St::S* __restrict__ fout, int w, int h,
+                         int pitch, int64_t P, int edge)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        int sr = r + CX[i], sc = c - CY[i];
+        bool valid = true;
+        if (edge == CS_EDGE_PERIODIC) {
+            sr = (sr + h) % h;
+            sc = (sc + w) % w;
+        } else {
+            valid = sr >= 0 && sr < h && sc >= 0 && sc < w;
+        }
+        fout[i * P + (int64_t)r * pitch + c] = valid ? fin[i * P + (int64_t)sr * pitch + sc] : (typename St::S)0;
+    }
+}
+
+// State::bounce_back (lbm.rs:741-751): at solid cells f_i <- f_opp(i), in place.
+template <class St>
+__global__ void k_bounce_back(typename St::S* __restrict__ f, const uint8_t* __restrict__ flags, int w, int h,
+                              int pitch, int64_t P)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t own = (int64_t)r * pitch + c;
+    const int t = flags[own];
+    if (!(t == CS_WALL || t == CS_LID)) return;
+    typename St::S v[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) v[i] = f[i * P + own];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i * P + own] = v[OPP[i]];
+}
+
+// State::collide (lbm.rs:731-739) with BGK::evaluate (lbm.rs:350-364), every cell, in place.
+template <class St>
+__global__ void k_collide(typename St::S* __restrict__ f, int w, int h, int pitch, int64_t P,
+                          Consts<typename St::T> k)
+{
+    using T = typename St::T;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t own = (int64_t)r * pitch + c;
+    T B[9];
+    load_own<St>(f, P, own, k, B);
+    bgk_collide<T>(k, B);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) St::st(f, i * P + own, B[i], k.base[i]);
+}
+
+// ---- FP64 reductions ---------------------------------------------------
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+// Fixed-shape tree reduction of NV doubles per thread into block partials.
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* partials)
+{
+    __shared__ double sh[NV][RED_THREADS];
+    for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] = v[j];
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int j = 0; j < NV; ++j) partials[(int64_t)blockIdx.x * NV + j] = sh[j][0];
+}
+
+template <int NV>
+__global__ void k_reduce_final(const double* __restrict__ partials, int nblocks, double* __restrict__ out)
+{
+    double v[NV];
+    for (int j = 0; j < NV; ++j) v[j] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += RED_THREADS)
+        for (int j = 0; j < NV; ++j) v[j] += partials[(int64_t)b * NV + j];
+    __shared__ double sh[NV][RED_THREADS];
+    for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] = v[j];
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int j = 0; j < NV; ++j) out[j] = sh[j][0];
+}
+
+template <class St>
+__global__ __launch_bounds__(RED_THREADS) void k_sums(const typename St::S* __restrict__ f,
+                                                      const uint8_t* __restrict__ flags, int w, int h, int pitch,
+                                                      int64_t P, Consts<typename St::T> k, double* partials)
+{
+    using T = typename St::T;
+    double v[3] = {0.0, 0.0, 0.0};
+    const int64_t n = (int64_t)w * h;
+    for (int64_t d = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; d < n; d += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const int r = (int)(d / w), c = (int)(d - (int64_t)r * w);
+        const int64_t own = (int64_t)r * pitch + c;
+        if (flags) {
+            const int t = flags[own];
+            if (t == CS_WALL || t == CS_LID) continue;
+        }
+        T B[9];
+        load_own<St>(f, P, own, k, B);
+        T rho, mx, my, ux, uy;
+        moments<T>(B, rho, mx, my, ux, uy);
+        v[0] += (double)rho;
+        v[1] += (double)mx;
+        v[2] += (double)my;
+    }
+    block_reduce_store<3>(v, partials);
+}
+
+// Velocity snapshot (dense double planes ux, uy).
+template <class St>
+__global__ void k_snapshot(const typename St::S* __restrict__ f, int w, int h, int pitch, int64_t P,
+                           Consts<typename St::T> k, double* __restrict__ snap)
+{
+    using T = typename St::T;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t own = (int64_t)r * pitch + c, d = (int64_t)r * w + c, n = (int64_t)w * h;
+    T B[9];
+    load_own<St>(f, P, own, k, B);
+    T rho, mx, my, ux, uy;
+    moments<T>(B, rho, mx, my, ux, uy);
+    snap[d] = (double)ux;
+    snap[n + d] = (double)uy;
+}
+
+template <class St>
+__global__ __launch_bounds__(RED_THREADS) void k_residual(const typename St::S* __restrict__ f,
+                                                          const uint8_t* __restrict__ flags, int w, int h, int pitch,
+                                                          int64_t P, Consts<typename St::T> k,
+                                                          const double* __restrict__ snap, double* partials)
+{
+    using T = typename St::T;
+    double v[2] = {0.0, 0.0};
+    const int64_t n = (int64_t)w * h;
+    for (int64_t d = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; d < n; d += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const int r = (int)(d / w), c = (int)(d - (int64_t)r * w);
+        const int64_t own = (int64_t)r * pitch + c;
+        if (flags) {
+            const int t = flags[own];
+            if (t == CS_WALL || t == CS_LID) continue;
+        }
+        T B[9];
+        load_own<St>(f, P, own, k, B);
+        T rho, mx, my, ux, uy;
+        moments<T>(B, rho, mx, my, ux, uy);
+        const double dx = (double)ux - snap[d], dy = (double)uy - snap[n + d];
+        v[0] += dx * dx + dy * dy;
+        v[1] += (double)ux * (double)ux + (double)uy * (double)uy;
+    }
+    block_reduce_store<2>(v, partials);
+}
+
+__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n4)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// --------------------------------------------------------- host helpers ---
+
+static void check_lat(const cs_lattice* lat) { CS_REQUIRE(lat != nullptr, CS_ERR_ARG, "null lattice"); }
+
+template <class St>
+static const Consts<typename St::T>& consts_of(const cs_lattice* lat);
+template <> const Consts<float>& consts_of<StF32>(const cs_lattice* l) { return l->k32; }
+template <> const Consts<float>& consts_of<StBF16>(const cs_lattice* l) { return l->k32; }
+template <> const Consts<float>& consts_of<StF16>(const cs_lattice* l) { return l->k32; }
+template <> const Consts<double>& consts_of<StF64>(const cs_lattice* l) { return l->k64; }
+
+// Calls f.template operator()<St>() for the lattice's storage type.
+template <class F>
+static void dispatch_storage(int storage, F&& f)
+{
+    switch (storage) {
+    case CS_F32: f.template operator()<StF32>(); break;
+    case CS_F64: f.template operator()<StF64>(); break;
+    case CS_BF16DEV: f.template operator()<StBF16>(); break;
+    case CS_F16DEV: f.template operator()<StF16>(); break;
+    default: throw Error{CS_ERR_ARG, "bad storage"};
+    }
+}
+
+static dim3 cell_grid(const cs_lattice* l) { return dim3((unsigned)((l->w + 255) / 256), (unsigned)l->h); }
+
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) { CS_HIP(hipMalloc(&p, bytes)); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+static int g_cpt_override = 0;
+
+template <class St, int EDGE, int MODE, int CPT>
+static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
+{
+    StepArgs<typename St::T> a;
+    a.fin = fin;
+    a.fout = fout;
+    a.node = l->node;
+    a.w = l->w;
+    a.h = l->h;
+    a.pitch = l->pitch;
+    a.plane = l->plane;
+    a.tiles_x = (l->w + 64 * CPT - 1) / (64 * CPT);
+    a.k = consts_of<St>(l);
+    const int tiles_y = (l->h + 3) / 4;
+    const int nblocks = a.tiles_x * tiles_y;
+    hipLaunchKernelGGL((k_step<St, EDGE, MODE, CPT>), dim3(nblocks), dim3(256), 0, l->stream, a);
+}
+
+template <class St, int EDGE, int MODE>
+static void launch_step_mode(cs_lattice* l, const void* fin, void* fout)
+{
+    int cpt = g_cpt_override;
+    if (cpt == 0) cpt = (l->storage == CS_F64) ? 1 : 2;
+    if (cpt >= 4) launch_step_cpt<St, EDGE, MODE, 4>(l, fin, fout);
+    else if (cpt == 2) launch_step_cpt<St, EDGE, MODE, 2>(l, fin, fout);
+    else launch_step_cpt<St, EDGE, MODE, 1>(l, fin, fout);
+}
+
+template <class St>
+static void launch_step(cs_lattice* l)
+{
+    const void* fin = l->buf[l->cur];
+    void* fout = l->buf[l->cur ^ 1];
+    const bool periodic = l->params.edge == CS_EDGE_PERIODIC;
+    int mode = MODE_FLUID;
+    if (l->has_flags) mode = l->params.boundary == CS_BB_HALFWAY ? MODE_HALF : MODE_REF;
+    if (periodic) {
+        if (mode == MODE_FLUID) launch_step_mode<St, CS_EDGE_PERIODIC, MODE_FLUID>(l, fin, fout);
+        else if (mode == MODE_REF) launch_step_mode<St, CS_EDGE_PERIODIC, MODE_REF>(l, fin, fout);
+        else launch_step_mode<St, CS_EDGE_PERIODIC, MODE_HALF>(l, fin, fout);
+    } else {
+        if (mode == MODE_FLUID) launch_step_mode<St, CS_EDGE_ZERO, MODE_FLUID>(l, fin, fout);
+        else if (mode == MODE_REF) launch_step_mode<St, CS_EDGE_ZERO, MODE_REF>(l, fin, fout);
+        else launch_step_mode<St, CS_EDGE_ZERO, MODE_HALF>(l, fin, fout);
+    }
+    l->cur ^= 1;
+}
+
+static void do_steps(cs_lattice* l, int n)
+{
+    for (int s = 0; s < n; ++s) {
+        dispatch_storage(l->storage, [&]<class St>() { launch_step<St>(l); });
+        l->time += l->params.delta_t;  // lbm.rs:713
+    }
+    CS_HIP(hipGetLastError());
+}
+
+// Runs k_moments and copies the requested planes (in mask bit order) to host.
+static void moments_to_host(cs_lattice* l, unsigned mask, double* host)
+{
+    const int nplanes = __builtin_popcount(mask);
+    const int64_t n = (int64_t)l->w * l->h;
+    DevBuf out(sizeof(double) * n * nplanes);
+    dispatch_storage(l->storage, [&]<class St>() {
+        hipLaunchKernelGGL((k_moments<St>), cell_grid(l), dim3(256), 0, l->stream,
+                           (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane, consts_of<St>(l),
+                           mask, (double*)out.p);
+    });
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipMemcpyAsync(host, out.p, sizeof(double) * n * nplanes, hipMemcpyDeviceToHost, l->stream));
+    CS_HIP(hipStreamSynchronize(l->stream));
+}
+
+static void moments_f32(cs_lattice* l, unsigned mask, float* const* outs)
+{
+    const int nplanes = __builtin_popcount(mask);
+    const int64_t n = (int64_t)l->w * l->h;
+    std::vector<double> tmp((size_t)n * nplanes);
+    moments_to_host(l, mask, tmp.data());
+    for (int j = 0; j < nplanes; ++j)
+        for (int64_t d = 0; d < n; ++d) outs[j][d] = (float)tmp[(size_t)j * n + d];
+}
+
+static void reduce(cs_lattice* l, int nv, double* out_host)
+{
+    if (nv == 3) hipLaunchKernelGGL((k_reduce_final<3>), dim3(1), dim3(RED_THREADS), 0, l->stream, l->partials, RED_BLOCKS, l->dscratch);
+    else hipLaunchKernelGGL((k_reduce_final<2>), dim3(1), dim3(RED_THREADS), 0, l->stream, l->partials, RED_BLOCKS, l->dscratch);
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipMemcpyAsync(out_host, l->dscratch, sizeof(double) * nv, hipMemcpyDeviceToHost, l->stream));
+    CS_HIP(hipStreamSynchronize(l->stream));
+}
+
+static double residual_now(cs_lattice* l)
+{
+    CS_REQUIRE(l->has_snap, CS_ERR_STATE, "residual requested before a velocity snapshot");
+    dispatch_storage(l->storage, [&]<class St>() {
+        hipLaunchKernelGGL((k_residual<St>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, l->stream,
+                           (const typename St::S*)l->buf[l->cur], l->has_flags ? l->flags : nullptr, l->w, l->h,
+                           l->pitch, l->plane, consts_of<St>(l), (const double*)l->snap, l->partials);
+    });
+    double v[2];
+    reduce(l, 2, v);
+    if (v[1] > 0) return std::sqrt(v[0] / v[1]);
+    return v[0] > 0 ? INFINITY : 0.0;
+}
+
+static void snapshot(cs_lattice* l)
+{
+    dispatch_storage(l->storage, [&]<class St>() {
+        hipLaunchKernelGGL((k_snapshot<St>), cell_grid(l), dim3(256), 0, l->stream,
+                           (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane, consts_of<St>(l),
+                           (double*)l->snap);
+    });
+    CS_HIP(hipGetLastError());
+    l->has_snap = true;
+}
+
+static void upload_pitched(cs_lattice* l, void* dst, const void* src, size_t elem)
+{
+    CS_HIP(hipMemcpy2DAsync(dst, (size_t)l->pitch * elem, src, (size_t)l->w * elem, (size_t)l->w * elem,
+                            (size_t)9 * l->h, hipMemcpyHostToDevice, l->stream));
+}
+
+static void download_pitched(cs_lattice* l, void* dst, const void* src, size_t elem)
+{
+    CS_HIP(hipMemcpy2DAsync(dst, (size_t)l->w * elem, src, (size_t)l->pitch * elem, (size_t)l->w * elem,
+                            (size_t)9 * l->h, hipMemcpyDeviceToHost, l->stream));
+}
+
+}  // namespace cs
+
+using namespace cs;
+
+// ======================================================================= ABI
+
+extern "C" const char* cs_last_error(void) { return g_error.c_str(); }
+extern "C" int cs_abi_version(void) { return CS_ABI_VERSION; }
+
+extern "C" int cs_ctx_create(int device, cs_ctx** out)
+{
+    return guarded([&] {
+        CS_REQUIRE(out != nullptr, CS_ERR_ARG, "null output");
+        int count = 0;
+        CS_HIP(hipGetDeviceCount(&count));
+        CS_REQUIRE(device >= 0 && device < count, CS_ERR_ARG, "device index out of range");
+        CS_HIP(hipSetDevice(device));
+        auto* c = new cs_ctx;
+        c->device = device;
+        CS_HIP(hipGetDeviceProperties(&c->prop, device));
+        *out = c;
+    });
+}
+
+extern "C" int cs_ctx_destroy(cs_ctx* ctx)
+{
+    return guarded([&] { delete ctx; });
+}
+
+extern "C" int cs_ctx_device_name(cs_ctx* ctx, char* buf, size_t len)
+{
+    return guarded([&] {
+        CS_REQUIRE(ctx && buf && len > 0, CS_ERR_ARG, "bad argument");
+        std::snprintf(buf, len, "%s (%s, %d CUs)", ctx->prop.name, ctx->prop.gcnArchName,
+                      ctx->prop.multiProcessorCount);
+    });
+}
+
+extern "C" int cs_lattice_create(cs_ctx* ctx, int w, int h, int storage, cs_lattice** out)
+{
+    return guarded([&] {
+        CS_REQUIRE(ctx && out, CS_ERR_ARG, "null argument");
+        CS_REQUIRE(w >= 1 && h >= 1 && (int64_t)w * h <= (int64_t)1 << 31, CS_ERR_ARG, "bad lattice size");
+        CS_REQUIRE(storage_bytes(storage) != 0, CS_ERR_ARG, "bad storage");
+        CS_HIP(hipSetDevice(ctx->device));
+        auto* l = new cs_lattice;
+        l->ctx = ctx;
+        l->w = w;
+        l->h = h;
+        l->storage = storage;
+        const int align = (int)(256 / storage_bytes(storage));  // 256-byte rows
+        l->pitch = (w + align - 1) / align * align;
+        l->plane = (int64_t)l->pitch * h;
+        try {
+            const size_t bytes = (size_t)9 * l->plane * storage_bytes(storage);
+            CS_HIP(hipMalloc(&l->buf[0], bytes));
+            CS_HIP(hipMalloc(&l->buf[1], bytes));
+            CS_HIP(hipMemset(l->buf[0], 0, bytes));
+            CS_HIP(hipMemset(l->buf[1], 0, bytes));
+            CS_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
+            CS_HIP(hipEventCreate(&l->ev0));
+            CS_HIP(hipEventCreate(&l->ev1));
+            CS_HIP(hipMalloc(&l->partials, sizeof(double) * 3 * RED_BLOCKS));
+            CS_HIP(hipMalloc(&l->dscratch, sizeof(double) * 8));
+            CS_HIP(hipMalloc(&l->iscratch, sizeof(int) * 4));
+            cs_params p{};
+            p.tau = 1.0;
+            p.delta_x = 1.0;
+            p.delta_t = 1.0;
+            p.edge = CS_EDGE_ZERO;
+            p.boundary = CS_BB_REFERENCE;
+            p.outlet_rho = 1.0;
+            p.rho0 = 1.0;
+            l->params = p;
+            l->k32 = make_consts_f32(p);
+            l->k64 = make_consts_f64(p);
+        } catch (...) {
+            cs_lattice_destroy(l);
+            throw;
+        }
+        *out = l;
+    });
+}
+
+extern "C" int cs_lattice_destroy(cs_lattice* l)
+{
+    return guarded([&] {
+        if (!l) return;
+        if (l->stream) (void)hipStreamSynchronize(l->stream);
+        for (void* p : {l->buf[0], l->buf[1], (void*)l->flags, (void*)l->node, (void*)l->snap, (void*)l->partials,
+                        (void*)l->dscratch, (void*)l->iscratch})
+            if (p) (void)hipFree(p);
+        if (l->ev0) (void)hipEventDestroy(l->ev0);
+        if (l->ev1) (void)hipEventDestroy(l->ev1);
+        if (l->stream) (void)hipStreamDestroy(l->stream);
+        delete l;
+    });
+}
+
+extern "C" int cs_lattice_shape(const cs_lattice* l, int* w, int* h, int* storage)
+{
+    return guarded([&] {
+        check_lat(l);
+        if (w) *w = l->w;
+        if (h) *h = l->h;
+        if (storage) *storage = l->storage;
+    });
+}
+
+extern "C" int cs_lattice_set_params(cs_lattice* l, const cs_params* p)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(p != nullptr, CS_ERR_ARG, "null params");
+        CS_REQUIRE(p->edge == CS_EDGE_ZERO || p->edge == CS_EDGE_PERIODIC, CS_ERR_ARG, "bad edge mode");
+        CS_REQUIRE(p->boundary == CS_BB_REFERENCE || p->boundary == CS_BB_HALFWAY, CS_ERR_ARG, "bad boundary mode");
+        CS_REQUIRE(p->tau > 0 && p->delta_x > 0 && p->delta_t > 0, CS_ERR_ARG, "tau, delta_x, delta_t must be > 0");
+        CS_REQUIRE(p->outlet_rho > 0, CS_ERR_ARG, "outlet_rho must be > 0");
+        const bool edge_changed = p->edge != l->params.edge;
+        // Re-encode 16-bit storage when the deviation base changes.
+        if ((l->storage == CS_BF16DEV || l->storage == CS_F16DEV) && p->rho0 != l->params.rho0) {
+            const int64_t n = (int64_t)9 * l->w * l->h;
+            std::vector<float> tmp((size_t)n);
+            if (cs_lattice_download_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};
+            l->params = *p;
+            l->k32 = make_consts_f32(*p);
+            l->k64 = make_consts_f64(*p);
+            if (cs_lattice_upload_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};
+        }
+        l->params = *p;
+        l->k32 = make_consts_f32(*p);
+        l->k64 = make_consts_f64(*p);
+        if (edge_changed && l->has_flags) {
+            dim3 g((unsigned)((l->w + 255) / 256), (unsigned)l->h);
+            hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, l->stream, l->flags, l->node, l->w, l->h, l->pitch,
+                               (int)l->params.edge);
+            CS_HIP(hipGetLastError());
+        }
+    });
+}
+
+extern "C" int cs_lattice_set_flags(cs_lattice* l, const uint8_t* flags)
+{
+    return guarded([&] {
+        check_lat(l);
+        if (!flags) {
+            l->has_flags = false;
+            return;
+        }
+        const int64_t cells = (int64_t)l->w * l->h;
+        for (int64_t d = 0; d < cells; ++d)
+            CS_REQUIRE(flags[d] <= CS_OUTLET, CS_ERR_ARG, "flag value out of range");
+        if (!l->flags) {
+            CS_HIP(hipMalloc(&l->flags, (size_t)l->plane));
+            CS_HIP(hipMalloc(&l->node, sizeof(uint32_t) * (size_t)l->plane));
+            CS_HIP(hipMemsetAsync(l->flags, 0, (size_t)l->plane, l->stream));
+        }
+        CS_HIP(hipMemcpy2DAsync(l->flags, (size_t)l->pitch, flags, (size_t)l->w, (size_t)l->w, (size_t)l->h,
+                                hipMemcpyHostToDevice, l->stream));
+        dim3 g((unsigned)((l->w + 255) / 256), (unsigned)l->h);
+        hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, l->stream, l->flags, l->node, l->w, l->h, l->pitch,
+                           (int)l->params.edge);
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipStreamSynchronize(l->stream));
+        l->has_flags = true;
+    });
+}
+
+extern "C" int cs_lattice_upload_f32(cs_lattice* l, const float* soa)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(soa != nullptr, CS_ERR_ARG, "null data");
+        if (l->storage == CS_F32) {
+            upload_pitched(l, l->buf[l->cur], soa, 4);
+        } else {
+            DevBuf tmp(sizeof(float) * 9 * (size_t)l->plane);
+            upload_pitched(l, tmp.p, soa, 4);
+            dispatch_storage(l->storage, [&]<class St>() {
+                hipLaunchKernelGGL((k_from_f32<St>), cell_grid(l), dim3(256), 0, l->stream, (const float*)tmp.p,
+                                   (typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane, consts_of<St>(l));
+            });
+            CS_HIP(hipGetLastError());
+            CS_HIP(hipStreamSynchronize(l->stream));
+        }
+        CS_HIP(hipStreamSynchronize(l->stream));
+    });
+}
+
+extern "C" int cs_lattice_download_f32(cs_lattice* l, float* soa)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(soa != nullptr, CS_ERR_ARG, "null data");
+        if (l->storage == CS_F32) {
+            download_pitched(l, soa, l->buf[l->cur], 4);
+        } else {
+            DevBuf tmp(sizeof(float) * 9 * (size_t)l->plane);
+            dispatch_storage(l->storage, [&]<class St>() {
+                hipLaunchKernelGGL((k_to_f32<St>), cell_grid(l), dim3(256), 0, l->stream,
+                                   (const typename St::S*)l->buf[l->cur], (float*)tmp.p, l->w, l->h, l->pitch, l->plane,
+                                   consts_of<St>(l));
+            });
+            CS_HIP(hipGetLastError());
+            download_pitched(l, soa, tmp.p, 4);
+            CS_HIP(hipStreamSynchronize(l->stream));
+        }
+        CS_HIP(hipStreamSynchronize(l->stream));
+    });
+}
+
+extern "C" int cs_lattice_upload_f64(cs_lattice* l, const double* soa)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(soa != nullptr, CS_ERR_ARG, "null data");
+        if (l->storage == CS_F64) {
+            upload_pitched(l, l->buf[l->cur], soa, 8);
+            CS_HIP(hipStreamSynchronize(l->stream));
+            return;
+        }
+        const size_t n = (size_t)9 * l->w * l->h;
+        std::vector<float> tmp(n);
+        for (size_t j = 0; j < n; ++j) tmp[j] = (float)soa[j];
+        if (cs_lattice_upload_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};
+    });
+}
+
+extern "C" int cs_lattice_download_f64(cs_lattice* l, double* soa)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(soa != nullptr, CS_ERR_ARG, "null data");
+        if (l->storage == CS_F64) {
+            download_pitched(l, soa, l->buf[l->cur], 8);
+            CS_HIP(hipStreamSynchronize(l->stream));
+            return;
+        }
+        const size_t n = (size_t)9 * l->w * l->h;
+        std::vector<float> tmp(n);
+        if (cs_lattice_download_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};
+        for (size_t j = 0; j < n; ++j) soa[j] = (double)tmp[j];
+    });
+}
+
+extern "C" int cs_lattice_upload_raw(cs_lattice* l, const void* data)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(data != nullptr, CS_ERR_ARG, "null data");
+        upload_pitched(l, l->buf[l->cur], data, storage_bytes(l->storage));
+        CS_HIP(hipStreamSynchronize(l->stream));
+    });
+}
+
+extern "C" int cs_lattice_download_raw(cs_lattice* l, void* data)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(data != nullptr, CS_ERR_ARG, "null data");
+        download_pitched(l, data, l->buf[l->cur], storage_bytes(l->storage));
+        CS_HIP(hipStreamSynchronize(l->stream));
+    });
+}
+
+template <class T>
+static void init_eq_impl(cs_lattice* l, const T* rho, const T* ux, const T* uy)
+{
+    CS_REQUIRE(rho && ux && uy, CS_ERR_ARG, "null input");
+    const int64_t n = (int64_t)l->w * l->h;
+    dispatch_storage(l->storage, [&]<class St>() {
+        using TT = typename St::T;
+        std::vector<TT> host((size_t)(3 * n));
+        for (int64_t d = 0; d < n; ++d) {
+            host[(size_t)d] = (TT)rho[d];
+            host[(size_t)(n + d)] = (TT)ux[d];
+            host[(size_t)(2 * n + d)] = (TT)uy[d];
+        }
+        DevBuf dev(sizeof(TT) * 3 * (size_t)n);
+        CS_HIP(hipMemcpyAsync(dev.p, host.data(), sizeof(TT) * 3 * (size_t)n, hipMemcpyHostToDevice, l->stream));
+        const TT* p = (const TT*)dev.p;
+        hipLaunchKernelGGL((k_init_eq<St>), cell_grid(l), dim3(256), 0, l->stream, p, p + n, p + 2 * n,
+                           (typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane, consts_of<St>(l));
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipStreamSynchronize(l->stream));
+    });
+}
+
+extern "C" int cs_lattice_init_equilibrium(cs_lattice* l, const float* rho, const float* ux, const float* uy)
+{
+    return guarded([&] {
+        check_lat(l);
+        init_eq_impl<float>(l, rho, ux, uy);
+    });
+}
+
+extern "C" int cs_lattice_init_equilibrium_f64(cs_lattice* l, const double* rho, const double* ux,
+                                               const double* uy)
+{
+    return guarded([&] {
+        check_lat(l);
+        init_eq_impl<double>(l, rho, ux, uy);
+    });
+}
+
+extern "C" int cs_lattice_step(cs_lattice* l, int n)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(n >= 0, CS_ERR_ARG, "negative step count");
+        do_steps(l, n);
+    });
+}
+
+extern "C" int cs_lattice_step_timed(cs_lattice* l, int n, float* ms)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(n >= 0 && ms, CS_ERR_ARG, "bad argument");
+        CS_HIP(hipEventRecord(l->ev0, l->stream));
+        do_steps(l, n);
+        CS_HIP(hipEventRecord(l->ev1, l->stream));
+        CS_HIP(hipEventSynchronize(l->ev1));
+        CS_HIP(hipEventElapsedTime(ms, l->ev0, l->ev1));
+    });
+}
+
+static void require_unfused(const cs_lattice* l)
+{
+    CS_REQUIRE(l->storage == CS_F32 || l->storage == CS_F64, CS_ERR_UNSUPPORTED,
+               "stream/bounce_back/collide alone need F32 or F64 storage");
+    CS_REQUIRE(l->params.boundary == CS_BB_REFERENCE, CS_ERR_UNSUPPORTED,
+               "stream/bounce_back/collide alone are defined for the reference boundary mode");
+}
+
+extern "C" int cs_lattice_stream(cs_lattice* l)
+{
+    return guarded([&] {
+        check_lat(l);
+        require_unfused(l);
+        dispatch_storage(l->storage, [&]<class St>() {
+            if constexpr (std::is_same_v<St, StF32> || std::is_same_v<St, StF64>) {
+                hipLaunchKernelGGL((k_stream<St>), cell_grid(l), dim3(256), 0, l->stream,
+                                   (const typename St::S*)l->buf[l->cur], (typename St::S*)l->buf[l->cur ^ 1], l->w,
+                                   l->h, l->pitch, l->plane, (int)l->params.edge);
+            }
+        });
+        CS_HIP(hipGetLastError());
+        l->cur ^= 1;
+    });
+}
+
+extern "C" int cs_lattice_bounce_back(cs_lattice* l)
+{
+    return guarded([&] {
+        check_lat(l);
+        require_unfused(l);
+        if (!l->has_flags) return;
+        dispatch_storage(l->storage, [&]<class St>() {
+            if constexpr (std::is_same_v<St, StF32> || std::is_same_v<St, StF64>) {
+                hipLaunchKernelGGL((k_bounce_back<St>), cell_grid(l), dim3(256), 0, l->stream,
+                                   (typename St::S*)l->buf[l->cur], (const uint8_t*)l->flags, l->w, l->h, l->pitch,
+                                   l->plane);
+            }
+        });
+        CS_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int cs_lattice_collide(cs_lattice* l)
+{
+    return guarded([&] {
+        check_lat(l);
+        require_unfused(l);
+        dispatch_storage(l->storage, [&]<class St>() {
+            if constexpr (std::is_same_v<St, StF32> || std::is_same_v<St, StF64>) {
+                hipLaunchKernelGGL((k_collide<St>), cell_grid(l), dim3(256), 0, l->stream,
+                                   (typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane, consts_of<St>(l));
+            }
+        });
+        CS_HIP(hipGetLastError());
+        // State::step advances time after collide (lbm.rs:713); the phases
+        // alone leave it to the caller, as the reference's methods do.
+    });
+}
+
+extern "C" int cs_lattice_time(const cs_lattice* l, double* t)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(t != nullptr, CS_ERR_ARG, "null output");
+        *t = l->time;
+    });
+}
+
+extern "C" int cs_lattice_sync(cs_lattice* l)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_HIP(hipStreamSynchronize(l->stream));
+    });
+}
+
+extern "C" int cs_lattice_device_ptr(cs_lattice* l, void** f, int* pitch, int64_t* plane)
+{
+    return guarded([&] {
+        check_lat(l);
+        if (f) *f = l->buf[l->cur];
+        if (pitch) *pitch = l->pitch;
+        if (plane) *plane = l->plane;
+    });
+}
+
+extern "C" int cs_lattice_stream_handle(cs_lattice* l, void** stream)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(stream != nullptr, CS_ERR_ARG, "null output");
+        *stream = (void*)l->stream;
+    });
+}
+
+extern "C" int cs_lattice_density(cs_lattice* l, float* rho)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(rho, CS_ERR_ARG, "null output");
+        float* o[] = {rho};
+        moments_f32(l, M_RHO, o);
+    });
+}
+
+extern "C" int cs_lattice_momentum_density(cs_lattice* l, float* mx, float* my)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(mx && my, CS_ERR_ARG, "null output");
+        float* o[] = {mx, my};
+        moments_f32(l, M_MX | M_MY, o);
+    });
+}
+
+extern "C" int cs_lattice_velocity(cs_lattice* l, float* ux, float* uy)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(ux && uy, CS_ERR_ARG, "null output");
+        float* o[] = {ux, uy};
+        moments_f32(l, M_UX | M_UY, o);
+    });
+}
+
+extern "C" int cs_lattice_speed(cs_lattice* l, float* speed)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(speed, CS_ERR_ARG, "null output");
+        float* o[] = {speed};
+        moments_f32(l, M_SPEED, o);
+    });
+}
+
+extern "C" int cs_lattice_pressure(cs_lattice* l, float* p)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(p, CS_ERR_ARG, "null output");
+        float* o[] = {p};
+        moments_f32(l, M_P, o);
+    });
+}
+
+static void equilibrium_out(cs_lattice* l, int non_eq, float* out)
+{
+    CS_REQUIRE(out, CS_ERR_ARG, "null output");
+    const int64_t n = (int64_t)l->w * l->h;
+    DevBuf dev(sizeof(double) * 9 * (size_t)n);
+    dispatch_storage(l->storage, [&]<class St>() {
+        hipLaunchKernelGGL((k_equilibrium<St>), cell_grid(l), dim3(256), 0, l->stream,
+                           (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane, consts_of<St>(l),
+                           non_eq, (double*)dev.p);
+    });
+    CS_HIP(hipGetLastError());
+    std::vector<double> tmp((size_t)(9 * n));
+    CS_HIP(hipMemcpyAsync(tmp.data(), dev.p, sizeof(double) * 9 * (size_t)n, hipMemcpyDeviceToHost, l->stream));
+    CS_HIP(hipStreamSynchronize(l->stream));
+    for (size_t j = 0; j < tmp.size(); ++j) out[j] = (float)tmp[j];
+}
+
+extern "C" int cs_lattice_equilibrium(cs_lattice* l, float* feq)
+{
+    return guarded([&] {
+        check_lat(l);
+        equilibrium_out(l, 0, feq);
+    });
+}
+
+extern "C" int cs_lattice_non_equilibrium(cs_lattice* l, float* fneq)
+{
+    return guarded([&] {
+        check_lat(l);
+        equilibrium_out(l, 1, fneq);
+    });
+}
+
+extern "C" int cs_lattice_is_unstable(cs_lattice* l, int* unstable)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(unstable, CS_ERR_ARG, "null output");
+        CS_HIP(hipMemsetAsync(l->iscratch, 0, sizeof(int), l->stream));
+        dispatch_storage(l->storage, [&]<class St>() {
+            hipLaunchKernelGGL((k_unstable<St>), cell_grid(l), dim3(256), 0, l->stream,
+                               (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
+                               consts_of<St>(l), l->iscratch);
+        });
+        CS_HIP(hipGetLastError());
+        int v = 0;
+        CS_HIP(hipMemcpyAsync(&v, l->iscratch, sizeof(int), hipMemcpyDeviceToHost, l->stream));
+        CS_HIP(hipStreamSynchronize(l->stream));
+        *unstable = v != 0;
+    });
+}
+
+extern "C" int cs_lattice_moments_f64(cs_lattice* l, double* rho, double* ux, double* uy)
+{
+    return guarded([&] {
+        check_lat(l);
+        const int64_t n = (int64_t)l->w * l->h;
+        std::vector<double> tmp((size_t)(3 * n));
+        moments_to_host(l, M_RHO | M_UX | M_UY, tmp.data());
+        if (rho) std::memcpy(rho, tmp.data(), sizeof(double) * n);
+        if (ux) std::memcpy(ux, tmp.data() + n, sizeof(double) * n);
+        if (uy) std::memcpy(uy, tmp.data() + 2 * n, sizeof(double) * n);
+    });
+}
+
+extern "C" int cs_lattice_sums_f64(cs_lattice* l, double out[3])
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(out, CS_ERR_ARG, "null output");
+        dispatch_storage(l->storage, [&]<class St>() {
+            hipLaunchKernelGGL((k_sums<St>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, l->stream,
+                               (const typename St::S*)l->buf[l->cur], l->has_flags ? l->flags : nullptr, l->w, l->h,
+                               l->pitch, l->plane, consts_of<St>(l), l->partials);
+        });
+        CS_HIP(hipGetLastError());
+        reduce(l, 3, out);
+    });
+}
+
+extern "C" int cs_lattice_snapshot_velocity(cs_lattice* l)
+{
+    return guarded([&] {
+        check_lat(l);
+        if (!l->snap) CS_HIP(hipMalloc(&l->snap, sizeof(double) * 2 * (size_t)l->w * l->h));
+        snapshot(l);
+        CS_HIP(hipStreamSynchronize(l->stream));
+    });
+}
+
+extern "C" int cs_lattice_residual_f64(cs_lattice* l, double* residual)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(residual, CS_ERR_ARG, "null output");
+        *residual = residual_now(l);
+    });
+}
+
+extern "C" int cs_lattice_run_to_residual(cs_lattice* l, double tol, int interval, int64_t max_steps,
+                                          int64_t* steps, double* residual)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(interval >= 1 && max_steps >= 0 && steps && residual, CS_ERR_ARG, "bad argument");
+        if (!l->snap) CS_HIP(hipMalloc(&l->snap, sizeof(double) * 2 * (size_t)l->w * l->h));
+        int64_t done = 0;
+        double res = INFINITY;
+        while (done < max_steps) {
+            snapshot(l);
+            const int n = (int)std::min<int64_t>(interval, max_steps - done);
+            do_steps(l, n);
+            done += n;
+            res = residual_now(l);
+            if (!(res == res)) break;  // NaN: diverged
+            if (res <= tol) break;
+        }
+        *steps = done;
+        *residual = res;
+    });
+}
+
+extern "C" int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy)
+{
+    return guarded([&] {
+        CS_REQUIRE(ctx && ms_per_copy && bytes >= 16 && iters >= 1, CS_ERR_ARG, "bad argument");
+        CS_HIP(hipSetDevice(ctx->device));
+        const int64_t n4 = bytes / 16;
+        DevBuf a((size_t)n4 * 16), b((size_t)n4 * 16);
+        CS_HIP(hipMemset(a.p, 0, (size_t)n4 * 16));
+        hipStream_t s;
+        CS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        CS_HIP(hipEventCreate(&e0));
+        CS_HIP(hipEventCreate(&e1));
+        const int blocks = ctx->prop.multiProcessorCount * 16;
+        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const float4*)a.p, (float4*)b.p, n4);
+        CS_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < iters; ++i)
+            hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const float4*)a.p, (float4*)b.p, n4);
+        CS_HIP(hipEventRecord(e1, s));
+        CS_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_copy = ms / iters;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
+// Launch-shape override used by the autotuner: cells per thread of the step
+// kernel (1, 2 or 4; 0 restores the default).
+extern "C" int cs_set_cells_per_thread(int cpt)
+{
+    return guarded([&] {
+        CS_REQUIRE(cpt == 0 || cpt == 1 || cpt == 2 || cpt == 4, CS_ERR_ARG, "cells per thread must be 0, 1, 2 or 4");
+        g_cpt_override = cpt;
+    });
+}

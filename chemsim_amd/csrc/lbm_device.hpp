@@ -1,0 +1,274 @@
+// Device code of the D2Q9 BGK hot path for gfx950: storage traits, the cell
+// update and the fused pull-step kernel.  Shared by the single-lattice
+// (lattice.hip) and batched (batch.hip) launches.
+//
+// One step of the reference (State::step, lbm.rs:694-714) is stream
+// (lbm.rs:716-729), bounce-back (lbm.rs:741-751) and BGK collide over every
+// cell (lbm.rs:731-739).  The stored state between steps is post-collision,
+// so a pull kernel that gathers each cell's nine incoming populations,
+// applies the boundary rule, collides and stores reproduces one reference
+// step per launch.  Streaming is the rotated shift of the reference: the
+// population i of cell (r, c) comes from (r + cx_i, c - cy_i).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cs_internal.hpp"
+
+namespace cs {
+
+// ----------------------------------------------------------- storage types --
+__device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Round to nearest even; NaN stays a quiet NaN.
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float f16_bits_to_f32(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
+// S: stored element type.  T: arithmetic type.  ld() widens a stored value of
+// the plane whose base is `base` to a full population; st() narrows.
+struct StF32 {
+    using S = float;
+    using T = float;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o, T) { return p[o]; }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v, T) { __builtin_nontemporal_store(v, p + o); }
+};
+struct StF64 {
+    using S = double;
+    using T = double;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o, T) { return p[o]; }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v, T) { __builtin_nontemporal_store(v, p + o); }
+};
+struct StBF16 {
+    using S = uint16_t;
+    using T = float;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o, T base) { return bf16_bits_to_f32(p[o]) + base; }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v, T base)
+    {
+        __builtin_nontemporal_store(f32_to_bf16_bits(v - base), p + o);
+    }
+};
+struct StF16 {
+    using S = uint16_t;
+    using T = float;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o, T base) { return f16_bits_to_f32(p[o]) + base; }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v, T base)
+    {
+        __builtin_nontemporal_store(f32_to_f16_bits(v - base), p + o);
+    }
+};
+
+// Boundary handling variants of the step kernel.
+enum StepMode : int {
+    MODE_FLUID = 0,  // no flags: every cell fluid
+    MODE_REF = 1,    // flags, reference on-node swap at solid cells, collide all
+    MODE_HALF = 2    // flags, half-way links, lid, Zou-He, solid carried
+};
+
+template <class T>
+struct StepArgs {
+    const void* fin;
+    void* fout;
+    const uint32_t* node;
+    int w, h, pitch;
+    int tiles_x;
+    int64_t plane;
+    Consts<T> k;
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD one
+// contiguous range of tiles so that the halo rows a tile shares with its
+// neighbours stay in the same L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks)
+{
+    const int xcd = bid & 7, k = bid >> 3;
+    const int per = nblocks >> 3, rem = nblocks & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+// ------------------------------------------------------------ cell update --
+// Moments, equilibrium and BGK (lbm.rs:117-149, 43-71, 350-364) with the
+// reference's operation order; terms that multiply by a zero lattice
+// component are dropped, which leaves every value unchanged.
+template <class T>
+__device__ __forceinline__ void bgk_collide(const Consts<T>& k, T (&B)[9])
+{
+#pragma clang fp contract(off)
+    const T rho = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
+    const T mx = ((((B[1] - B[3]) + B[5]) - B[6]) - B[7]) + B[8];
+    const T my = ((((B[2] - B[4]) + B[5]) + B[6]) - B[7]) - B[8];
+    const T inv = T(1) / rho;
+    const T ux = inv * mx, uy = inv * my;
+    const T v2 = ux * ux + uy * uy;
+    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T s = ((T(1) + vc[i] * k.A) + (vc[i] * vc[i]) * k.B) + v2 * k.C;
+        const T feq = (rho * k.w[i]) * s;
+        B[i] = B[i] + (B[i] - feq) * k.K;
+    }
+}
+
+// Zou-He velocity inlet (low-column edge; unknown populations 2, 5, 6).
+template <class T>
+__device__ __forceinline__ void zou_he_inlet(T (&B)[9], T ux, T uy)
+{
+#pragma clang fp contract(off)
+    const T rho = (((B[0] + B[1]) + B[3]) + T(2) * ((B[4] + B[7]) + B[8])) / (T(1) - uy);
+    const T ru_t = rho * ux, ru_a = rho * uy;
+    const T d13 = B[1] - B[3];
+    const T two3 = T(2) / T(3), six1 = T(1) / T(6), half = T(0.5);
+    B[2] = B[4] + two3 * ru_a;
+    B[5] = ((B[7] - half * d13) + half * ru_t) + six1 * ru_a;
+    B[6] = ((B[8] + half * d13) - half * ru_t) + six1 * ru_a;
+}
+
+// Zou-He density outlet (high-column edge; unknown populations 4, 7, 8).
+template <class T>
+__device__ __forceinline__ void zou_he_outlet(T (&B)[9], T rho)
+{
+#pragma clang fp contract(off)
+    const T uy = (((B[0] + B[1]) + B[3]) + T(2) * ((B[2] + B[5]) + B[6])) / rho - T(1);
+    const T ru_a = rho * uy;
+    const T d13 = B[1] - B[3];
+    const T two3 = T(2) / T(3), six1 = T(1) / T(6), half = T(0.5);
+    B[4] = B[2] - two3 * ru_a;
+    B[7] = (B[5] + half * d13) - six1 * ru_a;
+    B[8] = (B[6] - half * d13) - six1 * ru_a;
+}
+
+// Gathers the nine post-stream, post-boundary populations of cell (r, c).
+// Out-of-range (r, c) lanes read a clamped in-range address and are masked by
+// the caller.  Returns the cell's own flag (0 without flags).
+template <class St, int EDGE, int MODE>
+__device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, const uint32_t* __restrict__ node,
+                                      int w, int h, int pitch, int64_t P, const Consts<typename St::T>& k, int r,
+                                      int c, typename St::T (&B)[9])
+{
+    using T = typename St::T;
+    int rows[3], cols[3];
+    bool vr[3], vc[3];
+    if (EDGE == CS_EDGE_PERIODIC) {
+        rows[0] = r == 0 ? h - 1 : r - 1;
+        rows[2] = r == h - 1 ? 0 : r + 1;
+        cols[0] = c == 0 ? w - 1 : c - 1;
+        cols[2] = c == w - 1 ? 0 : c + 1;
+        vr[0] = vr[2] = vc[0] = vc[2] = true;
+    } else {
+        vr[0] = r > 0;
+        vr[2] = r < h - 1;
+        vc[0] = c > 0;
+        vc[2] = c < w - 1;
+        rows[0] = vr[0] ? r - 1 : r;
+        rows[2] = vr[2] ? r + 1 : r;
+        cols[0] = vc[0] ? c - 1 : c;
+        cols[2] = vc[2] ? c + 1 : c;
+    }
+    rows[1] = r;
+    cols[1] = c;
+    vr[1] = vc[1] = true;
+    const int64_t own = (int64_t)r * pitch + c;
+
+    uint32_t info = 0;
+    int type = CS_FLUID;
+    if (MODE != MODE_FLUID) {
+        info = node[own];
+        type = (int)(info >> NODE_TYPE_SHIFT);
+    }
+    const bool solid = (type == CS_WALL || type == CS_LID);
+
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        // Normal pull source of direction i.
+        int64_t off = (int64_t)i * P + (int64_t)rows[CX[i] + 1] * pitch + cols[1 - CY[i]];
+        bool valid = vr[CX[i] + 1] && vc[1 - CY[i]];
+        bool add_lid = false;
+        if (MODE == MODE_REF) {
+            // Solid cell: take the opposite population after streaming
+            // (lbm.rs:745), i.e. the normal source of direction opp(i).
+            if (solid) {
+                const int j = OPP[i];
+                off = (int64_t)j * P + (int64_t)rows[CX[j] + 1] * pitch + cols[1 - CY[j]];
+                valid = vr[CX[j] + 1] && vc[1 - CY[j]];
+            }
+        } else if (MODE == MODE_HALF) {
+            if (solid) {
+                off = (int64_t)i * P + own;  // carried unchanged
+                valid = true;
+            } else if ((info >> i) & 1u) {
+                off = (int64_t)OPP[i] * P + own;  // half-way bounce-back
+                valid = true;
+                add_lid = (info >> (9 + i)) & 1u;
+            }
+        }
+        T v = St::ld(fin, off, k.base[i]);
+        v = valid ? v : T(0);
+        if (MODE == MODE_HALF) v = add_lid ? v + k.lid[i] : v;
+        B[i] = v;
+    }
+    if (MODE == MODE_HALF) {
+        if (type == CS_INLET) zou_he_inlet<T>(B, k.in_ux, k.in_uy);
+        else if (type == CS_OUTLET) zou_he_outlet<T>(B, k.out_rho);
+    }
+    return type;
+}
+
+// Collides and stores cell (r, c).  Solid cells of the half-way mode are
+// carried unchanged: their stored values are copied bit for bit.
+template <class St, int MODE>
+__device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, typename St::S* __restrict__ fout,
+                                       int pitch, int64_t P, const Consts<typename St::T>& k, int r, int c,
+                                       int type, typename St::T (&B)[9])
+{
+    const int64_t own = (int64_t)r * pitch + c;
+    if (MODE == MODE_HALF && (type == CS_WALL || type == CS_LID)) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) fout[(int64_t)i * P + own] = fin[(int64_t)i * P + own];
+        return;
+    }
+    bgk_collide(k, B);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) St::st(fout, (int64_t)i * P + own, B[i], k.base[i]);
+}
+
+// Fused step.  A block is 4 waves; wave y handles row 4*ty + y, lane l the
+// columns 64*CPT*tx + l + 64*j, j < CPT.  All loads of a thread's CPT cells
+// are issued before any arithmetic.
+template <class St, int EDGE, int MODE, int CPT>
+__global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
+{
+    using S = typename St::S;
+    using T = typename St::T;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = tile / a.tiles_x;
+    const int tx = tile - ty * a.tiles_x;
+    const int r = ty * 4 + (threadIdx.x >> 6);
+    if (r >= a.h) return;
+    const int c0 = tx * (64 * CPT) + (threadIdx.x & 63);
+    const S* __restrict__ fin = (const S*)a.fin;
+    S* __restrict__ fout = (S*)a.fout;
+    T B[CPT][9];
+    int type[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = c0 + 64 * j;
+        const int cc = c < a.w ? c : a.w - 1;
+        type[j] = gather<St, EDGE, MODE>(fin, a.node, a.w, a.h, a.pitch, a.plane, a.k, r, cc, B[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int c = c0 + 64 * j;
+        if (c < a.w) finish<St, MODE>(fin, fout, a.pitch, a.plane, a.k, r, c, type[j], B[j]);
+    }
+}
+
+}  // namespace cs

@@ -1,0 +1,188 @@
+/*
+ * chemsim.h -- C ABI of libchemsim_hip.so, the MI355X (gfx950) implementation
+ * of taktoa/chemsim's D2Q9 lattice-Boltzmann hot path (src/lbm.rs).
+ *
+ * In the reference, the path sits behind the Rust API of src/lbm.rs
+ * (Lattice trait :111-176, CollisionOperator plugin trait :327-341, State
+ * :670-819) and reaches the GPU through the arrayfire crate's C API
+ * (af_convolve2 at lbm.rs:722, af_replace at :745, JIT element-wise arithmetic
+ * through src/matrix.rs, af_imin_all at :817, af_get_data_ptr via
+ * Matrix::get_underlying at matrix.rs:120-126).  This header is what a thin
+ * FFI crate binds instead; each entry point names the reference interface it
+ * replaces.
+ *
+ * Conventions
+ *  - Every function returns int status (CS_OK = 0).  On failure
+ *    cs_last_error() returns a thread-local message (the arrayfire crate
+ *    panics on a non-zero af_err; the Rust binding does the same on a non-zero
+ *    status).
+ *  - Host arrays are in the reference host layout (matrix.rs:24-30,120-126):
+ *    one plane is h*w values, row-major, index r*w + c; a population set is 9
+ *    such planes in direction order 0..8 (lbm.rs:209-231).
+ *  - Host pointers are borrowed only for the call.  Upload/download/moment
+ *    calls are synchronous; cs_lattice_step is asynchronous on the lattice's
+ *    own HIP stream.  Calls on one handle are not re-entrant; different
+ *    handles may be used from different threads.
+ *  - Handles own their device memory; *_destroy frees it.
+ */
+#ifndef CHEMSIM_H
+#define CHEMSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_ABI_VERSION 1
+
+typedef struct cs_ctx cs_ctx;         /* one GPU (af::init + set_backend, main.rs:331-332) */
+typedef struct cs_lattice cs_lattice; /* State<D2Q9> with BGK (lbm.rs:670-676) */
+typedef struct cs_batch cs_batch;     /* many independent lattices, one launch (extension) */
+
+enum cs_status {
+    CS_OK = 0,
+    CS_ERR_ARG = 1,         /* bad argument (null, size, enum) */
+    CS_ERR_HIP = 2,         /* HIP runtime error */
+    CS_ERR_UNSUPPORTED = 3, /* operation not defined for this storage/mode */
+    CS_ERR_STATE = 4,       /* call out of order (e.g. residual before snapshot) */
+    CS_ERR_NOMEM = 5
+};
+
+/* Storage precision of the populations. */
+enum cs_storage {
+    CS_F32 = 0,     /* Scalar = f32, lbm.rs:13 */
+    CS_F64 = 1,     /* FP64 oracle run (extension) */
+    CS_BF16DEV = 2, /* bf16 deviations f_i - w_i*rho0, FP32 arithmetic (extension) */
+    CS_F16DEV = 3   /* fp16 deviations f_i - w_i*rho0, FP32 arithmetic (extension) */
+};
+
+/* Treatment of the lattice edge.  ZERO is the reference: af::convolve2 with
+ * ConvMode::DEFAULT pads with zeros (lbm.rs:722-724). */
+enum cs_edge { CS_EDGE_ZERO = 0, CS_EDGE_PERIODIC = 1 };
+
+/* REFERENCE: after streaming, solid cells take the opposite population
+ * (State::bounce_back, lbm.rs:741-751) and every cell is then collided
+ * (State::collide, lbm.rs:731-739).  HALFWAY (extension): links into solid
+ * cells reflect half-way, moving lids add the Ladd term, inlet/outlet cells
+ * apply Zou-He, solid cells are carried unchanged. */
+enum cs_boundary { CS_BB_REFERENCE = 0, CS_BB_HALFWAY = 1 };
+
+/* Node flags, one byte per cell.  In the REFERENCE mode the solid mask
+ * (Geometry = af::Array<bool>, lbm.rs:99) is flag == WALL || flag == LID. */
+enum cs_flag { CS_FLUID = 0, CS_WALL = 1, CS_LID = 2, CS_INLET = 3, CS_OUTLET = 4 };
+
+typedef struct cs_params {
+    double tau;        /* BGK { tau } (lbm.rs:345-347) */
+    double delta_x;    /* Discretization { delta_x, delta_t } (lbm.rs:75-79) */
+    double delta_t;
+    int32_t edge;      /* enum cs_edge */
+    int32_t boundary;  /* enum cs_boundary */
+    double lid_ux;     /* moving-lid velocity, lattice components (extension) */
+    double lid_uy;
+    double inlet_ux;   /* Zou-He inlet velocity (extension) */
+    double inlet_uy;
+    double outlet_rho; /* Zou-He outlet density (extension) */
+    double rho0;       /* base density of 16-bit deviation storage (extension) */
+} cs_params;
+
+/* ---- errors, context -------------------------------------------------- */
+const char* cs_last_error(void);
+int cs_abi_version(void);
+/* af::init + af::set_backend (main.rs:331-332): bind one HIP device. */
+int cs_ctx_create(int device, cs_ctx** out);
+int cs_ctx_destroy(cs_ctx* ctx);
+/* Name of the bound device (af::device_info, main.rs:340). */
+int cs_ctx_device_name(cs_ctx* ctx, char* buf, size_t len);
+
+/* ---- lattice ------------------------------------------------------------ */
+/* D2Q9::new + State::initial (lbm.rs:187-200, 679-692): w columns, h rows.
+ * Unlike the reference (matrix.rs:40-44 vs 24-30) non-square sizes work. */
+int cs_lattice_create(cs_ctx* ctx, int w, int h, int storage, cs_lattice** out);
+int cs_lattice_destroy(cs_lattice* lat);
+int cs_lattice_shape(const cs_lattice* lat, int* w, int* h, int* storage);
+/* Collision operator + Discretization + boundary choices (lbm.rs:670-676). */
+int cs_lattice_set_params(cs_lattice* lat, const cs_params* params);
+/* State::geometry (lbm.rs:673; built at main.rs:269-312, edited at main.rs:77-89).
+ * flags: h*w bytes of enum cs_flag, or NULL for all fluid.  Allowed between
+ * any two steps. */
+int cs_lattice_set_flags(cs_lattice* lat, const uint8_t* flags);
+/* Population upload/download in the reference host layout
+ * (Matrix::new / Matrix::get_underlying, matrix.rs:24-30,120-126).
+ * The f32 pair works for every storage (16-bit storage converts); the f64
+ * pair keeps full precision of an F64 lattice. */
+int cs_lattice_upload_f32(cs_lattice* lat, const float* soa);
+int cs_lattice_download_f32(cs_lattice* lat, float* soa);
+int cs_lattice_upload_f64(cs_lattice* lat, const double* soa);
+int cs_lattice_download_f64(cs_lattice* lat, double* soa);
+/* Raw stored values (9*h*w elements of the storage type: float, double or the
+ * 16-bit deviation bit patterns). */
+int cs_lattice_upload_raw(cs_lattice* lat, const void* data);
+int cs_lattice_download_raw(cs_lattice* lat, void* data);
+/* f = compute_equilibrium(rho, (ux, uy)) (lbm.rs:43-71), as main.rs:257-267
+ * initialises a lattice.  Inputs are h*w planes. */
+int cs_lattice_init_equilibrium(cs_lattice* lat, const float* rho, const float* ux, const float* uy);
+int cs_lattice_init_equilibrium_f64(cs_lattice* lat, const double* rho, const double* ux,
+                                    const double* uy);
+
+/* ---- the step ------------------------------------------------------------ */
+/* n x State::step (lbm.rs:694-714): stream, bounce-back and BGK collide fused
+ * into one pull kernel per step.  Asynchronous. */
+int cs_lattice_step(cs_lattice* lat, int n);
+/* As cs_lattice_step, bracketed by HIP events on the lattice's stream;
+ * synchronises and returns the elapsed milliseconds. */
+int cs_lattice_step_timed(cs_lattice* lat, int n, float* ms);
+/* The three phases alone (State::stream :716-729, State::bounce_back
+ * :741-751, State::collide :731-739), REFERENCE mode, F32/F64 storage.
+ * stream + bounce_back + collide equals one cs_lattice_step. */
+int cs_lattice_stream(cs_lattice* lat);
+int cs_lattice_bounce_back(cs_lattice* lat);
+int cs_lattice_collide(cs_lattice* lat);
+/* State::time (lbm.rs:671, advanced at :713). */
+int cs_lattice_time(const cs_lattice* lat, double* t);
+int cs_lattice_sync(cs_lattice* lat);
+/* Device address of the current populations: 9 planes of h rows with a row
+ * pitch of *pitch elements, plane stride *plane elements. */
+int cs_lattice_device_ptr(cs_lattice* lat, void** f, int* pitch, int64_t* plane);
+/* HIP stream the lattice's kernels run on (hipStream_t). */
+int cs_lattice_stream_handle(cs_lattice* lat, void** stream);
+
+/* ---- moments (render-facing API) --------------------------------------- */
+int cs_lattice_density(cs_lattice* lat, float* rho);                /* lbm.rs:117-121, 779-781 */
+int cs_lattice_momentum_density(cs_lattice* lat, float* mx, float* my); /* :123-131, 789-792 */
+int cs_lattice_velocity(cs_lattice* lat, float* ux, float* uy);     /* :133-149, 794-797 */
+int cs_lattice_speed(cs_lattice* lat, float* speed);                /* :151-154, 799-802 */
+int cs_lattice_pressure(cs_lattice* lat, float* p);                 /* :784-787 */
+int cs_lattice_equilibrium(cs_lattice* lat, float* feq);            /* :156-160, 804-807 */
+int cs_lattice_non_equilibrium(cs_lattice* lat, float* fneq);       /* :162-173, 809-812 */
+int cs_lattice_is_unstable(cs_lattice* lat, int* unstable);         /* :814-818 */
+/* Moments in the working precision, widened to f64 (any null output is skipped). */
+int cs_lattice_moments_f64(cs_lattice* lat, double* rho, double* ux, double* uy);
+
+/* ---- FP64 reductions (extension) -------------------------------------- */
+/* out = {sum rho, sum m_x, sum m_y} over non-solid cells, f64 accumulation. */
+int cs_lattice_sums_f64(cs_lattice* lat, double out[3]);
+/* Keep the current velocity field; cs_lattice_residual_f64 then returns
+ * ||u - u_snapshot||_2 / ||u||_2 over non-solid cells in f64. */
+int cs_lattice_snapshot_velocity(cs_lattice* lat);
+int cs_lattice_residual_f64(cs_lattice* lat, double* residual);
+/* Step until the residual taken every `interval` steps is <= tol, or
+ * max_steps.  Returns the steps taken and the last residual. */
+int cs_lattice_run_to_residual(cs_lattice* lat, double tol, int interval, int64_t max_steps,
+                               int64_t* steps, double* residual);
+
+/* ---- measurement helpers ---------------------------------------------- */
+/* Plain device-to-device copy kernel of `bytes` bytes, `iters` times; returns
+ * the average milliseconds per copy (HIP events).  Calibrates bandwidth and
+ * the HBM byte counters. */
+int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy);
+/* Launch shape of the step kernel: cells per thread (1, 2 or 4; 0 = the
+ * built-in default).  Process-wide; used by the autotuner.  No reference
+ * counterpart (ArrayFire picks its own launch shapes). */
+int cs_set_cells_per_thread(int cells_per_thread);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,277 @@
+/*
+ * lbm_oracle.c -- CPU restatement of taktoa/chemsim's D2Q9 BGK hot path
+ * (src/lbm.rs), used as the parity checker.  TEST INFRASTRUCTURE ONLY: the
+ * product library never links it.  Build: oracle/Makefile (-ffp-contract=off).
+ */
+#include "lbm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* D2Q9::directions, lbm.rs:221-231: c_i = (cx, cy). */
+static const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+static const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+/* D2Q9::swap_populations, lbm.rs:298-309. */
+static const int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+/* Exact weights of the extension terms (Ladd moving wall). */
+static const double W_EXACT[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9,
+                                  1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+
+static inline int is_solid(int t) { return t == ORC_WALL || t == ORC_LID; }
+
+/* Constants in f32 exactly as lbm.rs evaluates them in Scalar = f32. */
+void orc_consts_f32_make(const orc_params* p, orc_consts_f32* k)
+{
+    volatile float dx = (float)p->delta_x, dt = (float)p->delta_t, tau = (float)p->tau;
+    volatile float three = 3.0f;
+    float cs = dx / (sqrtf(three) * dt);           /* lbm.rs:84 */
+    float cs2 = cs * cs;                           /* lbm.rs:55 */
+    float cs4 = cs2 * cs2;                         /* lbm.rs:56 */
+    k->cs2 = cs2;
+    k->A = 1.0f / cs2;                             /* lbm.rs:64 */
+    k->B = 1.0f / (2.0f * cs4);                    /* lbm.rs:65 */
+    k->C = -1.0f / (2.0f * cs2);                   /* lbm.rs:66 */
+    k->K = -dt / tau;                              /* lbm.rs:357 */
+    volatile float w0n = 16.0f, w1n = 4.0f, w5n = 1.0f, d = 36.0f; /* lbm.rs:209-219 */
+    float w0 = w0n / d, w1 = w1n / d, w5 = w5n / d;
+    const float ws[9] = {w0, w1, w1, w1, w1, w5, w5, w5, w5};
+    for (int i = 0; i < 9; ++i) {
+        k->w[i] = ws[i];
+        k->lid[i] = (float)(6.0 * W_EXACT[i] * (CX[i] * p->lid_ux + CY[i] * p->lid_uy));
+        k->base[i] = ws[i] * (float)p->rho0;
+    }
+}
+
+void orc_consts_f64_make(const orc_params* p, orc_consts_f64* k)
+{
+    volatile double dx = p->delta_x, dt = p->delta_t, tau = p->tau, three = 3.0;
+    double cs = dx / (sqrt(three) * dt);
+    double cs2 = cs * cs;
+    double cs4 = cs2 * cs2;
+    k->cs2 = cs2;
+    k->A = 1.0 / cs2;
+    k->B = 1.0 / (2.0 * cs4);
+    k->C = -1.0 / (2.0 * cs2);
+    k->K = -dt / tau;
+    volatile double w0n = 16.0, w1n = 4.0, w5n = 1.0, d = 36.0;
+    double w0 = w0n / d, w1 = w1n / d, w5 = w5n / d;
+    const double ws[9] = {w0, w1, w1, w1, w1, w5, w5, w5, w5};
+    for (int i = 0; i < 9; ++i) {
+        k->w[i] = ws[i];
+        k->lid[i] = 6.0 * W_EXACT[i] * (CX[i] * p->lid_ux + CY[i] * p->lid_uy);
+        k->base[i] = ws[i] * p->rho0;
+    }
+}
+
+static void orc_consts_make_body_f32(const orc_params* p, orc_consts_f32* k) { orc_consts_f32_make(p, k); }
+static void orc_consts_make_body_f64(const orc_params* p, orc_consts_f64* k) { orc_consts_f64_make(p, k); }
+
+#define REAL float
+#define SFX f32
+#define KT orc_consts_f32
+#include "lbm_oracle_impl.inc"
+#undef REAL
+#undef SFX
+#undef KT
+
+#define REAL double
+#define SFX f64
+#define KT orc_consts_f64
+#include "lbm_oracle_impl.inc"
+#undef REAL
+#undef SFX
+#undef KT
+
+/* ---------------------------------------------------------------- runs --- */
+
+int orc_run_f32_omp(const orc_params* p, const uint8_t* flags, float* f, int nsteps, int threads)
+{
+    const long n = 9L * p->w * p->h;
+    float* a = f;
+    float* b = (float*)malloc(sizeof(float) * n);
+    if (!b) return -1;
+    int used = 1;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+    used = threads;
+#else
+    (void)threads;
+#endif
+    for (int s = 0; s < nsteps; ++s) {
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+        for (int r = 0; r < p->h; ++r) orc_step_rows_f32(p, flags, a, b, r, r + 1);
+        float* t = a;
+        a = b;
+        b = t;
+    }
+    if (a != f) {
+        memcpy(f, a, sizeof(float) * n);
+        free(a);
+    } else {
+        free(b);
+    }
+    return used;
+}
+
+int orc_run_unfused_f32(const orc_params* p, const uint8_t* flags, float* f, int nsteps)
+{
+    const long n = (long)p->w * p->h;
+    float* scratch = (float*)malloc(sizeof(float) * 21 * n);
+    float* out = (float*)malloc(sizeof(float) * 9 * n);
+    if (!scratch || !out) {
+        free(scratch);
+        free(out);
+        return -1;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        orc_step_unfused_f32(p, flags, f, out, scratch);
+        memcpy(f, out, sizeof(float) * 9 * n);
+    }
+    free(scratch);
+    free(out);
+    return 0;
+}
+
+/* ------------------------------------------------------ 16-bit storage --- */
+
+static inline uint32_t f32_bits(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+static inline float bits_f32(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
+
+uint16_t orc_f32_to_bf16(float x)
+{
+    uint32_t u = f32_bits(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u); /* quiet NaN */
+    u += 0x7fffu + ((u >> 16) & 1u);                                          /* nearest even */
+    return (uint16_t)(u >> 16);
+}
+
+float orc_bf16_to_f32(uint16_t b) { return bits_f32((uint32_t)b << 16); }
+
+/* IEEE binary16, round to nearest even, subnormals kept, overflow to inf. */
+uint16_t orc_f32_to_f16(float x)
+{
+    uint32_t u = f32_bits(x);
+    uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+    uint32_t a = u & 0x7fffffffu;
+    if (a > 0x7f800000u) return sign | 0x7e00u;            /* NaN */
+    if (a >= 0x477ff000u) return sign | 0x7c00u;           /* >= 65520 rounds to inf */
+    if (a >= 0x38800000u) {                                /* normal half (>= 2^-14) */
+        uint32_t mant = a & 0x7fffffu;
+        int e = (int)(a >> 23) - 127 + 15;
+        uint32_t r = ((uint32_t)e << 10) | (mant >> 13);
+        uint32_t rem = mant & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) r += 1u; /* carry may bump exponent */
+        return sign | (uint16_t)r;
+    }
+    if (a < 0x33000000u) return sign;                      /* < 2^-25 rounds to zero */
+    /* subnormal half: value = m * 2^-24 with m in [0, 1024] */
+    uint32_t mant = (a & 0x7fffffu) | 0x800000u;
+    int e = (int)(a >> 23) - 127;                          /* in [-25, -15] */
+    int shift = -e - 1;                                    /* 24-bit mantissa -> units of 2^-24 */
+    shift = shift + 0;                                     /* m = mant * 2^(e-23) / 2^-24 = mant >> (-1-e) */
+    uint32_t m = mant >> shift;
+    uint32_t rem = mant & ((1u << shift) - 1u);
+    uint32_t half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (m & 1u))) m += 1u;
+    return sign | (uint16_t)m;
+}
+
+float orc_f16_to_f32(uint16_t b)
+{
+    uint32_t sign = (uint32_t)(b & 0x8000u) << 16;
+    uint32_t e = (b >> 10) & 0x1fu, m = b & 0x3ffu;
+    if (e == 0x1fu) return bits_f32(sign | 0x7f800000u | (m << 13));
+    if (e == 0) {
+        float v = (float)m * (1.0f / 16777216.0f);        /* m * 2^-24, exact */
+        return sign ? -v : v;
+    }
+    return bits_f32(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+void orc_f32_to_bf16_n(const float* x, uint16_t* y, long n)
+{
+    for (long k = 0; k < n; ++k) y[k] = orc_f32_to_bf16(x[k]);
+}
+
+void orc_f32_to_f16_n(const float* x, uint16_t* y, long n)
+{
+    for (long k = 0; k < n; ++k) y[k] = orc_f32_to_f16(x[k]);
+}
+
+static inline float dev_load(int kind, uint16_t v) { return kind == ORC_BF16 ? orc_bf16_to_f32(v) : orc_f16_to_f32(v); }
+static inline uint16_t dev_store(int kind, float v) { return kind == ORC_BF16 ? orc_f32_to_bf16(v) : orc_f32_to_f16(v); }
+
+void orc_encode_dev16(const orc_params* p, int kind, const float* f, uint16_t* d)
+{
+    orc_consts_f32 k;
+    orc_consts_f32_make(p, &k);
+    const long n = (long)p->w * p->h;
+    for (int i = 0; i < 9; ++i)
+        for (long j = 0; j < n; ++j) d[i * n + j] = dev_store(kind, f[i * n + j] - k.base[i]);
+}
+
+void orc_decode_dev16(const orc_params* p, int kind, const uint16_t* d, float* f)
+{
+    orc_consts_f32 k;
+    orc_consts_f32_make(p, &k);
+    const long n = (long)p->w * p->h;
+    for (int i = 0; i < 9; ++i)
+        for (long j = 0; j < n; ++j) f[i * n + j] = dev_load(kind, d[i * n + j]) + k.base[i];
+}
+
+/* One 16-bit step: widen the 9 neighbours' stored deviations to FP32 full
+ * populations (decode), run the FP32 cell update, narrow (encode).  Solid
+ * cells of the half-way mode keep their stored bits. */
+void orc_step_dev16(const orc_params* p, int kind, const uint8_t* flags, const uint16_t* P,
+                    uint16_t* out)
+{
+    const long n = (long)p->w * p->h;
+    float* full = (float*)malloc(sizeof(float) * 9 * n);
+    float* o = (float*)malloc(sizeof(float) * 9 * n);
+    orc_decode_dev16(p, kind, P, full);
+    orc_step_f32(p, flags, full, o);
+    orc_encode_dev16(p, kind, o, out);
+    if (p->boundary == ORC_BB_HALFWAY && flags) {
+        for (long j = 0; j < n; ++j)
+            if (is_solid(flags[j]))
+                for (int i = 0; i < 9; ++i) out[i * n + j] = P[i * n + j];
+    }
+    free(full);
+    free(o);
+}
+
+int orc_run_dev16(const orc_params* p, int kind, const uint8_t* flags, uint16_t* d, int nsteps)
+{
+    const long n = 9L * p->w * p->h;
+    uint16_t* tmp = (uint16_t*)malloc(sizeof(uint16_t) * n);
+    if (!tmp) return -1;
+    for (int s = 0; s < nsteps; ++s) {
+        orc_step_dev16(p, kind, flags, d, tmp);
+        memcpy(d, tmp, sizeof(uint16_t) * n);
+    }
+    free(tmp);
+    return 0;
+}
+
+/* ----------------------------------------------------------- residual --- */
+
+double orc_residual(int w, int h, const uint8_t* flags, const float* ux, const float* uy,
+                    const float* ux_prev, const float* uy_prev)
+{
+    const long n = (long)w * h;
+    double num = 0, den = 0;
+    for (long k = 0; k < n; ++k) {
+        if (flags && is_solid(flags[k])) continue;
+        double dx = (double)ux[k] - (double)ux_prev[k];
+        double dy = (double)uy[k] - (double)uy_prev[k];
+        num += dx * dx + dy * dy;
+        den += (double)ux[k] * (double)ux[k] + (double)uy[k] * (double)uy[k];
+    }
+    return den > 0 ? sqrt(num / den) : (num > 0 ? INFINITY : 0.0);
+}

@@ -1,0 +1,261 @@
+"""ctypes binding of the CPU oracle (oracle/lbm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liblbm_oracle.so")
+
+EDGE_ZERO, EDGE_PERIODIC = 0, 1
+BB_REFERENCE, BB_HALFWAY = 0, 1
+FLUID, WALL, LID, INLET, OUTLET = range(5)
+BF16, F16 = 0, 1
+
+
+class Params(C.Structure):
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("edge", C.c_int), ("boundary", C.c_int),
+                ("tau", C.c_double), ("delta_x", C.c_double), ("delta_t", C.c_double),
+                ("lid_ux", C.c_double), ("lid_uy", C.c_double),
+                ("inlet_ux", C.c_double), ("inlet_uy", C.c_double),
+                ("outlet_rho", C.c_double), ("rho0", C.c_double)]
+
+
+class Consts32(C.Structure):
+    _fields_ = [("K", C.c_float), ("A", C.c_float), ("B", C.c_float), ("C", C.c_float), ("cs2", C.c_float),
+                ("w", C.c_float * 9), ("lid", C.c_float * 9), ("base", C.c_float * 9)]
+
+
+class Consts64(C.Structure):
+    _fields_ = [("K", C.c_double), ("A", C.c_double), ("B", C.c_double), ("C", C.c_double), ("cs2", C.c_double),
+                ("w", C.c_double * 9), ("lid", C.c_double * 9), ("base", C.c_double * 9)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built (make -C oracle)")
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+_P = C.c_void_p
+_PP = C.POINTER(Params)
+for name, args, res in [
+    ("orc_consts_f32_make", [_PP, C.POINTER(Consts32)], None),
+    ("orc_consts_f64_make", [_PP, C.POINTER(Consts64)], None),
+    ("orc_stream_f32", [C.c_int, C.c_int, C.c_int, _P, _P], None),
+    ("orc_stream_f64", [C.c_int, C.c_int, C.c_int, _P, _P], None),
+    ("orc_bounce_back_f32", [C.c_int, C.c_int, _P, _P, _P], None),
+    ("orc_bounce_back_f64", [C.c_int, C.c_int, _P, _P, _P], None),
+    ("orc_density_f32", [C.c_int, C.c_int, _P, _P], None),
+    ("orc_density_f64", [C.c_int, C.c_int, _P, _P], None),
+    ("orc_momentum_f32", [C.c_int, C.c_int, _P, _P, _P], None),
+    ("orc_momentum_f64", [C.c_int, C.c_int, _P, _P, _P], None),
+    ("orc_velocity_f32", [C.c_int, C.c_int, _P, _P, _P], None),
+    ("orc_velocity_f64", [C.c_int, C.c_int, _P, _P, _P], None),
+    ("orc_equilibrium_f32", [_PP, C.c_int, _P, _P, _P, _P], None),
+    ("orc_equilibrium_f64", [_PP, C.c_int, _P, _P, _P, _P], None),
+    ("orc_bgk_f32", [_PP, C.c_int, _P, _P, _P], None),
+    ("orc_bgk_f64", [_PP, C.c_int, _P, _P, _P], None),
+    ("orc_step_unfused_f32", [_PP, _P, _P, _P, _P], None),
+    ("orc_step_unfused_f64", [_PP, _P, _P, _P, _P], None),
+    ("orc_step_f32", [_PP, _P, _P, _P], None),
+    ("orc_step_f64", [_PP, _P, _P, _P], None),
+    ("orc_run_f32", [_PP, _P, _P, C.c_int], C.c_int),
+    ("orc_run_f64", [_PP, _P, _P, C.c_int], C.c_int),
+    ("orc_run_f32_omp", [_PP, _P, _P, C.c_int, C.c_int], C.c_int),
+    ("orc_run_unfused_f32", [_PP, _P, _P, C.c_int], C.c_int),
+    ("orc_f32_to_bf16_n", [_P, _P, C.c_long], None),
+    ("orc_f32_to_f16_n", [_P, _P, C.c_long], None),
+    ("orc_step_dev16", [_PP, C.c_int, _P, _P, _P], None),
+    ("orc_run_dev16", [_PP, C.c_int, _P, _P, C.c_int], C.c_int),
+    ("orc_encode_dev16", [_PP, C.c_int, _P, _P], None),
+    ("orc_decode_dev16", [_PP, C.c_int, _P, _P], None),
+    ("orc_sums_f32", [C.c_int, C.c_int, _P, _P, _P], None),
+    ("orc_sums_f64", [C.c_int, C.c_int, _P, _P, _P], None),
+    ("orc_residual", [C.c_int, C.c_int, _P, _P, _P, _P, _P], C.c_double),
+]:
+    fn = getattr(lib, name)
+    fn.argtypes = args
+    fn.restype = res
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def params(w, h, tau=1.0, edge=EDGE_ZERO, boundary=BB_REFERENCE, lid=(0.0, 0.0), inlet=(0.0, 0.0),
+           outlet_rho=1.0, rho0=1.0, delta_x=1.0, delta_t=1.0) -> Params:
+    return Params(int(w), int(h), int(edge), int(boundary), float(tau), float(delta_x), float(delta_t),
+                  float(lid[0]), float(lid[1]), float(inlet[0]), float(inlet[1]), float(outlet_rho), float(rho0))
+
+
+def consts_f32(p: Params) -> Consts32:
+    k = Consts32()
+    lib.orc_consts_f32_make(C.byref(p), C.byref(k))
+    return k
+
+
+def consts_f64(p: Params) -> Consts64:
+    k = Consts64()
+    lib.orc_consts_f64_make(C.byref(p), C.byref(k))
+    return k
+
+
+def _flags(flags, p):
+    if flags is None:
+        return None
+    f = np.ascontiguousarray(flags, dtype=np.uint8)
+    assert f.shape == (p.h, p.w)
+    return f
+
+
+def _real(a):
+    return np.float64 if a.dtype == np.float64 else np.float32
+
+
+def _sfx(dt):
+    return "f64" if dt == np.float64 else "f32"
+
+
+def stream(f, edge=EDGE_ZERO):
+    f = np.ascontiguousarray(f)
+    out = np.empty_like(f)
+    getattr(lib, "orc_stream_" + _sfx(f.dtype))(f.shape[2], f.shape[1], edge, _p(f), _p(out))
+    return out
+
+
+def bounce_back(f, flags):
+    f = np.ascontiguousarray(f)
+    fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+    out = np.empty_like(f)
+    getattr(lib, "orc_bounce_back_" + _sfx(f.dtype))(f.shape[2], f.shape[1], _p(fl), _p(f), _p(out))
+    return out
+
+
+def density(f):
+    f = np.ascontiguousarray(f)
+    out = np.empty(f.shape[1:], f.dtype)
+    getattr(lib, "orc_density_" + _sfx(f.dtype))(f.shape[2], f.shape[1], _p(f), _p(out))
+    return out
+
+
+def momentum(f):
+    f = np.ascontiguousarray(f)
+    mx, my = np.empty(f.shape[1:], f.dtype), np.empty(f.shape[1:], f.dtype)
+    getattr(lib, "orc_momentum_" + _sfx(f.dtype))(f.shape[2], f.shape[1], _p(f), _p(mx), _p(my))
+    return mx, my
+
+
+def velocity(f):
+    f = np.ascontiguousarray(f)
+    ux, uy = np.empty(f.shape[1:], f.dtype), np.empty(f.shape[1:], f.dtype)
+    getattr(lib, "orc_velocity_" + _sfx(f.dtype))(f.shape[2], f.shape[1], _p(f), _p(ux), _p(uy))
+    return ux, uy
+
+
+def equilibrium(p: Params, rho, ux, uy):
+    dt = _real(np.asarray(rho))
+    rho, ux, uy = (np.ascontiguousarray(a, dtype=dt) for a in (rho, ux, uy))
+    out = np.empty((9,) + rho.shape, dt)
+    getattr(lib, "orc_equilibrium_" + _sfx(dt))(C.byref(p), int(rho.size), _p(rho), _p(ux), _p(uy), _p(out))
+    return out
+
+
+def bgk(p: Params, f, feq):
+    f = np.ascontiguousarray(f)
+    feq = np.ascontiguousarray(feq, dtype=f.dtype)
+    out = np.empty_like(f)
+    getattr(lib, "orc_bgk_" + _sfx(f.dtype))(C.byref(p), int(f[0].size), _p(f), _p(feq), _p(out))
+    return out
+
+
+def step_unfused(p: Params, f, flags=None):
+    f = np.ascontiguousarray(f)
+    out = np.empty_like(f)
+    scratch = np.empty(21 * f[0].size, f.dtype)
+    getattr(lib, "orc_step_unfused_" + _sfx(f.dtype))(C.byref(p), _p(_flags(flags, p)), _p(f), _p(out),
+                                                       _p(scratch))
+    return out
+
+
+def step(p: Params, f, flags=None):
+    f = np.ascontiguousarray(f)
+    out = np.empty_like(f)
+    getattr(lib, "orc_step_" + _sfx(f.dtype))(C.byref(p), _p(_flags(flags, p)), _p(f), _p(out))
+    return out
+
+
+def run(p: Params, f, nsteps, flags=None):
+    f = np.array(f, copy=True, order="C")
+    rc = getattr(lib, "orc_run_" + _sfx(f.dtype))(C.byref(p), _p(_flags(flags, p)), _p(f), int(nsteps))
+    assert rc == 0
+    return f
+
+
+def run_omp(p: Params, f, nsteps, flags=None, threads=0):
+    f = np.array(f, copy=True, order="C", dtype=np.float32)
+    used = lib.orc_run_f32_omp(C.byref(p), _p(_flags(flags, p)), _p(f), int(nsteps), int(threads))
+    assert used > 0
+    return f, used
+
+
+def run_unfused(p: Params, f, nsteps, flags=None):
+    f = np.array(f, copy=True, order="C", dtype=np.float32)
+    assert lib.orc_run_unfused_f32(C.byref(p), _p(_flags(flags, p)), _p(f), int(nsteps)) == 0
+    return f
+
+
+def to_bf16(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, np.uint16)
+    lib.orc_f32_to_bf16_n(_p(x), _p(out), x.size)
+    return out
+
+
+def to_f16(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, np.uint16)
+    lib.orc_f32_to_f16_n(_p(x), _p(out), x.size)
+    return out
+
+
+def encode_dev16(p: Params, kind, f):
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    out = np.empty(f.shape, np.uint16)
+    lib.orc_encode_dev16(C.byref(p), int(kind), _p(f), _p(out))
+    return out
+
+
+def decode_dev16(p: Params, kind, d):
+    d = np.ascontiguousarray(d, dtype=np.uint16)
+    out = np.empty(d.shape, np.float32)
+    lib.orc_decode_dev16(C.byref(p), int(kind), _p(d), _p(out))
+    return out
+
+
+def run_dev16(p: Params, kind, d, nsteps, flags=None):
+    d = np.array(d, copy=True, order="C", dtype=np.uint16)
+    assert lib.orc_run_dev16(C.byref(p), int(kind), _p(_flags(flags, p)), _p(d), int(nsteps)) == 0
+    return d
+
+
+def sums(f, flags=None):
+    f = np.ascontiguousarray(f)
+    out = np.zeros(3, np.float64)
+    fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+    getattr(lib, "orc_sums_" + _sfx(f.dtype))(f.shape[2], f.shape[1], _p(fl), _p(f), _p(out))
+    return out
+
+
+def residual(ux, uy, ux_prev, uy_prev, flags=None):
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (ux, uy, ux_prev, uy_prev)]
+    fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+    h, w = arrs[0].shape
+    return lib.orc_residual(w, h, _p(fl), *[_p(a) for a in arrs])

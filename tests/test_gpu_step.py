@@ -1,0 +1,274 @@
+"""Parity of the HIP step (through the C ABI) with the CPU oracle.
+
+Bar: FP32 and FP64 results equal the oracle's value for value
+(np.testing.assert_array_equal); 16-bit storage equals the oracle's 16-bit
+restatement bit for bit.  Full-size runs are checked through conservation,
+analytic decay and the FP64 GPU run.
+"""
+import numpy as np
+import pytest
+
+from chemsim_amd import Lattice, Params, capi
+from chemsim_amd import workloads as W
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+STORE = {"f32": np.float32, "f64": np.float64}
+
+
+def rand_state(rng, h, w, dtype):
+    rho = 1 + 0.02 * rng.standard_normal((h, w))
+    ux = 0.03 * rng.standard_normal((h, w))
+    uy = 0.03 * rng.standard_normal((h, w))
+    p = O.params(w, h)
+    f = O.equilibrium(p, rho, ux, uy) + 0.002 * rng.standard_normal((9, h, w))
+    return f.astype(dtype)
+
+
+def random_flags(rng, h, w, lids=True):
+    fl = (rng.random((h, w)) < 0.15).astype(np.uint8)
+    if lids:
+        fl[rng.random((h, w)) < 0.05] = O.LID
+    return fl
+
+
+def gpu_params(op: O.Params) -> Params:
+    return Params(tau=op.tau, delta_x=op.delta_x, delta_t=op.delta_t, edge=op.edge, boundary=op.boundary,
+                  lid_ux=op.lid_ux, lid_uy=op.lid_uy, inlet_ux=op.inlet_ux, inlet_uy=op.inlet_uy,
+                  outlet_rho=op.outlet_rho, rho0=op.rho0)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+@pytest.mark.parametrize("with_flags", [False, True])
+def test_reference_mode_matches_oracle(ctx, storage, edge, with_flags):
+    rng = np.random.default_rng(11)
+    h, w = 21, 137
+    f = rand_state(rng, h, w, STORE[storage])
+    flags = random_flags(rng, h, w) if with_flags else None
+    op = O.params(w, h, tau=0.6, edge=edge)
+    with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+        lat.upload(f)
+        ref = f
+        for n in (1, 24):
+            lat.step(n)
+            ref = O.run(op, ref, n, flags)
+            np.testing.assert_array_equal(lat.download(STORE[storage]), ref)
+        assert lat.time == 25.0
+
+
+@pytest.mark.parametrize("cpt", [1, 2, 4])
+def test_launch_shapes_agree(ctx, cpt):
+    rng = np.random.default_rng(12)
+    h, w = 9, 300
+    f = rand_state(rng, h, w, np.float32)
+    op = O.params(w, h, tau=0.7, edge=O.EDGE_PERIODIC)
+    try:
+        capi.call("cs_set_cells_per_thread", cpt)
+        with Lattice(ctx, w, h, "f32", gpu_params(op)) as lat:
+            lat.upload(f)
+            lat.step(3)
+            np.testing.assert_array_equal(lat.download(), O.run(op, f, 3))
+    finally:
+        capi.call("cs_set_cells_per_thread", 0)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_unfused_phases(ctx, storage):
+    """stream, bounce_back and collide alone (lbm.rs:716-751) compose to one step."""
+    rng = np.random.default_rng(13)
+    h, w = 17, 23
+    dt = STORE[storage]
+    f = rand_state(rng, h, w, dt)
+    flags = random_flags(rng, h, w)
+    op = O.params(w, h, tau=0.9)
+    with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+        lat.upload(f)
+        lat.stream()
+        np.testing.assert_array_equal(lat.download(dt), O.stream(f))
+        lat.bounce_back()
+        b = O.bounce_back(O.stream(f), flags)
+        np.testing.assert_array_equal(lat.download(dt), b)
+        lat.collide()
+        np.testing.assert_array_equal(lat.download(dt), O.step_unfused(op, f, flags))
+        np.testing.assert_array_equal(lat.download(dt), O.step(op, f, flags))
+
+
+def test_cavity_128_halfway_lid_10000_steps(ctx):
+    """Lid-driven cavity 128x128, tau 0.6, lid 0.1 on row 0, half-way walls,
+    f = f_eq(1, 0), 10,000 steps: f, rho and u equal the oracle's."""
+    n, steps = 128, 10000
+    flags = W.cavity_flags(n, n)
+    op = O.params(n, n, tau=0.6, boundary=O.BB_HALFWAY, lid=W.lid_velocity(0.1))
+    rho, ux, uy = W.rest_state(n, n)
+    f0 = O.equilibrium(op, rho, ux, uy)
+    ref = O.run(op, f0, steps, flags)
+    with Lattice(ctx, n, n, "f32", gpu_params(op), flags) as lat:
+        lat.init_equilibrium(rho, ux, uy)
+        np.testing.assert_array_equal(lat.download(), f0)
+        lat.step(steps)
+        got = lat.download()
+        np.testing.assert_array_equal(got, ref)
+        np.testing.assert_array_equal(lat.density(), O.density(ref))
+        gx, gy = lat.velocity()
+        rx, ry = O.velocity(ref)
+        np.testing.assert_array_equal(gx, rx)
+        np.testing.assert_array_equal(gy, ry)
+    # the flow is developed: the lid drags the top row, a vortex forms
+    assert ry[1, n // 2] > 0.03 and ry[n - 10, n // 2] < 0
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+def test_halfway_random_geometry(ctx, storage, edge):
+    rng = np.random.default_rng(14)
+    h, w = 33, 70
+    dt = STORE[storage]
+    f = rand_state(rng, h, w, dt)
+    flags = random_flags(rng, h, w)
+    op = O.params(w, h, tau=0.75, edge=edge, boundary=O.BB_HALFWAY, lid=(0.02, 0.05))
+    with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+        lat.upload(f)
+        lat.step(40)
+        np.testing.assert_array_equal(lat.download(dt), O.run(op, f, 40, flags))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_zou_he_channel(ctx, storage):
+    ch = W.Channel(w=96, h=40, circles=[(20, 30, 7)], re=30.0, u_in=0.05)
+    flags = ch.flags()
+    op = O.params(ch.w, ch.h, tau=ch.tau, boundary=O.BB_HALFWAY, inlet=ch.inlet(), outlet_rho=1.0)
+    dt = STORE[storage]
+    rho, ux, uy = (a.astype(dt) for a in W.rest_state(ch.w, ch.h))
+    f0 = O.equilibrium(op, rho, ux, uy)
+    with Lattice(ctx, ch.w, ch.h, storage, gpu_params(op), flags) as lat:
+        lat.init_equilibrium(rho, ux, uy)
+        lat.step(300)
+        np.testing.assert_array_equal(lat.download(dt), O.run(op, f0, 300, flags))
+
+
+@pytest.mark.parametrize("kind", ["bf16", "f16"])
+@pytest.mark.parametrize("halfway", [False, True])
+def test_dev16_matches_oracle(ctx, kind, halfway):
+    n, m = 48, 72
+    rho, ux, uy = W.taylor_green(n, 0.05, np.float32, m=m)
+    okind = O.BF16 if kind == "bf16" else O.F16
+    if halfway:
+        flags = W.cavity_flags(m, n)
+        op = O.params(m, n, tau=0.8, boundary=O.BB_HALFWAY, lid=W.lid_velocity(0.08))
+    else:
+        flags = None
+        op = O.params(m, n, tau=0.8, edge=O.EDGE_PERIODIC)
+    f0 = O.equilibrium(op, rho, ux, uy)
+    d0 = O.encode_dev16(op, okind, f0)
+    with Lattice(ctx, m, n, kind, gpu_params(op), flags) as lat:
+        lat.init_equilibrium(rho, ux, uy)
+        np.testing.assert_array_equal(lat.download_raw(), d0)
+        lat.step(20)
+        np.testing.assert_array_equal(lat.download_raw(), O.run_dev16(op, okind, d0, 20, flags))
+        np.testing.assert_array_equal(lat.download(), O.decode_dev16(op, okind, lat.download_raw()))
+
+
+def test_moment_api(ctx):
+    rng = np.random.default_rng(15)
+    h, w = 19, 31
+    f = rand_state(rng, h, w, np.float32)
+    op = O.params(w, h, tau=0.6)
+    with Lattice(ctx, w, h, "f32", gpu_params(op)) as lat:
+        lat.upload(f)
+        np.testing.assert_array_equal(lat.density(), O.density(f))
+        mx, my = lat.momentum_density()
+        rx, ry = O.momentum(f)
+        np.testing.assert_array_equal(mx, rx)
+        np.testing.assert_array_equal(my, ry)
+        ux, uy = lat.velocity()
+        vx, vy = O.velocity(f)
+        np.testing.assert_array_equal(ux, vx)
+        np.testing.assert_array_equal(uy, vy)
+        np.testing.assert_array_equal(lat.speed(), np.sqrt(vx * vx + vy * vy))
+        cs2 = np.float32(O.consts_f32(op).cs2)
+        np.testing.assert_array_equal(lat.pressure(), O.density(f) * cs2)
+        feq = O.equilibrium(op, O.density(f), vx, vy)
+        np.testing.assert_array_equal(lat.equilibrium(), feq)
+        np.testing.assert_array_equal(lat.non_equilibrium(), f - feq)
+        assert not lat.is_unstable()
+        # |u|^2 > 2/3 makes f_eq[0] negative (lbm.rs:815-818)
+        lat.init_equilibrium(np.ones((h, w), np.float32), np.full((h, w), 0.9, np.float32),
+                             np.zeros((h, w), np.float32))
+        assert lat.is_unstable()
+
+
+def test_sums_and_residual(ctx):
+    rng = np.random.default_rng(16)
+    h, w = 40, 50
+    f = rand_state(rng, h, w, np.float32)
+    flags = random_flags(rng, h, w, lids=False)
+    op = O.params(w, h, tau=0.8, boundary=O.BB_HALFWAY)
+    with Lattice(ctx, w, h, "f32", gpu_params(op), flags) as lat:
+        lat.upload(f)
+        np.testing.assert_allclose(lat.sums_f64(), O.sums(f, flags), rtol=1e-12, atol=1e-12)
+        lat.snapshot_velocity()
+        u0 = O.velocity(f)
+        lat.step(5)
+        g = lat.download()
+        u1 = O.velocity(g)
+        np.testing.assert_allclose(lat.residual_f64(), O.residual(*u1, *u0, flags), rtol=1e-12)
+
+
+def test_geometry_edit_between_steps(ctx):
+    """The mask may change at any step boundary (main.rs:77-89)."""
+    rng = np.random.default_rng(17)
+    h, w = 24, 24
+    f = rand_state(rng, h, w, np.float32)
+    fl1 = random_flags(rng, h, w, lids=False)
+    fl2 = random_flags(rng, h, w, lids=False)
+    op = O.params(w, h, tau=0.7)
+    with Lattice(ctx, w, h, "f32", gpu_params(op), fl1) as lat:
+        lat.upload(f)
+        lat.step(3)
+        lat.set_flags(fl2)
+        lat.step(3)
+        lat.set_flags(None)
+        lat.step(2)
+        ref = O.run(op, O.run(op, O.run(op, f, 3, fl1), 3, fl2), 2, None)
+        np.testing.assert_array_equal(lat.download(), ref)
+
+
+def test_run_to_residual_converges(ctx):
+    n = 32
+    flags = W.cavity_flags(n, n)
+    op = O.params(n, n, tau=0.8, boundary=O.BB_HALFWAY, lid=W.lid_velocity(0.05))
+    with Lattice(ctx, n, n, "f32", gpu_params(op), flags) as lat:
+        lat.init_equilibrium(*W.rest_state(n, n))
+        steps, res = lat.run_to_residual(1e-6, 100, 200000)
+        assert res <= 1e-6 and 0 < steps < 200000
+        lat.snapshot_velocity()
+        lat.step(100)
+        assert lat.residual_f64() < 2e-6
+
+
+def test_taylor_green_4096_fp32_vs_fp64(ctx):
+    """Full size: TGV 4096^2, tau 0.8, u0 0.05, 1,000 steps; FP32 against the FP64
+    GPU run, analytic decay and FP64 mass conservation."""
+    n, steps, tau, u0 = 4096, 1000, 0.8, 0.05
+    rho, ux, uy = W.taylor_green(n, u0, np.float64)
+    p = Params(tau=tau, edge=capi.CS_EDGE_PERIODIC)
+    res = {}
+    for storage in ("f64", "f32"):
+        with Lattice(ctx, n, n, storage, p) as lat:
+            lat.init_equilibrium(*(a if storage == "f64" else a.astype(np.float32) for a in (rho, ux, uy)))
+            s0 = lat.sums_f64()
+            lat.step(steps)
+            s1 = lat.sums_f64()
+            res[storage] = (lat.moments_f64(), s0, s1)
+    (r64, x64, y64), s0, s1 = res["f64"]
+    assert abs(s1[0] - s0[0]) < 1e-9 * s0[0]
+    (r32, x32, y32), t0, t1 = res["f32"]
+    assert abs(t1[0] - t0[0]) < 1e-6 * t0[0]
+    num = np.sqrt(((x32 - x64) ** 2 + (y32 - y64) ** 2).sum())
+    den = np.sqrt((x64 ** 2 + y64 ** 2).sum())
+    assert num / den < 1e-4
+    assert np.abs(r32 - r64).max() < 1e-5
+    amp = ((x64 * ux + y64 * uy).sum() / (ux * ux + uy * uy).sum())
+    assert abs(amp / W.taylor_green_decay(n, tau, steps) - 1) < 1e-6
