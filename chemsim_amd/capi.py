@@ -15,7 +15,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "chemsim.h")
 
 # enums (include/chemsim.h)
 CS_OK, CS_ERR_ARG, CS_ERR_HIP, CS_ERR_UNSUPPORTED, CS_ERR_STATE, CS_ERR_NOMEM = range(6)
-CS_F32, CS_F64, CS_BF16DEV, CS_F16DEV = range(4)
+CS_F32, CS_F64, CS_BF16DEV, CS_F16DEV, CS_F32DEV = range(5)
 CS_EDGE_ZERO, CS_EDGE_PERIODIC = range(2)
 CS_BB_REFERENCE, CS_BB_HALFWAY = range(2)
 CS_FLUID, CS_WALL, CS_LID, CS_INLET, CS_OUTLET = range(5)

@@ -60,8 +60,8 @@ constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 
 // Per-run constants in the working precision T (float or double):
 // K = -dt/tau (lbm.rs:357); A, B, C of compute_equilibrium (lbm.rs:54-66);
-// weights (lbm.rs:209-219); Ladd lid terms 6 w_i (c_i . u_lid); 16-bit
-// storage bases w_i * rho0; Zou-He boundary values.
+// weights (lbm.rs:209-219); Ladd lid terms 6 w_i (c_i . u_lid); deviation
+// storage bases w_i * rho0 and rho0; Zou-He boundary values.
 template <class T>
 struct Consts {
     T K, A, B, C, cs2;
@@ -69,6 +69,7 @@ struct Consts {
     T lid[9];
     T base[9];
     T in_ux, in_uy, out_rho;
+    T rho0;
 };
 
 Consts<float> make_consts_f32(const cs_params& p);

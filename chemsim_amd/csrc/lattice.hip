@@ -21,7 +21,8 @@ void clear_error() { g_error.clear(); }
 size_t storage_bytes(int storage)
 {
     switch (storage) {
-    case CS_F32: return 4;
+    case CS_F32:
+    case CS_F32DEV: return 4;
     case CS_F64: return 8;
     case CS_BF16DEV:
     case CS_F16DEV: return 2;
@@ -63,6 +64,7 @@ static Consts<T> make_consts(const cs_params& p)
     k.in_ux = (T)p.inlet_ux;
     k.in_uy = (T)p.inlet_uy;
     k.out_rho = (T)p.outlet_rho;
+    k.rho0 = (T)p.rho0;
     return k;
 }
 
@@ -108,11 +110,32 @@ __device__ __forceinline__ void equilibrium(const Consts<T>& k, T rho, T ux, T u
     }
 }
 
+// Deviation-form equilibrium g_eq_i = w_i (drho + rho s'_i) with drho = rho - rho0.
 template <class T>
-__device__ __forceinline__ void moments(const T (&B)[9], T& rho, T& mx, T& my, T& ux, T& uy)
+__device__ __forceinline__ void equilibrium_dev(const Consts<T>& k, T rho, T ux, T uy, T (&G)[9])
 {
 #pragma clang fp contract(off)
-    rho = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
+    const T drho = rho - k.rho0;
+    const T v2 = ux * ux + uy * uy;
+    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T s = (vc[i] * k.A + (vc[i] * vc[i]) * k.B) + v2 * k.C;
+        G[i] = k.w[i] * (drho + rho * s);
+    }
+}
+
+// Lattice::density / momentum_density / velocity (lbm.rs:117-149) of the
+// stored values (deviations add rho0 to the density).
+template <class St>
+__device__ __forceinline__ void moments(const Consts<typename St::T>& k, const typename St::T (&B)[9],
+                                        typename St::T& rho, typename St::T& mx, typename St::T& my,
+                                        typename St::T& ux, typename St::T& uy)
+{
+#pragma clang fp contract(off)
+    using T = typename St::T;
+    const T sum = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
+    rho = St::DEV ? k.rho0 + sum : sum;
     mx = ((((B[1] - B[3]) + B[5]) - B[6]) - B[7]) + B[8];
     my = ((((B[2] - B[4]) + B[5]) + B[6]) - B[7]) - B[8];
     const T inv = T(1) / rho;
@@ -121,11 +144,10 @@ __device__ __forceinline__ void moments(const T (&B)[9], T& rho, T& mx, T& my, T
 }
 
 template <class St>
-__device__ __forceinline__ void load_own(const typename St::S* f, int64_t P, int64_t own,
-                                         const Consts<typename St::T>& k, typename St::T (&B)[9])
+__device__ __forceinline__ void load_own(const typename St::S* f, int64_t P, int64_t own, typename St::T (&B)[9])
 {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) B[i] = St::ld(f, (int64_t)i * P + own, k.base[i]);
+    for (int i = 0; i < 9; ++i) B[i] = St::ld(f, (int64_t)i * P + own);
 }
 
 // Dense f32 (pitched) <-> storage.
@@ -138,7 +160,10 @@ __global__ void k_from_f32(const float* __restrict__ src, typename St::S* __rest
     if (c >= w) return;
     const int64_t own = (int64_t)r * pitch + c;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) St::st(dst, i * P + own, (typename St::T)src[i * P + own], k.base[i]);
+    for (int i = 0; i < 9; ++i) {
+        const typename St::T v = (typename St::T)src[i * P + own];
+        St::st(dst, i * P + own, St::DEV ? v - k.base[i] : v);
+    }
 }
 
 template <class St>
@@ -150,7 +175,10 @@ __global__ void k_to_f32(const typename St::S* __restrict__ src, float* __restri
     if (c >= w) return;
     const int64_t own = (int64_t)r * pitch + c;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) dst[i * P + own] = (float)St::ld(src, i * P + own, k.base[i]);
+    for (int i = 0; i < 9; ++i) {
+        const typename St::T v = St::ld(src, i * P + own);
+        dst[i * P + own] = (float)(St::DEV ? v + k.base[i] : v);
+    }
 }
 
 // f = compute_equilibrium(rho, u) (lbm.rs:43-71); inputs dense h*w planes in T.
@@ -165,9 +193,10 @@ __global__ void k_init_eq(const typename St::T* __restrict__ rho, const typename
     if (c >= w) return;
     const int64_t d = (int64_t)r * w + c, own = (int64_t)r * pitch + c;
     T F[9];
-    equilibrium<T>(k, rho[d], ux[d], uy[d], F);
+    if constexpr (St::DEV) equilibrium_dev<T>(k, rho[d], ux[d], uy[d], F);
+    else equilibrium<T>(k, rho[d], ux[d], uy[d], F);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) St::st(f, i * P + own, F[i], k.base[i]);
+    for (int i = 0; i < 9; ++i) St::st(f, i * P + own, F[i]);
 }
 
 enum : unsigned {
@@ -185,9 +214,9 @@ __global__ void k_moments(const typename St::S* __restrict__ f, int w, int h, in
     if (c >= w) return;
     const int64_t own = (int64_t)r * pitch + c, d = (int64_t)r * w + c, n = (int64_t)w * h;
     T B[9];
-    load_own<St>(f, P, own, k, B);
+    load_own<St>(f, P, own, B);
     T rho, mx, my, ux, uy;
-    moments<T>(B, rho, mx, my, ux, uy);
+    moments<St>(k, B, rho, mx, my, ux, uy);
     int slot = 0;
     if (mask & M_RHO) out[(slot++) * n + d] = (double)rho;
     if (mask & M_MX) out[(slot++) * n + d] = (double)mx;
@@ -209,10 +238,11 @@ __global__ void k_equilibrium(const typename St::S* __restrict__ f, int w, int h
     if (c >= w) return;
     const int64_t own = (int64_t)r * pitch + c, d = (int64_t)r * w + c, n = (int64_t)w * h;
     T B[9], F[9];
-    load_own<St>(f, P, own, k, B);
+    load_own<St>(f, P, own, B);
     T rho, mx, my, ux, uy;
-    moments<T>(B, rho, mx, my, ux, uy);
-    equilibrium<T>(k, rho, ux, uy, F);
+    moments<St>(k, B, rho, mx, my, ux, uy);
+    if (non_eq && St::DEV) equilibrium_dev<T>(k, rho, ux, uy, F);  // g - g_eq = f - f_eq
+    else equilibrium<T>(k, rho, ux, uy, F);
 #pragma unroll
     for (int i = 0; i < 9; ++i) out[i * n + d] = (double)(non_eq ? B[i] - F[i] : F[i]);
 }
@@ -228,9 +258,9 @@ __global__ void k_unstable(const typename St::S* __restrict__ f, int w, int h, i
     if (c >= w) return;
     const int64_t own = (int64_t)r * pitch + c;
     T B[9], F[9];
-    load_own<St>(f, P, own, k, B);
+    load_own<St>(f, P, own, B);
     T rho, mx, my, ux, uy;
-    moments<T>(B, rho, mx, my, ux, uy);
+    moments<St>(k, B, rho, mx, my, ux, uy);
     equilibrium<T>(k, rho, ux, uy, F);
     if (F[0] < T(0)) atomicOr(flag, 1);
 }
@@ -288,10 +318,10 @@ __global__ void k_collide(typename St::S* __restrict__ f, int w, int h, int pitc
     if (c >= w) return;
     const int64_t own = (int64_t)r * pitch + c;
     T B[9];
-    load_own<St>(f, P, own, k, B);
+    load_own<St>(f, P, own, B);
     bgk_collide<T>(k, B);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) St::st(f, i * P + own, B[i], k.base[i]);
+    for (int i = 0; i < 9; ++i) St::st(f, i * P + own, B[i]);
 }
 
 // ---- FP64 reductions ---------------------------------------------------
@@ -349,9 +379,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_sums(const typename St::S* __re
             if (t == CS_WALL || t == CS_LID) continue;
         }
         T B[9];
-        load_own<St>(f, P, own, k, B);
+        load_own<St>(f, P, own, B);
         T rho, mx, my, ux, uy;
-        moments<T>(B, rho, mx, my, ux, uy);
+        moments<St>(k, B, rho, mx, my, ux, uy);
         v[0] += (double)rho;
         v[1] += (double)mx;
         v[2] += (double)my;
@@ -370,9 +400,9 @@ __global__ void k_snapshot(const typename St::S* __restrict__ f, int w, int h, i
     if (c >= w) return;
     const int64_t own = (int64_t)r * pitch + c, d = (int64_t)r * w + c, n = (int64_t)w * h;
     T B[9];
-    load_own<St>(f, P, own, k, B);
+    load_own<St>(f, P, own, B);
     T rho, mx, my, ux, uy;
-    moments<T>(B, rho, mx, my, ux, uy);
+    moments<St>(k, B, rho, mx, my, ux, uy);
     snap[d] = (double)ux;
     snap[n + d] = (double)uy;
 }
@@ -394,9 +424,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_residual(const typename St::S* 
             if (t == CS_WALL || t == CS_LID) continue;
         }
         T B[9];
-        load_own<St>(f, P, own, k, B);
+        load_own<St>(f, P, own, B);
         T rho, mx, my, ux, uy;
-        moments<T>(B, rho, mx, my, ux, uy);
+        moments<St>(k, B, rho, mx, my, ux, uy);
         const double dx = (double)ux - snap[d], dy = (double)uy - snap[n + d];
         v[0] += dx * dx + dy * dy;
         v[1] += (double)ux * (double)ux + (double)uy * (double)uy;
@@ -417,6 +447,7 @@ static void check_lat(const cs_lattice* lat) { CS_REQUIRE(lat != nullptr, CS_ERR
 template <class St>
 static const Consts<typename St::T>& consts_of(const cs_lattice* lat);
 template <> const Consts<float>& consts_of<StF32>(const cs_lattice* l) { return l->k32; }
+template <> const Consts<float>& consts_of<StF32Dev>(const cs_lattice* l) { return l->k32; }
 template <> const Consts<float>& consts_of<StBF16>(const cs_lattice* l) { return l->k32; }
 template <> const Consts<float>& consts_of<StF16>(const cs_lattice* l) { return l->k32; }
 template <> const Consts<double>& consts_of<StF64>(const cs_lattice* l) { return l->k64; }
@@ -428,6 +459,7 @@ static void dispatch_storage(int storage, F&& f)
     switch (storage) {
     case CS_F32: f.template operator()<StF32>(); break;
     case CS_F64: f.template operator()<StF64>(); break;
+    case CS_F32DEV: f.template operator()<StF32Dev>(); break;
     case CS_BF16DEV: f.template operator()<StBF16>(); break;
     case CS_F16DEV: f.template operator()<StF16>(); break;
     default: throw Error{CS_ERR_ARG, "bad storage"};
@@ -695,7 +727,8 @@ extern "C" int cs_lattice_set_params(cs_lattice* l, const cs_params* p)
         CS_REQUIRE(p->outlet_rho > 0, CS_ERR_ARG, "outlet_rho must be > 0");
         const bool edge_changed = p->edge != l->params.edge;
         // Re-encode 16-bit storage when the deviation base changes.
-        if ((l->storage == CS_BF16DEV || l->storage == CS_F16DEV) && p->rho0 != l->params.rho0) {
+        if ((l->storage == CS_F32DEV || l->storage == CS_BF16DEV || l->storage == CS_F16DEV) &&
+            p->rho0 != l->params.rho0) {
             const int64_t n = (int64_t)9 * l->w * l->h;
             std::vector<float> tmp((size_t)n);
             if (cs_lattice_download_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};

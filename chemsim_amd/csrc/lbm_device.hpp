@@ -34,37 +34,43 @@ __device__ __forceinline__ uint16_t f32_to_bf16_bits(float x)
 __device__ __forceinline__ float f16_bits_to_f32(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
 __device__ __forceinline__ uint16_t f32_to_f16_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 
-// S: stored element type.  T: arithmetic type.  ld() widens a stored value of
-// the plane whose base is `base` to a full population; st() narrows.
+// S: stored element type.  T: arithmetic type.  DEV: the stored value is the
+// deviation g_i = f_i - w_i*rho0 and the step runs in deviation arithmetic.
+// ld() widens a stored value, st() narrows (round to nearest even).
 struct StF32 {
     using S = float;
     using T = float;
-    __device__ static __forceinline__ T ld(const S* p, int64_t o, T) { return p[o]; }
-    __device__ static __forceinline__ void st(S* p, int64_t o, T v, T) { __builtin_nontemporal_store(v, p + o); }
+    static constexpr bool DEV = false;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o) { return p[o]; }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(v, p + o); }
 };
 struct StF64 {
     using S = double;
     using T = double;
-    __device__ static __forceinline__ T ld(const S* p, int64_t o, T) { return p[o]; }
-    __device__ static __forceinline__ void st(S* p, int64_t o, T v, T) { __builtin_nontemporal_store(v, p + o); }
+    static constexpr bool DEV = false;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o) { return p[o]; }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(v, p + o); }
+};
+struct StF32Dev {
+    using S = float;
+    using T = float;
+    static constexpr bool DEV = true;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o) { return p[o]; }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(v, p + o); }
 };
 struct StBF16 {
     using S = uint16_t;
     using T = float;
-    __device__ static __forceinline__ T ld(const S* p, int64_t o, T base) { return bf16_bits_to_f32(p[o]) + base; }
-    __device__ static __forceinline__ void st(S* p, int64_t o, T v, T base)
-    {
-        __builtin_nontemporal_store(f32_to_bf16_bits(v - base), p + o);
-    }
+    static constexpr bool DEV = true;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o) { return bf16_bits_to_f32(p[o]); }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(f32_to_bf16_bits(v), p + o); }
 };
 struct StF16 {
     using S = uint16_t;
     using T = float;
-    __device__ static __forceinline__ T ld(const S* p, int64_t o, T base) { return f16_bits_to_f32(p[o]) + base; }
-    __device__ static __forceinline__ void st(S* p, int64_t o, T v, T base)
-    {
-        __builtin_nontemporal_store(f32_to_f16_bits(v - base), p + o);
-    }
+    static constexpr bool DEV = true;
+    __device__ static __forceinline__ T ld(const S* p, int64_t o) { return f16_bits_to_f32(p[o]); }
+    __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(f32_to_f16_bits(v), p + o); }
 };
 
 // Boundary handling variants of the step kernel.
@@ -116,6 +122,61 @@ __device__ __forceinline__ void bgk_collide(const Consts<T>& k, T (&B)[9])
         const T feq = (rho * k.w[i]) * s;
         B[i] = B[i] + (B[i] - feq) * k.K;
     }
+}
+
+// Deviation arithmetic: with g_i = f_i - w_i rho0,
+//   drho = sum g_i, rho = rho0 + drho, m = sum g_i c_i, u = m / rho,
+//   g_eq_i = w_i (drho + rho ((vc A + vc^2 B) + v2 C)),  g'_i = g_i + (g_i - g_eq_i) K,
+// the same BGK update as bgk_collide without carrying the O(1) rest
+// populations through the rounding.
+template <class T>
+__device__ __forceinline__ void bgk_collide_dev(const Consts<T>& k, T (&B)[9])
+{
+#pragma clang fp contract(off)
+    const T drho = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
+    const T rho = k.rho0 + drho;
+    const T mx = ((((B[1] - B[3]) + B[5]) - B[6]) - B[7]) + B[8];
+    const T my = ((((B[2] - B[4]) + B[5]) + B[6]) - B[7]) - B[8];
+    const T inv = T(1) / rho;
+    const T ux = inv * mx, uy = inv * my;
+    const T v2 = ux * ux + uy * uy;
+    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T s = (vc[i] * k.A + (vc[i] * vc[i]) * k.B) + v2 * k.C;
+        const T geq = k.w[i] * (drho + rho * s);
+        B[i] = B[i] + (B[i] - geq) * k.K;
+    }
+}
+
+// Zou-He on deviations: the unknown populations follow the same formulas
+// (opposite directions share w_i); only the density sums carry rho0.
+template <class T>
+__device__ __forceinline__ void zou_he_inlet_dev(T (&B)[9], T rho0, T ux, T uy)
+{
+#pragma clang fp contract(off)
+    const T G = ((B[0] + B[1]) + B[3]) + T(2) * ((B[4] + B[7]) + B[8]);
+    const T rho = (rho0 + G) / (T(1) - uy);
+    const T ru_t = rho * ux, ru_a = rho * uy;
+    const T d13 = B[1] - B[3];
+    const T two3 = T(2) / T(3), six1 = T(1) / T(6), half = T(0.5);
+    B[2] = B[4] + two3 * ru_a;
+    B[5] = ((B[7] - half * d13) + half * ru_t) + six1 * ru_a;
+    B[6] = ((B[8] + half * d13) - half * ru_t) + six1 * ru_a;
+}
+
+template <class T>
+__device__ __forceinline__ void zou_he_outlet_dev(T (&B)[9], T rho0, T rho)
+{
+#pragma clang fp contract(off)
+    const T G = ((B[0] + B[1]) + B[3]) + T(2) * ((B[2] + B[5]) + B[6]);
+    const T uy = ((rho0 - rho) + G) / rho;
+    const T ru_a = rho * uy;
+    const T d13 = B[1] - B[3];
+    const T two3 = T(2) / T(3), six1 = T(1) / T(6), half = T(0.5);
+    B[4] = B[2] - two3 * ru_a;
+    B[7] = (B[5] + half * d13) - six1 * ru_a;
+    B[8] = (B[6] - half * d13) - six1 * ru_a;
 }
 
 // Zou-He velocity inlet (low-column edge; unknown populations 2, 5, 6).
@@ -210,14 +271,20 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
                 add_lid = (info >> (9 + i)) & 1u;
             }
         }
-        T v = St::ld(fin, off, k.base[i]);
-        v = valid ? v : T(0);
+        // A population from outside a zero-filled edge is f = 0, i.e. g = -w_i rho0.
+        T v = St::ld(fin, off);
+        v = valid ? v : (St::DEV ? -k.base[i] : T(0));
         if (MODE == MODE_HALF) v = add_lid ? v + k.lid[i] : v;
         B[i] = v;
     }
     if (MODE == MODE_HALF) {
-        if (type == CS_INLET) zou_he_inlet<T>(B, k.in_ux, k.in_uy);
-        else if (type == CS_OUTLET) zou_he_outlet<T>(B, k.out_rho);
+        if constexpr (St::DEV) {
+            if (type == CS_INLET) zou_he_inlet_dev<T>(B, k.rho0, k.in_ux, k.in_uy);
+            else if (type == CS_OUTLET) zou_he_outlet_dev<T>(B, k.rho0, k.out_rho);
+        } else {
+            if (type == CS_INLET) zou_he_inlet<T>(B, k.in_ux, k.in_uy);
+            else if (type == CS_OUTLET) zou_he_outlet<T>(B, k.out_rho);
+        }
     }
     return type;
 }
@@ -235,9 +302,10 @@ __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, t
         for (int i = 0; i < 9; ++i) fout[(int64_t)i * P + own] = fin[(int64_t)i * P + own];
         return;
     }
-    bgk_collide(k, B);
+    if constexpr (St::DEV) bgk_collide_dev(k, B);
+    else bgk_collide(k, B);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) St::st(fout, (int64_t)i * P + own, B[i], k.base[i]);
+    for (int i = 0; i < 9; ++i) St::st(fout, (int64_t)i * P + own, B[i]);
 }
 
 // Fused step.  A block is 4 waves; wave y handles row 4*ty + y, lane l the

@@ -14,9 +14,10 @@ import numpy as np
 from . import capi
 from .capi import call
 
-STORAGE = {"f32": capi.CS_F32, "f64": capi.CS_F64, "bf16": capi.CS_BF16DEV, "f16": capi.CS_F16DEV}
+STORAGE = {"f32": capi.CS_F32, "f64": capi.CS_F64, "bf16": capi.CS_BF16DEV, "f16": capi.CS_F16DEV,
+           "f32dev": capi.CS_F32DEV}
 RAW_DTYPE = {capi.CS_F32: np.float32, capi.CS_F64: np.float64,
-             capi.CS_BF16DEV: np.uint16, capi.CS_F16DEV: np.uint16}
+             capi.CS_BF16DEV: np.uint16, capi.CS_F16DEV: np.uint16, capi.CS_F32DEV: np.float32}
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:

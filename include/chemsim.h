@@ -55,8 +55,13 @@ enum cs_storage {
     CS_F32 = 0,     /* Scalar = f32, lbm.rs:13 */
     CS_F64 = 1,     /* FP64 oracle run (extension) */
     CS_BF16DEV = 2, /* bf16 deviations f_i - w_i*rho0, FP32 arithmetic (extension) */
-    CS_F16DEV = 3   /* fp16 deviations f_i - w_i*rho0, FP32 arithmetic (extension) */
+    CS_F16DEV = 3,  /* fp16 deviations f_i - w_i*rho0, FP32 arithmetic (extension) */
+    CS_F32DEV = 4   /* f32 deviations f_i - w_i*rho0, FP32 arithmetic (extension) */
 };
+/* The deviation storages (BF16DEV, F16DEV, F32DEV) run the step in deviation
+ * arithmetic: drho = sum g_i, g_eq_i = w_i (drho + rho s_i), g' = g + (g - g_eq) K,
+ * the BGK update of lbm.rs:350-364 written for g = f - w rho0.  F32 and F64
+ * keep the reference's operation order on full populations. */
 
 /* Treatment of the lattice edge.  ZERO is the reference: af::convolve2 with
  * ConvMode::DEFAULT pads with zeros (lbm.rs:722-724). */

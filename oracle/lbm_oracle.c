@@ -225,24 +225,164 @@ void orc_decode_dev16(const orc_params* p, int kind, const uint16_t* d, float* f
         for (long j = 0; j < n; ++j) f[i * n + j] = dev_load(kind, d[i * n + j]) + k.base[i];
 }
 
-/* One 16-bit step: widen the 9 neighbours' stored deviations to FP32 full
- * populations (decode), run the FP32 cell update, narrow (encode).  Solid
- * cells of the half-way mode keep their stored bits. */
+/* ---- deviation arithmetic (extension) ------------------------------------
+ * With g_i = f_i - w_i rho0 the step is evaluated on g directly:
+ *   drho = sum g_i, rho = rho0 + drho, m = sum g_i c_i, u = m / rho,
+ *   g_eq_i = w_i (drho + rho ((vc A + vc^2 B) + v2 C)),
+ *   g'_i = g_i + (g_i - g_eq_i) K,
+ * which is f' - w_i rho0 of the BGK update (lbm.rs:350-364) without carrying
+ * the O(1) rest populations through the rounding.  Streaming, bounce-back and
+ * the Ladd term act on g unchanged (opposite directions share w_i); a zero-
+ * filled population outside the lattice is g = -w_i rho0. */
+static inline void collide_dev(const orc_consts_f32* k, float rho0, float* B)
+{
+    float drho = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
+    float rho = rho0 + drho;
+    float mx = ((((B[1] - B[3]) + B[5]) - B[6]) - B[7]) + B[8];
+    float my = ((((B[2] - B[4]) + B[5]) + B[6]) - B[7]) - B[8];
+    float inv = 1.0f / rho;
+    float ux = inv * mx, uy = inv * my;
+    float v2 = ux * ux + uy * uy;
+    const float vc[9] = {0.0f, ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+    for (int i = 0; i < 9; ++i) {
+        float s = (vc[i] * k->A + (vc[i] * vc[i]) * k->B) + v2 * k->C;
+        float geq = k->w[i] * (drho + rho * s);
+        B[i] = B[i] + (B[i] - geq) * k->K;
+    }
+}
+
+static inline void zou_he_inlet_dev(float* B, float rho0, float ux, float uy)
+{
+    float G = ((B[0] + B[1]) + B[3]) + 2.0f * ((B[4] + B[7]) + B[8]);
+    float rho = (rho0 + G) / (1.0f - uy);
+    float ru_t = rho * ux, ru_a = rho * uy;
+    float d13 = B[1] - B[3];
+    B[2] = B[4] + (2.0f / 3.0f) * ru_a;
+    B[5] = ((B[7] - 0.5f * d13) + 0.5f * ru_t) + (1.0f / 6.0f) * ru_a;
+    B[6] = ((B[8] + 0.5f * d13) - 0.5f * ru_t) + (1.0f / 6.0f) * ru_a;
+}
+
+static inline void zou_he_outlet_dev(float* B, float rho0, float rho)
+{
+    float G = ((B[0] + B[1]) + B[3]) + 2.0f * ((B[2] + B[5]) + B[6]);
+    float uy = ((rho0 - rho) + G) / rho;
+    float ru_a = rho * uy;
+    float d13 = B[1] - B[3];
+    B[4] = B[2] - (2.0f / 3.0f) * ru_a;
+    B[7] = (B[5] + 0.5f * d13) - (1.0f / 6.0f) * ru_a;
+    B[8] = (B[6] - 0.5f * d13) - (1.0f / 6.0f) * ru_a;
+}
+
+static inline float fetch_dev(const orc_params* p, const orc_consts_f32* k, const float* G, int i, int j,
+                              int r, int c)
+{
+    const int w = p->w, h = p->h;
+    if (p->edge == ORC_EDGE_PERIODIC) {
+        r = (r % h + h) % h;
+        c = (c % w + w) % w;
+    } else if (r < 0 || r >= h || c < 0 || c >= w) {
+        return -k->base[i];
+    }
+    return G[(long)j * w * h + (long)r * w + c];
+}
+
+void orc_step_devf32(const orc_params* p, const uint8_t* flags, const float* G, float* out)
+{
+    orc_consts_f32 k;
+    orc_consts_f32_make(p, &k);
+    const int w = p->w, h = p->h;
+    const long n = (long)w * h;
+    const float rho0 = (float)p->rho0;
+    float B[9];
+    for (int r = 0; r < h; ++r)
+        for (int c = 0; c < w; ++c) {
+            const long idx = (long)r * w + c;
+            const int t = flags ? flags[idx] : ORC_FLUID;
+            const int solid = is_solid(t);
+            if (p->boundary == ORC_BB_HALFWAY) {
+                if (solid) {
+                    for (int i = 0; i < 9; ++i) out[i * n + idx] = G[i * n + idx];
+                    continue;
+                }
+                for (int i = 0; i < 9; ++i) {
+                    int sr = r + CX[i], sc = c - CY[i];
+                    if (p->edge == ORC_EDGE_PERIODIC) {
+                        sr = (sr % h + h) % h;
+                        sc = (sc % w + w) % w;
+                    }
+                    int inside = sr >= 0 && sr < h && sc >= 0 && sc < w;
+                    int st = (inside && flags) ? flags[(long)sr * w + sc] : ORC_FLUID;
+                    if (is_solid(st)) {
+                        float v = G[OPP[i] * n + idx];
+                        if (st == ORC_LID) v = v + k.lid[i];
+                        B[i] = v;
+                    } else {
+                        B[i] = inside ? G[i * n + (long)sr * w + sc] : -k.base[i];
+                    }
+                }
+                if (t == ORC_INLET) zou_he_inlet_dev(B, rho0, (float)p->inlet_ux, (float)p->inlet_uy);
+                else if (t == ORC_OUTLET) zou_he_outlet_dev(B, rho0, (float)p->outlet_rho);
+            } else {
+                for (int i = 0; i < 9; ++i) {
+                    int j = solid ? OPP[i] : i;
+                    B[i] = fetch_dev(p, &k, G, i, j, r + CX[j], c - CY[j]);
+                }
+            }
+            collide_dev(&k, rho0, B);
+            for (int i = 0; i < 9; ++i) out[i * n + idx] = B[i];
+        }
+}
+
+/* Deviation-form equilibrium g_eq_i = w_i (drho + rho s'_i), drho = rho - rho0. */
+void orc_equilibrium_devf32(const orc_params* p, int n, const float* rho, const float* ux,
+                            const float* uy, float* G)
+{
+    orc_consts_f32 k;
+    orc_consts_f32_make(p, &k);
+    const float rho0 = (float)p->rho0;
+    for (long j = 0; j < n; ++j) {
+        float r = rho[j], vx = ux[j], vy = uy[j];
+        float drho = r - rho0;
+        float v2 = vx * vx + vy * vy;
+        const float vc[9] = {0.0f, vx, vy, -vx, -vy, vx + vy, -vx + vy, -vx - vy, vx - vy};
+        for (int i = 0; i < 9; ++i) {
+            float s = (vc[i] * k.A + (vc[i] * vc[i]) * k.B) + v2 * k.C;
+            G[(long)i * n + j] = k.w[i] * (drho + r * s);
+        }
+    }
+}
+
+int orc_run_devf32(const orc_params* p, const uint8_t* flags, float* g, int nsteps)
+{
+    const long n = 9L * p->w * p->h;
+    float* tmp = (float*)malloc(sizeof(float) * n);
+    if (!tmp) return -1;
+    for (int s = 0; s < nsteps; ++s) {
+        orc_step_devf32(p, flags, g, tmp);
+        memcpy(g, tmp, sizeof(float) * n);
+    }
+    free(tmp);
+    return 0;
+}
+
+/* One 16-bit step: widen the stored deviations to FP32 (exact), run the
+ * deviation-form FP32 step, narrow with round to nearest even.  Solid cells
+ * of the half-way mode keep their stored bits. */
 void orc_step_dev16(const orc_params* p, int kind, const uint8_t* flags, const uint16_t* P,
                     uint16_t* out)
 {
     const long n = (long)p->w * p->h;
-    float* full = (float*)malloc(sizeof(float) * 9 * n);
+    float* g = (float*)malloc(sizeof(float) * 9 * n);
     float* o = (float*)malloc(sizeof(float) * 9 * n);
-    orc_decode_dev16(p, kind, P, full);
-    orc_step_f32(p, flags, full, o);
-    orc_encode_dev16(p, kind, o, out);
+    for (long j = 0; j < 9 * n; ++j) g[j] = dev_load(kind, P[j]);
+    orc_step_devf32(p, flags, g, o);
+    for (long j = 0; j < 9 * n; ++j) out[j] = dev_store(kind, o[j]);
     if (p->boundary == ORC_BB_HALFWAY && flags) {
         for (long j = 0; j < n; ++j)
             if (is_solid(flags[j]))
                 for (int i = 0; i < 9; ++i) out[i * n + j] = P[i * n + j];
     }
-    free(full);
+    free(g);
     free(o);
 }
 

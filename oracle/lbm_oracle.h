@@ -112,7 +112,8 @@ int orc_run_unfused_f32(const orc_params* p, const uint8_t* flags, float* f, int
 
 /* ---- 16-bit deviation storage (extension) ------------------------------
  * Stored value d_i = f_i - w_i*rho0 in BF16 or FP16 (round to nearest even);
- * the step widens to FP32, runs orc_step_f32's arithmetic, and narrows. */
+ * the step widens to FP32, runs the deviation-form step (orc_step_devf32)
+ * and narrows. */
 uint16_t orc_f32_to_bf16(float x);
 float    orc_bf16_to_f32(uint16_t b);
 uint16_t orc_f32_to_f16(float x);
@@ -122,6 +123,11 @@ void orc_f32_to_f16_n(const float* x, uint16_t* y, long n);
 void orc_step_dev16(const orc_params* p, int kind, const uint8_t* flags, const uint16_t* P,
                     uint16_t* out);
 int orc_run_dev16(const orc_params* p, int kind, const uint8_t* flags, uint16_t* d, int nsteps);
+/* Deviation arithmetic on f32 deviations g_i = f_i - w_i*rho0 (extension). */
+void orc_step_devf32(const orc_params* p, const uint8_t* flags, const float* G, float* out);
+int orc_run_devf32(const orc_params* p, const uint8_t* flags, float* g, int nsteps);
+void orc_equilibrium_devf32(const orc_params* p, int n, const float* rho, const float* ux,
+                            const float* uy, float* G);
 /* Encode/decode between full populations (f32) and stored deviations. */
 void orc_encode_dev16(const orc_params* p, int kind, const float* f, uint16_t* d);
 void orc_decode_dev16(const orc_params* p, int kind, const uint16_t* d, float* f);

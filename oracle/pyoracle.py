@@ -71,6 +71,9 @@ for name, args, res in [
     ("orc_run_f64", [_PP, _P, _P, C.c_int], C.c_int),
     ("orc_run_f32_omp", [_PP, _P, _P, C.c_int, C.c_int], C.c_int),
     ("orc_run_unfused_f32", [_PP, _P, _P, C.c_int], C.c_int),
+    ("orc_step_devf32", [_PP, _P, _P, _P], None),
+    ("orc_run_devf32", [_PP, _P, _P, C.c_int], C.c_int),
+    ("orc_equilibrium_devf32", [_PP, C.c_int, _P, _P, _P, _P], None),
     ("orc_f32_to_bf16_n", [_P, _P, C.c_long], None),
     ("orc_f32_to_f16_n", [_P, _P, C.c_long], None),
     ("orc_step_dev16", [_PP, C.c_int, _P, _P, _P], None),
@@ -238,6 +241,38 @@ def decode_dev16(p: Params, kind, d):
     out = np.empty(d.shape, np.float32)
     lib.orc_decode_dev16(C.byref(p), int(kind), _p(d), _p(out))
     return out
+
+
+def step_devf32(p: Params, g, flags=None):
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    out = np.empty_like(g)
+    lib.orc_step_devf32(C.byref(p), _p(_flags(flags, p)), _p(g), _p(out))
+    return out
+
+
+def run_devf32(p: Params, g, nsteps, flags=None):
+    g = np.array(g, copy=True, order="C", dtype=np.float32)
+    assert lib.orc_run_devf32(C.byref(p), _p(_flags(flags, p)), _p(g), int(nsteps)) == 0
+    return g
+
+
+def equilibrium_dev(p: Params, rho, ux, uy):
+    """Deviation-form equilibrium g_eq = f_eq - w_i rho0 in f32."""
+    rho, ux, uy = (np.ascontiguousarray(a, dtype=np.float32) for a in (rho, ux, uy))
+    out = np.empty((9,) + rho.shape, np.float32)
+    lib.orc_equilibrium_devf32(C.byref(p), int(rho.size), _p(rho), _p(ux), _p(uy), _p(out))
+    return out
+
+
+def to_dev(p: Params, f):
+    """f32 deviations g_i = f_i - w_i rho0 of full populations."""
+    k = consts_f32(p)
+    return (np.asarray(f, np.float32) - np.array(k.base, np.float32)[:, None, None]).astype(np.float32)
+
+
+def from_dev(p: Params, g):
+    k = consts_f32(p)
+    return (np.asarray(g, np.float32) + np.array(k.base, np.float32)[:, None, None]).astype(np.float32)
 
 
 def run_dev16(p: Params, kind, d, nsteps, flags=None):

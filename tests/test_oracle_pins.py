@@ -183,6 +183,13 @@ def test_bf16_conversion_matches_torch():
     np.testing.assert_array_equal(O.to_bf16(x), ref)
 
 
+def widen16(kind, d):
+    d = np.asarray(d, np.uint16)
+    if kind == O.BF16:
+        return (d.astype(np.uint32) << 16).view(np.float32)
+    return d.view(np.float16).astype(np.float32)
+
+
 def test_dev16_roundtrip_and_step():
     p = O.params(8, 6, tau=0.8, edge=O.EDGE_PERIODIC)
     rng = np.random.default_rng(10)
@@ -190,13 +197,42 @@ def test_dev16_roundtrip_and_step():
     ux = (0.05 * rng.standard_normal((6, 8))).astype(np.float32)
     uy = (0.05 * rng.standard_normal((6, 8))).astype(np.float32)
     f = O.equilibrium(p, rho, ux, uy)
-    for kind in (O.BF16, O.F16):
+    k = O.consts_f32(p)
+    dev = f - np.array(k.base, np.float32)[:, None, None]
+    for kind, narrow, tol in ((O.BF16, O.to_bf16, 2 ** -8), (O.F16, O.to_f16, 2 ** -11)):
         d = O.encode_dev16(p, kind, f)
+        np.testing.assert_array_equal(d, narrow(dev))
         back = O.decode_dev16(p, kind, d)
-        tol = 2 ** -8 if kind == O.BF16 else 2 ** -11
-        k = O.consts_f32(p)
-        dev = f - np.array(k.base, np.float32)[:, None, None]
         assert np.max(np.abs(back - f)) <= tol * np.max(np.abs(dev)) + 1e-12
-        # the 16-bit step is decode -> f32 step -> encode
+        # the 16-bit step is widen -> deviation-form f32 step -> narrow
         np.testing.assert_array_equal(O.run_dev16(p, kind, d, 1),
-                                      O.encode_dev16(p, kind, O.step(p, O.decode_dev16(p, kind, d))))
+                                      narrow(O.step_devf32(p, widen16(kind, d))))
+
+
+@pytest.mark.parametrize("boundary", [O.BB_REFERENCE, O.BB_HALFWAY])
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+def test_deviation_form_is_the_same_step_more_precisely(boundary, edge):
+    """The deviation-form step computes f' - w_i rho0 of the same update: against
+    the FP64 step it is at least as accurate as the full-form FP32 step."""
+    rng = np.random.default_rng(21)
+    h, w = 14, 18
+    rho = 1 + 0.01 * rng.standard_normal((h, w))
+    ux = 0.05 * rng.standard_normal((h, w))
+    uy = 0.05 * rng.standard_normal((h, w))
+    flags = (rng.random((h, w)) < 0.15).astype(np.uint8)
+    flags[rng.random((h, w)) < 0.05] = O.LID
+    p = O.params(w, h, tau=0.7, edge=edge, boundary=boundary, lid=(0.01, 0.04))
+    f64 = O.equilibrium(p, rho, ux, uy) + 1e-3 * rng.standard_normal((9, h, w))
+    f32 = f64.astype(np.float32)
+    # zero-filled edges drain mass (the reference's af::convolve2 padding), so
+    # keep that case short
+    steps = 20 if edge == O.EDGE_PERIODIC else 2
+    ref = O.run(p, f32.astype(np.float64), steps, flags)
+    full = O.run(p, f32, steps, flags).astype(np.float64)
+    dev = O.from_dev(p, O.run_devf32(p, O.to_dev(p, f32), steps, flags)).astype(np.float64)
+    base = np.array(O.consts_f64(p).base)[:, None, None]
+    scale = np.abs(ref - base).max()
+    err_full = np.abs(full - ref).max() / scale
+    err_dev = np.abs(dev - ref).max() / scale
+    assert err_dev < 1e-5 and err_full < 1e-3
+    assert err_dev <= err_full * 1.5
