@@ -234,6 +234,21 @@ void orc_decode_dev16(const orc_params* p, int kind, const uint16_t* d, float* f
  * the O(1) rest populations through the rounding.  Streaming, bounce-back and
  * the Ladd term act on g unchanged (opposite directions share w_i); a zero-
  * filled population outside the lattice is g = -w_i rho0. */
+/* Constants of the deviation path: those of orc_consts_f32_make except A, B, C,
+ * which are evaluated in f64 and rounded once (exactly 3, 4.5, -1.5 for
+ * dx = dt = 1), so that sum_i g_eq_i = drho and sum_i g_eq_i c_i = rho u hold
+ * up to rounding. */
+void orc_consts_devf32_make(const orc_params* p, orc_consts_f32* k)
+{
+    orc_consts_f64 d;
+    orc_consts_f32_make(p, k);
+    orc_consts_f64_make(p, &d);
+    k->A = (float)d.A;
+    k->B = (float)d.B;
+    k->C = (float)d.C;
+    k->cs2 = (float)d.cs2;
+}
+
 static inline void collide_dev(const orc_consts_f32* k, float rho0, float* B)
 {
     float drho = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
@@ -289,7 +304,7 @@ static inline float fetch_dev(const orc_params* p, const orc_consts_f32* k, cons
 void orc_step_devf32(const orc_params* p, const uint8_t* flags, const float* G, float* out)
 {
     orc_consts_f32 k;
-    orc_consts_f32_make(p, &k);
+    orc_consts_devf32_make(p, &k);
     const int w = p->w, h = p->h;
     const long n = (long)w * h;
     const float rho0 = (float)p->rho0;
@@ -338,7 +353,7 @@ void orc_equilibrium_devf32(const orc_params* p, int n, const float* rho, const 
                             const float* uy, float* G)
 {
     orc_consts_f32 k;
-    orc_consts_f32_make(p, &k);
+    orc_consts_devf32_make(p, &k);
     const float rho0 = (float)p->rho0;
     for (long j = 0; j < n; ++j) {
         float r = rho[j], vx = ux[j], vy = uy[j];

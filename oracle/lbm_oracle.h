@@ -62,6 +62,9 @@ typedef struct orc_consts_f64 {
 
 void orc_consts_f32_make(const orc_params* p, orc_consts_f32* out);
 void orc_consts_f64_make(const orc_params* p, orc_consts_f64* out);
+/* Deviation-path constants: as orc_consts_f32_make, with A, B, C, cs2 taken
+ * from the f64 evaluation rounded once (extension). */
+void orc_consts_devf32_make(const orc_params* p, orc_consts_f32* out);
 
 /* ---- Reference operations, one pass each, as lbm.rs runs them ---------- */
 /* State::stream (lbm.rs:716-729): Q_i[r][c] = P_i[r+cx_i][c-cy_i], zero outside

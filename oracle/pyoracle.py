@@ -49,6 +49,7 @@ _PP = C.POINTER(Params)
 for name, args, res in [
     ("orc_consts_f32_make", [_PP, C.POINTER(Consts32)], None),
     ("orc_consts_f64_make", [_PP, C.POINTER(Consts64)], None),
+    ("orc_consts_devf32_make", [_PP, C.POINTER(Consts32)], None),
     ("orc_stream_f32", [C.c_int, C.c_int, C.c_int, _P, _P], None),
     ("orc_stream_f64", [C.c_int, C.c_int, C.c_int, _P, _P], None),
     ("orc_bounce_back_f32", [C.c_int, C.c_int, _P, _P, _P], None),
@@ -102,6 +103,12 @@ def params(w, h, tau=1.0, edge=EDGE_ZERO, boundary=BB_REFERENCE, lid=(0.0, 0.0),
 def consts_f32(p: Params) -> Consts32:
     k = Consts32()
     lib.orc_consts_f32_make(C.byref(p), C.byref(k))
+    return k
+
+
+def consts_devf32(p: Params) -> Consts32:
+    k = Consts32()
+    lib.orc_consts_devf32_make(C.byref(p), C.byref(k))
     return k
 
 

@@ -41,6 +41,13 @@ def test_equilibrium_constants_bit_patterns():
     assert np.float32(k8.K) == np.float32(-1.25)
 
 
+def test_deviation_constants_are_exact():
+    k = O.consts_devf32(O.params(4, 4, tau=0.6))
+    assert (k.A, k.B, k.C) == (3.0, 4.5, -1.5)
+    k32 = O.consts_f32(O.params(4, 4, tau=0.6))
+    assert k.K == k32.K and list(k.w) == list(k32.w)
+
+
 def test_f64_constants():
     k = O.consts_f64(O.params(4, 4, tau=0.8))
     assert abs(k.A - 3.0) < 1e-15 and abs(k.B - 4.5) < 1e-14 and abs(k.C + 1.5) < 1e-15
