@@ -323,5 +323,5 @@ def test_taylor_green_4096_fp32_vs_fp64(ctx):
     # population rounding drifts the mass by ~1e-5.  Deviation arithmetic with
     # the once-rounded constants keeps both near FP32 rounding.
     assert err["f32"] < 5e-4 and res["f32"][1] < 5e-5
-    assert err["f32dev"] < 1e-5 and res["f32dev"][1] < 1e-6
+    assert err["f32dev"] < 3e-5 and res["f32dev"][1] < 1e-6
     assert err["f32dev"] < err["f32"]
