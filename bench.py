@@ -27,18 +27,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "D2Q9 GLUPS & % HBM roofline; time-to-steady-state, warm-start vs cold"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
-BYTES_PER_CELL = {"f32": 72, "f64": 144, "bf16": 36, "f16": 36}
+BYTES_PER_CELL = {"f32": 72, "f32dev": 72, "f64": 144, "bf16": 36, "f16": 36}
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def tgv_lattice(ctx, n, storage, tau, u0):
+def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr"):
     from chemsim_amd import Lattice, Params, capi
     from chemsim_amd import workloads as W
     rho, ux, uy = W.taylor_green(n, u0, np.float64 if storage == "f64" else np.float32)
-    lat = Lattice(ctx, n, n, storage, Params(tau=tau, edge=capi.CS_EDGE_PERIODIC))
+    rnd = capi.CS_ROUND_STOCHASTIC if rounding == "sr" else capi.CS_ROUND_NEAREST
+    lat = Lattice(ctx, n, n, storage, Params(tau=tau, edge=capi.CS_EDGE_PERIODIC, rounding=rnd, seed=2024))
     lat.init_equilibrium(rho, ux, uy)
     return lat
 
@@ -89,7 +90,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f64", "bf16", "f16"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f64", "bf16", "f16", "f32dev"])
     ap.add_argument("--tau", type=float, default=0.8)
     ap.add_argument("--u0", type=float, default=0.05)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -148,20 +149,20 @@ def main():
         del rho0, ux0, uy0
         den = np.sqrt((x64 ** 2 + y64 ** 2).sum())
         rden = np.sqrt(((r64 - 1.0) ** 2).sum())
-        for st in ("f32", "bf16", "f16"):
-            l2 = tgv_lattice(ctx, n, st, args.tau, args.u0)
+        for st, rnd in (("f32", "rn"), ("f32dev", "rn"), ("bf16", "sr"), ("f16", "sr"), ("bf16", "rn"),
+                        ("f16", "rn")):
+            l2 = tgv_lattice(ctx, n, st, args.tau, args.u0, rnd)
             m0 = l2.sums_f64()[0]
-            if args.warmup:
-                pass
             w2, ms2 = time_steps(l2, args.steps)
             r, x, y = l2.moments_f64()
             m1 = l2.sums_f64()[0]
             l2.close()
-            extras[st] = {"GLUPS": round(cells * args.steps / w2 / 1e9, 3),
+            key = st if st in ("f32", "f32dev") else f"{st}_{rnd}"
+            extras[key] = {"GLUPS": round(cells * args.steps / w2 / 1e9, 3),
                           "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
                           "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden),
                           "mass_drift_rel": abs(m1 - m0) / m0}
-            log(f"[bench] {st}: {extras[st]}")
+            log(f"[bench] {key}: {extras[key]}")
         del x64, y64, r64
 
     cpu = None
@@ -183,7 +184,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f32": "f32", "f64": "f64", "bf16": "bf16-storage/f32-math", "f16": "f16-storage/f32-math"}[storage],
+        "dtype": {"f32": "f32", "f32dev": "f32", "f64": "f64", "bf16": "bf16-storage/f32-math",
+                  "f16": "f16-storage/f32-math"}[storage],
         "data": "synthetic (analytic Taylor-Green initial state)",
         "config": {"workload": f"D2Q9 BGK periodic Taylor-Green {n}x{n}, tau={args.tau}, u0={args.u0}, "
                                f"k=2pi/{n}, {storage} storage",

@@ -19,6 +19,7 @@ CS_F32, CS_F64, CS_BF16DEV, CS_F16DEV, CS_F32DEV = range(5)
 CS_EDGE_ZERO, CS_EDGE_PERIODIC = range(2)
 CS_BB_REFERENCE, CS_BB_HALFWAY = range(2)
 CS_FLUID, CS_WALL, CS_LID, CS_INLET, CS_OUTLET = range(5)
+CS_ROUND_NEAREST, CS_ROUND_STOCHASTIC = range(2)
 
 
 class cs_params(C.Structure):
@@ -34,6 +35,8 @@ class cs_params(C.Structure):
         ("inlet_uy", C.c_double),
         ("outlet_rho", C.c_double),
         ("rho0", C.c_double),
+        ("rounding", C.c_int32),
+        ("seed", C.c_uint32),
     ]
 
 

@@ -115,4 +115,5 @@ struct cs_lattice {
     double* dscratch = nullptr; // small device scratch for reduction results
     int* iscratch = nullptr;
     double time = 0.0;
+    uint64_t steps = 0;         // fused steps taken (keys the stochastic rounding)
 };

@@ -458,10 +458,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_residual(const typename St::S* 
     block_reduce_store<2>(v, partials);
 }
 
-__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n4)
+// Plain copy: each block moves 1024 16-byte vectors (16 KiB), 4 per thread,
+// stride 256.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(const f32x4* __restrict__ src, f32x4* __restrict__ dst, int64_t n4)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    f32x4 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t i = base + 256 * j;
+        if (i < n4) v[j] = src[i];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t i = base + 256 * j;
+        if (i < n4) __builtin_nontemporal_store(v[j], dst + i);
+    }
 }
 
 // --------------------------------------------------------- host helpers ---
@@ -502,7 +515,7 @@ struct DevBuf {
 
 static int g_cpt_override = 0;
 
-template <class St, int EDGE, int MODE, int CPT>
+template <class St, int EDGE, int MODE, int CPT, bool SR>
 static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
 {
     StepArgs<typename St::T> a;
@@ -514,20 +527,34 @@ static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
     a.pitch = l->pitch;
     a.plane = l->plane;
     a.tiles_x = (l->w + 64 * CPT - 1) / (64 * CPT);
+    a.sr_key = sr_key(l->params.seed, l->steps);
+    a.cell_base = 0;
     a.k = consts_of<St>(l);
     const int tiles_y = (l->h + 3) / 4;
     const int nblocks = a.tiles_x * tiles_y;
-    hipLaunchKernelGGL((k_step<St, EDGE, MODE, CPT>), dim3(nblocks), dim3(256), 0, l->stream, a);
+    hipLaunchKernelGGL((k_step<St, EDGE, MODE, CPT, SR>), dim3(nblocks), dim3(256), 0, l->stream, a);
+}
+
+template <class St, int EDGE, int MODE, bool SR>
+static void launch_step_sr(cs_lattice* l, const void* fin, void* fout)
+{
+    int cpt = g_cpt_override;
+    if (cpt == 0) cpt = (l->storage == CS_BF16DEV || l->storage == CS_F16DEV) ? 4 : 1;
+    if (cpt >= 4) launch_step_cpt<St, EDGE, MODE, 4, SR>(l, fin, fout);
+    else if (cpt == 2) launch_step_cpt<St, EDGE, MODE, 2, SR>(l, fin, fout);
+    else launch_step_cpt<St, EDGE, MODE, 1, SR>(l, fin, fout);
 }
 
 template <class St, int EDGE, int MODE>
 static void launch_step_mode(cs_lattice* l, const void* fin, void* fout)
 {
-    int cpt = g_cpt_override;
-    if (cpt == 0) cpt = (l->storage == CS_F64) ? 1 : 2;
-    if (cpt >= 4) launch_step_cpt<St, EDGE, MODE, 4>(l, fin, fout);
-    else if (cpt == 2) launch_step_cpt<St, EDGE, MODE, 2>(l, fin, fout);
-    else launch_step_cpt<St, EDGE, MODE, 1>(l, fin, fout);
+    if constexpr (is_16bit<St>()) {
+        if (l->params.rounding == CS_ROUND_STOCHASTIC) {
+            launch_step_sr<St, EDGE, MODE, true>(l, fin, fout);
+            return;
+        }
+    }
+    launch_step_sr<St, EDGE, MODE, false>(l, fin, fout);
 }
 
 template <class St>
@@ -548,6 +575,7 @@ static void launch_step(cs_lattice* l)
         else launch_step_mode<St, CS_EDGE_ZERO, MODE_HALF>(l, fin, fout);
     }
     l->cur ^= 1;
+    l->steps += 1;
 }
 
 static void do_steps(cs_lattice* l, int n)
@@ -748,6 +776,8 @@ extern "C" int cs_lattice_set_params(cs_lattice* l, const cs_params* p)
         CS_REQUIRE(p->boundary == CS_BB_REFERENCE || p->boundary == CS_BB_HALFWAY, CS_ERR_ARG, "bad boundary mode");
         CS_REQUIRE(p->tau > 0 && p->delta_x > 0 && p->delta_t > 0, CS_ERR_ARG, "tau, delta_x, delta_t must be > 0");
         CS_REQUIRE(p->outlet_rho > 0, CS_ERR_ARG, "outlet_rho must be > 0");
+        CS_REQUIRE(p->rounding == CS_ROUND_NEAREST || p->rounding == CS_ROUND_STOCHASTIC, CS_ERR_ARG,
+                   "bad rounding mode");
         const bool edge_changed = p->edge != l->params.edge;
         // Re-encode 16-bit storage when the deviation base changes.
         if ((l->storage == CS_F32DEV || l->storage == CS_BF16DEV || l->storage == CS_F16DEV) &&
@@ -1236,11 +1266,11 @@ extern "C" int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_pe
         hipEvent_t e0, e1;
         CS_HIP(hipEventCreate(&e0));
         CS_HIP(hipEventCreate(&e1));
-        const int blocks = ctx->prop.multiProcessorCount * 16;
-        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const float4*)a.p, (float4*)b.p, n4);
+        const int blocks = (int)((n4 + 1023) / 1024);
+        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const f32x4*)a.p, (f32x4*)b.p, n4);
         CS_HIP(hipEventRecord(e0, s));
         for (int i = 0; i < iters; ++i)
-            hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const float4*)a.p, (float4*)b.p, n4);
+            hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const f32x4*)a.p, (f32x4*)b.p, n4);
         CS_HIP(hipEventRecord(e1, s));
         CS_HIP(hipEventSynchronize(e1));
         float ms = 0;

@@ -34,6 +34,43 @@ __device__ __forceinline__ uint16_t f32_to_bf16_bits(float x)
 __device__ __forceinline__ float f16_bits_to_f32(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
 __device__ __forceinline__ uint16_t f32_to_f16_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 
+// Stochastic rounding with gfx950's v_cvt_sr_{bf16,f16}_f32: the magnitude
+// plus random bits below the kept mantissa (top 16 bits of r for bf16, top 13
+// for f16), truncated.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint16_t f32_to_bf16_sr_bits(float x, uint32_t r)
+{
+    const bf16x2_t v = __builtin_amdgcn_cvt_sr_bf16_f32(bf16x2_t{}, x, r, false);
+    return __builtin_bit_cast(uint16_t, v[0]);
+}
+__device__ __forceinline__ uint16_t f32_to_f16_sr_bits(float x, uint32_t r)
+{
+    const f16x2_t v = __builtin_amdgcn_cvt_sr_f16_f32(f16x2_t{}, x, r, false);
+    return __builtin_bit_cast(uint16_t, v[0]);
+}
+
+// Random bits of the stochastic rounding: h = lowbias32(cell ^ key), then
+// xorshift32; each 32-bit word serves two directions (word, word << 16).
+__host__ __device__ __forceinline__ uint32_t lowbias32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t xorshift32(uint32_t x)
+{
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+// Per-step key, computed on the host: lowbias32(seed ^ step * phi).
+inline uint32_t sr_key(uint32_t seed, uint64_t step) { return lowbias32(seed ^ ((uint32_t)step * 0x9e3779b9u)); }
+
 // S: stored element type.  T: arithmetic type.  DEV: the stored value is the
 // deviation g_i = f_i - w_i*rho0 and the step runs in deviation arithmetic.
 // ld() widens a stored value, st() narrows (round to nearest even).
@@ -64,6 +101,10 @@ struct StBF16 {
     static constexpr bool DEV = true;
     __device__ static __forceinline__ T ld(const S* p, int64_t o) { return bf16_bits_to_f32(p[o]); }
     __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(f32_to_bf16_bits(v), p + o); }
+    __device__ static __forceinline__ void st_sr(S* p, int64_t o, T v, uint32_t r)
+    {
+        __builtin_nontemporal_store(f32_to_bf16_sr_bits(v, r), p + o);
+    }
 };
 struct StF16 {
     using S = uint16_t;
@@ -71,7 +112,13 @@ struct StF16 {
     static constexpr bool DEV = true;
     __device__ static __forceinline__ T ld(const S* p, int64_t o) { return f16_bits_to_f32(p[o]); }
     __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(f32_to_f16_bits(v), p + o); }
+    __device__ static __forceinline__ void st_sr(S* p, int64_t o, T v, uint32_t r)
+    {
+        __builtin_nontemporal_store(f32_to_f16_sr_bits(v, r), p + o);
+    }
 };
+template <class St>
+constexpr bool is_16bit() { return sizeof(typename St::S) == 2; }
 
 // Boundary handling variants of the step kernel.
 enum StepMode : int {
@@ -88,6 +135,8 @@ struct StepArgs {
     int w, h, pitch;
     int tiles_x;
     int64_t plane;
+    uint32_t sr_key;     // stochastic rounding key of this step
+    uint32_t cell_base;  // added to r*w + c to index the random bits
     Consts<T> k;
 };
 
@@ -290,11 +339,12 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
 }
 
 // Collides and stores cell (r, c).  Solid cells of the half-way mode are
-// carried unchanged: their stored values are copied bit for bit.
-template <class St, int MODE>
+// carried unchanged: their stored values are copied bit for bit.  With SR the
+// 16-bit narrowing is stochastic, keyed by (sr_key, cell index).
+template <class St, int MODE, bool SR>
 __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, typename St::S* __restrict__ fout,
                                        int pitch, int64_t P, const Consts<typename St::T>& k, int r, int c,
-                                       int type, typename St::T (&B)[9])
+                                       int type, typename St::T (&B)[9], uint32_t sr_key, uint32_t cell)
 {
     const int64_t own = (int64_t)r * pitch + c;
     if (MODE == MODE_HALF && (type == CS_WALL || type == CS_LID)) {
@@ -304,14 +354,25 @@ __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, t
     }
     if constexpr (St::DEV) bgk_collide_dev(k, B);
     else bgk_collide(k, B);
+    if constexpr (SR) {
+        uint32_t h = lowbias32(cell ^ sr_key);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) St::st(fout, (int64_t)i * P + own, B[i]);
+        for (int j = 0; j < 4; ++j) {
+            St::st_sr(fout, (int64_t)(2 * j) * P + own, B[2 * j], h);
+            St::st_sr(fout, (int64_t)(2 * j + 1) * P + own, B[2 * j + 1], h << 16);
+            h = xorshift32(h);
+        }
+        St::st_sr(fout, (int64_t)8 * P + own, B[8], h);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) St::st(fout, (int64_t)i * P + own, B[i]);
+    }
 }
 
 // Fused step.  A block is 4 waves; wave y handles row 4*ty + y, lane l the
 // columns 64*CPT*tx + l + 64*j, j < CPT.  All loads of a thread's CPT cells
 // are issued before any arithmetic.
-template <class St, int EDGE, int MODE, int CPT>
+template <class St, int EDGE, int MODE, int CPT, bool SR>
 __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
 {
     using S = typename St::S;
@@ -335,7 +396,9 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int c = c0 + 64 * j;
-        if (c < a.w) finish<St, MODE>(fin, fout, a.pitch, a.plane, a.k, r, c, type[j], B[j]);
+        if (c < a.w)
+            finish<St, MODE, SR>(fin, fout, a.pitch, a.plane, a.k, r, c, type[j], B[j], a.sr_key,
+                                 a.cell_base + (uint32_t)(r * a.w + c));
     }
 }
 

@@ -38,11 +38,13 @@ class Params:
     inlet_uy: float = 0.0
     outlet_rho: float = 1.0
     rho0: float = 1.0
+    rounding: int = capi.CS_ROUND_NEAREST
+    seed: int = 0
 
     def to_c(self) -> capi.cs_params:
         return capi.cs_params(self.tau, self.delta_x, self.delta_t, int(self.edge), int(self.boundary),
                               self.lid_ux, self.lid_uy, self.inlet_ux, self.inlet_uy, self.outlet_rho,
-                              self.rho0)
+                              self.rho0, int(self.rounding), int(self.seed) & 0xffffffff)
 
 
 class Context:

@@ -89,8 +89,17 @@ typedef struct cs_params {
     double inlet_ux;   /* Zou-He inlet velocity (extension) */
     double inlet_uy;
     double outlet_rho; /* Zou-He outlet density (extension) */
-    double rho0;       /* base density of 16-bit deviation storage (extension) */
+    double rho0;       /* base density of deviation storage (extension) */
+    int32_t rounding;  /* 16-bit storage: CS_ROUND_NEAREST or CS_ROUND_STOCHASTIC */
+    uint32_t seed;     /* seed of the stochastic rounding */
 } cs_params;
+
+/* Narrowing of the 16-bit storages.  STOCHASTIC rounds each stored value up
+ * with probability equal to its fractional position between the two
+ * neighbouring 16-bit values (gfx950 v_cvt_sr_*), using random bits keyed by
+ * (seed, step index, cell index r*w + c); it removes the bias of round-to-
+ * nearest on fields that change by less than an ulp per step. */
+enum cs_rounding { CS_ROUND_NEAREST = 0, CS_ROUND_STOCHASTIC = 1 };
 
 /* ---- errors, context -------------------------------------------------- */
 const char* cs_last_error(void);

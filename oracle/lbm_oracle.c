@@ -194,6 +194,70 @@ float orc_f16_to_f32(uint16_t b)
     return bits_f32(sign | ((e + 112u) << 23) | (m << 13));
 }
 
+/* Stochastic rounding (extension): the magnitude plus random bits below the
+ * kept mantissa, truncated.  BF16 takes the top 16 bits of r; FP16 the top 13
+ * (shifted further for half subnormals).  This is the rounding of gfx950's
+ * v_cvt_sr_bf16_f32 / v_cvt_sr_f16_f32, established by comparing against the
+ * instructions' outputs on 2^20 (x, r) pairs. */
+uint16_t orc_f32_to_bf16_sr(float x, uint32_t r)
+{
+    uint32_t u = f32_bits(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
+    return (uint16_t)((u + (r >> 16)) >> 16);
+}
+
+uint16_t orc_f32_to_f16_sr(float x, uint32_t r)
+{
+    uint32_t u = f32_bits(x);
+    uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+    uint32_t a = u & 0x7fffffffu;
+    uint32_t rnd = r >> 19;
+    if (a > 0x7f800000u) return sign | 0x7e00u;
+    if (a >= 0x38800000u) {
+        uint32_t v = a + rnd;
+        int e = (int)(v >> 23) - 112;
+        if (e >= 31) return sign | 0x7c00u;
+        return sign | (uint16_t)(((uint32_t)e << 10) | ((v >> 13) & 0x3ffu));
+    }
+    if (a < 0x00800000u) return sign;
+    uint64_t mant = (a & 0x7fffffu) | 0x800000u;
+    int shift = -((int)(a >> 23) - 127) - 1;  /* >= 14 */
+    if (shift > 40) return sign;
+    uint64_t m = (mant + ((uint64_t)rnd << (shift - 13))) >> shift;
+    return sign | (uint16_t)m;
+}
+
+/* Random bits of the stochastic rounding: key = lowbias32(seed ^ step * phi),
+ * h = lowbias32(cell ^ key), then xorshift32; each 32-bit word serves two
+ * directions (the word and the word shifted left by 16). */
+static inline uint32_t lowbias32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+static inline uint32_t xorshift32(uint32_t x)
+{
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+uint32_t orc_sr_key(uint32_t seed, uint32_t step) { return lowbias32(seed ^ (step * 0x9e3779b9u)); }
+void orc_sr_bits(uint32_t key, uint32_t cell, uint32_t R[9])
+{
+    uint32_t h = lowbias32(cell ^ key);
+    for (int j = 0; j < 4; ++j) {
+        R[2 * j] = h;
+        R[2 * j + 1] = h << 16;
+        h = xorshift32(h);
+    }
+    R[8] = h;
+}
+
 void orc_f32_to_bf16_n(const float* x, uint16_t* y, long n)
 {
     for (long k = 0; k < n; ++k) y[k] = orc_f32_to_bf16(x[k]);
@@ -381,17 +445,29 @@ int orc_run_devf32(const orc_params* p, const uint8_t* flags, float* g, int nste
 }
 
 /* One 16-bit step: widen the stored deviations to FP32 (exact), run the
- * deviation-form FP32 step, narrow with round to nearest even.  Solid cells
+ * deviation-form FP32 step, narrow with round to nearest even (rounding 0) or
+ * stochastic rounding keyed by (seed, step, cell) (rounding 1).  Solid cells
  * of the half-way mode keep their stored bits. */
-void orc_step_dev16(const orc_params* p, int kind, const uint8_t* flags, const uint16_t* P,
-                    uint16_t* out)
+void orc_step_dev16_ex(const orc_params* p, int kind, int rounding, uint32_t seed, uint32_t step,
+                       const uint8_t* flags, const uint16_t* P, uint16_t* out)
 {
     const long n = (long)p->w * p->h;
     float* g = (float*)malloc(sizeof(float) * 9 * n);
     float* o = (float*)malloc(sizeof(float) * 9 * n);
     for (long j = 0; j < 9 * n; ++j) g[j] = dev_load(kind, P[j]);
     orc_step_devf32(p, flags, g, o);
-    for (long j = 0; j < 9 * n; ++j) out[j] = dev_store(kind, o[j]);
+    if (rounding == 0) {
+        for (long j = 0; j < 9 * n; ++j) out[j] = dev_store(kind, o[j]);
+    } else {
+        const uint32_t key = orc_sr_key(seed, step);
+        uint32_t R[9];
+        for (long c = 0; c < n; ++c) {
+            orc_sr_bits(key, (uint32_t)c, R);
+            for (int i = 0; i < 9; ++i)
+                out[i * n + c] = kind == ORC_BF16 ? orc_f32_to_bf16_sr(o[i * n + c], R[i])
+                                                  : orc_f32_to_f16_sr(o[i * n + c], R[i]);
+        }
+    }
     if (p->boundary == ORC_BB_HALFWAY && flags) {
         for (long j = 0; j < n; ++j)
             if (is_solid(flags[j]))
@@ -401,17 +477,29 @@ void orc_step_dev16(const orc_params* p, int kind, const uint8_t* flags, const u
     free(o);
 }
 
-int orc_run_dev16(const orc_params* p, int kind, const uint8_t* flags, uint16_t* d, int nsteps)
+void orc_step_dev16(const orc_params* p, int kind, const uint8_t* flags, const uint16_t* P,
+                    uint16_t* out)
+{
+    orc_step_dev16_ex(p, kind, 0, 0, 0, flags, P, out);
+}
+
+int orc_run_dev16_ex(const orc_params* p, int kind, int rounding, uint32_t seed, uint32_t step0,
+                     const uint8_t* flags, uint16_t* d, int nsteps)
 {
     const long n = 9L * p->w * p->h;
     uint16_t* tmp = (uint16_t*)malloc(sizeof(uint16_t) * n);
     if (!tmp) return -1;
     for (int s = 0; s < nsteps; ++s) {
-        orc_step_dev16(p, kind, flags, d, tmp);
+        orc_step_dev16_ex(p, kind, rounding, seed, step0 + (uint32_t)s, flags, d, tmp);
         memcpy(d, tmp, sizeof(uint16_t) * n);
     }
     free(tmp);
     return 0;
+}
+
+int orc_run_dev16(const orc_params* p, int kind, const uint8_t* flags, uint16_t* d, int nsteps)
+{
+    return orc_run_dev16_ex(p, kind, 0, 0, 0, flags, d, nsteps);
 }
 
 /* ----------------------------------------------------------- residual --- */

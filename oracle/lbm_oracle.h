@@ -126,6 +126,16 @@ void orc_f32_to_f16_n(const float* x, uint16_t* y, long n);
 void orc_step_dev16(const orc_params* p, int kind, const uint8_t* flags, const uint16_t* P,
                     uint16_t* out);
 int orc_run_dev16(const orc_params* p, int kind, const uint8_t* flags, uint16_t* d, int nsteps);
+/* Stochastic rounding (rounding = 1): see lbm_oracle.c for the bit model and
+ * the random-bit schedule keyed by (seed, step, cell index r*w + c). */
+uint16_t orc_f32_to_bf16_sr(float x, uint32_t r);
+uint16_t orc_f32_to_f16_sr(float x, uint32_t r);
+uint32_t orc_sr_key(uint32_t seed, uint32_t step);
+void orc_sr_bits(uint32_t key, uint32_t cell, uint32_t R[9]);
+void orc_step_dev16_ex(const orc_params* p, int kind, int rounding, uint32_t seed, uint32_t step,
+                       const uint8_t* flags, const uint16_t* P, uint16_t* out);
+int orc_run_dev16_ex(const orc_params* p, int kind, int rounding, uint32_t seed, uint32_t step0,
+                     const uint8_t* flags, uint16_t* d, int nsteps);
 /* Deviation arithmetic on f32 deviations g_i = f_i - w_i*rho0 (extension). */
 void orc_step_devf32(const orc_params* p, const uint8_t* flags, const float* G, float* out);
 int orc_run_devf32(const orc_params* p, const uint8_t* flags, float* g, int nsteps);

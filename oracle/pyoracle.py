@@ -79,6 +79,11 @@ for name, args, res in [
     ("orc_f32_to_f16_n", [_P, _P, C.c_long], None),
     ("orc_step_dev16", [_PP, C.c_int, _P, _P, _P], None),
     ("orc_run_dev16", [_PP, C.c_int, _P, _P, C.c_int], C.c_int),
+    ("orc_run_dev16_ex", [_PP, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _P, _P, C.c_int], C.c_int),
+    ("orc_f32_to_bf16_sr", [C.c_float, C.c_uint32], C.c_uint16),
+    ("orc_f32_to_f16_sr", [C.c_float, C.c_uint32], C.c_uint16),
+    ("orc_sr_key", [C.c_uint32, C.c_uint32], C.c_uint32),
+    ("orc_sr_bits", [C.c_uint32, C.c_uint32, _P], None),
     ("orc_encode_dev16", [_PP, C.c_int, _P, _P], None),
     ("orc_decode_dev16", [_PP, C.c_int, _P, _P], None),
     ("orc_sums_f32", [C.c_int, C.c_int, _P, _P, _P], None),
@@ -282,10 +287,17 @@ def from_dev(p: Params, g):
     return (np.asarray(g, np.float32) + np.array(k.base, np.float32)[:, None, None]).astype(np.float32)
 
 
-def run_dev16(p: Params, kind, d, nsteps, flags=None):
+def run_dev16(p: Params, kind, d, nsteps, flags=None, rounding=0, seed=0, step0=0):
     d = np.array(d, copy=True, order="C", dtype=np.uint16)
-    assert lib.orc_run_dev16(C.byref(p), int(kind), _p(_flags(flags, p)), _p(d), int(nsteps)) == 0
+    assert lib.orc_run_dev16_ex(C.byref(p), int(kind), int(rounding), int(seed) & 0xffffffff,
+                                int(step0) & 0xffffffff, _p(_flags(flags, p)), _p(d), int(nsteps)) == 0
     return d
+
+
+def sr_bits(key, cell):
+    out = np.zeros(9, np.uint32)
+    lib.orc_sr_bits(int(key) & 0xffffffff, int(cell) & 0xffffffff, _p(out))
+    return out
 
 
 def sums(f, flags=None):

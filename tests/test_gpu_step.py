@@ -177,6 +177,35 @@ def test_dev16_matches_oracle(ctx, kind, halfway):
         np.testing.assert_array_equal(lat.download_raw(), O.encode_dev16(op, okind, f))
 
 
+@pytest.mark.parametrize("kind", ["bf16", "f16"])
+@pytest.mark.parametrize("halfway", [False, True])
+def test_dev16_stochastic_rounding_matches_oracle(ctx, kind, halfway):
+    n, m = 40, 64
+    rho, ux, uy = W.taylor_green(n, 0.05, np.float32, m=m)
+    okind = O.BF16 if kind == "bf16" else O.F16
+    if halfway:
+        flags = W.cavity_flags(m, n)
+        op = O.params(m, n, tau=0.8, boundary=O.BB_HALFWAY, lid=W.lid_velocity(0.08))
+    else:
+        flags = None
+        op = O.params(m, n, tau=0.8, edge=O.EDGE_PERIODIC)
+    seed = 0xC0FFEE
+    gp = gpu_params(op)
+    gp.rounding = capi.CS_ROUND_STOCHASTIC
+    gp.seed = seed
+    d0 = narrow16(okind, O.equilibrium_dev(op, rho, ux, uy))
+    with Lattice(ctx, m, n, kind, gp, flags) as lat:
+        lat.init_equilibrium(rho, ux, uy)
+        lat.step(15)
+        d15 = O.run_dev16(op, okind, d0, 15, flags, rounding=1, seed=seed)
+        np.testing.assert_array_equal(lat.download_raw(), d15)
+        lat.step(5)
+        np.testing.assert_array_equal(lat.download_raw(),
+                                      O.run_dev16(op, okind, d15, 5, flags, rounding=1, seed=seed, step0=15))
+        # stochastic and nearest rounding differ
+        assert not np.array_equal(d15, O.run_dev16(op, okind, d0, 15, flags))
+
+
 @pytest.mark.parametrize("case", ["ref_zero", "ref_periodic", "halfway", "channel"])
 def test_f32dev_matches_oracle(ctx, case):
     rng = np.random.default_rng(18)
