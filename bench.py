@@ -71,6 +71,33 @@ def cpu_baseline(n, tau, u0, budget_s=12.0):
     return n * n * steps / dt / 1e9, used, steps
 
 
+def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20):
+    """512 independent lid-driven cavities (sides U{64..512}, tau U[0.55,1],
+    lid U[0.02,0.1], seed 0), FP32, half-way walls: autotuned persistent launch
+    of `steps` steps vs one launch per lattice per step."""
+    from chemsim_amd import Params, capi
+    from chemsim_amd import workloads as W
+    from chemsim_amd.batch import Batch, tune
+    specs = W.batch_cavities(seed=0, count=512)
+    b = Batch(ctx, [(s.side, s.side) for s in specs], "f32")
+    for i, s in enumerate(specs):
+        b.set_params(i, Params(tau=s.tau, boundary=capi.CS_BB_HALFWAY, lid_uy=s.u_lid))
+        b.set_flags(i, W.cavity_flags(s.side, s.side))
+        b.init_equilibrium(i, *W.rest_state(s.side, s.side))
+    cache = os.environ.get("CHEMSIM_TUNING_CACHE",
+                           os.path.join(os.path.expanduser("~"), ".cache", "chemsim_amd", "batch_tuning.json"))
+    rec = tune(ctx, b, cache, trial_steps=20)
+    ms = b.step(steps)
+    ms_pl = b.step_per_lattice(per_lattice_steps)
+    cells = b.cells
+    out = {"lattices": len(b), "cells": cells, "steps": steps,
+           "GLUPS_persistent": round(cells * steps / ms / 1e6, 3),
+           "GLUPS_per_lattice_launches": round(cells * per_lattice_steps / ms_pl / 1e6, 3),
+           "seconds": round(ms / 1e3, 3), "tuning": rec["tuning"]}
+    b.close()
+    return out
+
+
 def load_profile_traffic(kernel_key: str):
     """HBM bytes per launch of the step kernel measured with rocprofv3 --pmc
     (FETCH_SIZE, WRITE_SIZE) and scaled by a copy kernel of known size; the
@@ -95,6 +122,7 @@ def main():
     ap.add_argument("--u0", type=float, default=0.05)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the FP64 / 16-bit comparison runs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-cavity batched run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -164,6 +192,10 @@ def main():
                           "mass_drift_rel": abs(m1 - m0) / m0}
             log(f"[bench] {key}: {extras[key]}")
         del x64, y64, r64
+
+    if not args.no_batch and not args.no_extras:
+        extras["batch_512_cavities"] = batch_cavities_run(ctx)
+        log(f"[bench] batch: {extras['batch_512_cavities']}")
 
     cpu = None
     if not args.no_cpu:

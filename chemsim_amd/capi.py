@@ -20,6 +20,7 @@ CS_EDGE_ZERO, CS_EDGE_PERIODIC = range(2)
 CS_BB_REFERENCE, CS_BB_HALFWAY = range(2)
 CS_FLUID, CS_WALL, CS_LID, CS_INLET, CS_OUTLET = range(5)
 CS_ROUND_NEAREST, CS_ROUND_STOCHASTIC = range(2)
+CS_ORDER_LARGEST_FIRST, CS_ORDER_GIVEN, CS_ORDER_SMALLEST_FIRST = range(3)
 
 
 class cs_params(C.Structure):
@@ -40,21 +41,12 @@ class cs_params(C.Structure):
     ]
 
 
-class cs_batch_lattice(C.Structure):
-    """One lattice of a batch (include/chemsim.h)."""
-    _fields_ = [
-        ("w", C.c_int32),
-        ("h", C.c_int32),
-        ("params", cs_params),
-    ]
-
-
 class cs_batch_tuning(C.Structure):
     _fields_ = [
-        ("tile_rows", C.c_int32),
+        ("block_threads", C.c_int32),
         ("workgroups_per_cu", C.c_int32),
+        ("cells_per_thread", C.c_int32),
         ("order", C.c_int32),
-        ("reserved", C.c_int32),
     ]
 
 
@@ -124,6 +116,21 @@ SIGNATURES = {
     "cs_lattice_snapshot_velocity": [_P],
     "cs_lattice_residual_f64": [_P, _DP],
     "cs_lattice_run_to_residual": [_P, _D, _I, _I64, _I64P, _DP],
+    "cs_batch_create": [_P, _I, _P, _P, _I, C.POINTER(_P)],
+    "cs_batch_destroy": [_P],
+    "cs_batch_count": [_P, _IP],
+    "cs_batch_set_params": [_P, _I, C.POINTER(cs_params)],
+    "cs_batch_set_flags": [_P, _I, _P],
+    "cs_batch_init_equilibrium": [_P, _I, _P, _P, _P],
+    "cs_batch_upload_f32": [_P, _I, _P],
+    "cs_batch_download_f32": [_P, _I, _P],
+    "cs_batch_download_raw": [_P, _I, _P],
+    "cs_batch_velocity": [_P, _I, _P, _P],
+    "cs_batch_set_tuning": [_P, C.POINTER(cs_batch_tuning)],
+    "cs_batch_get_tuning": [_P, C.POINTER(cs_batch_tuning)],
+    "cs_batch_step": [_P, _I, _FP],
+    "cs_batch_step_per_lattice": [_P, _I, _FP],
+    "cs_batch_autotune": [_P, _I, C.POINTER(cs_batch_tuning), _FP],
     "cs_bench_copy": [_P, _I64, _I, _FP],
     "cs_set_cells_per_thread": [_I],
 }

@@ -1,0 +1,688 @@
+// Batch of independent lattices stepped by one persistent launch.
+//
+// The reference steps one lattice per State::step (lbm.rs:694-714).  A batch
+// holds many lattices of uneven sizes in one device arena; k_batch is a
+// persistent grid whose workgroups take whole lattices from a device-side
+// queue (largest first by default) and run every step on them, with a
+// workgroup barrier between steps.  No inter-workgroup synchronisation is
+// needed, and the per-cell code is the same gather/finish as the single-
+// lattice step, so the results equal per-lattice launches bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <functional>
+#include <numeric>
+#include <type_traits>
+#include <vector>
+
+#include "cs_internal.hpp"
+#include "lattice_kernels.hpp"
+#include "lbm_device.hpp"
+
+namespace cs {
+
+// Per-lattice descriptor in device memory.
+template <class T>
+struct BatchLat {
+    int w, h, pitch, mode;  // mode bits: 1 periodic, 2 half-way, 4 has flags, 8 stochastic rounding
+    int64_t plane;          // plane stride (elements)
+    int64_t off;            // element offset of buffer 0 in the arena (buffer 1 follows, 9 planes later)
+    int64_t node_off;       // element offset in the node arena
+    uint32_t seed;
+    Consts<T> k;
+};
+
+enum : int { BM_PERIODIC = 1, BM_HALFWAY = 2, BM_FLAGS = 4, BM_SR = 8 };
+
+// One step of one lattice by one workgroup: wave `wave` of `nwaves` takes
+// rows wave, wave + nwaves, ...; lane l the columns l + 64 j.
+template <class St, int EDGE, int MODE, bool SR, int CPT>
+__device__ __forceinline__ void batch_lattice_step(const BatchLat<typename St::T>& L, const typename St::S* fin,
+                                                   typename St::S* fout, const uint32_t* node, uint32_t key,
+                                                   int wave, int nwaves, int lane)
+{
+    using T = typename St::T;
+    for (int r = wave; r < L.h; r += nwaves) {
+        for (int c0 = lane; c0 < L.w; c0 += 64 * CPT) {
+            T B[CPT][9];
+            int type[CPT];
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                const int c = c0 + 64 * j;
+                const int cc = c < L.w ? c : L.w - 1;
+                type[j] = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.plane, L.k, r, cc, B[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                const int c = c0 + 64 * j;
+                if (c < L.w)
+                    finish<St, MODE, SR, uint32_t>(fin, fout, L.pitch, L.plane, L.k, r, c, type[j], B[j], key,
+                                         (uint32_t)(r * L.w + c));
+            }
+        }
+    }
+}
+
+// Persistent grid over the lattices of one boundary-mode group (order[0..n)).
+// The queue counter starts at 0; every workgroup leaves the loop once the
+// counter passes n, so the grid always drains.
+template <class St, int EDGE, int MODE, bool SR, int CPT>
+__global__ __launch_bounds__(1024) void k_batch(const BatchLat<typename St::T>* __restrict__ lats,
+                                                const int* __restrict__ order, int n, int* __restrict__ queue,
+                                                void* __restrict__ arena, const uint32_t* __restrict__ nodes,
+                                                int steps, int cur0, uint64_t step0)
+{
+    using S = typename St::S;
+    __shared__ int s_q;
+    const int nwaves = (int)(blockDim.x >> 6), wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
+        __syncthreads();
+        const int q = __builtin_amdgcn_readfirstlane(s_q);
+        __syncthreads();
+        if (q >= n) return;
+        const int li = __builtin_amdgcn_readfirstlane(order[q]);  // uniform: descriptor loads go scalar
+        const BatchLat<typename St::T> L = lats[li];
+        S* base = (S*)arena + L.off;
+        const uint32_t* node = nodes + L.node_off;
+        for (int s = 0; s < steps; ++s) {
+            const int cur = (cur0 + s) & 1;
+            const S* fin = base + (int64_t)cur * 9 * L.plane;
+            S* fout = base + (int64_t)(cur ^ 1) * 9 * L.plane;
+            const uint32_t key = sr_key(L.seed, step0 + (uint64_t)s);
+            batch_lattice_step<St, EDGE, MODE, SR, CPT>(L, fin, fout, node, key, wave, nwaves, lane);
+            __syncthreads();  // step s fully stored before step s + 1 reads it
+        }
+    }
+}
+
+// Mode group of a lattice: periodic, mode (fluid/ref/half), stochastic rounding.
+inline int mode_group(int mode_bits)
+{
+    const int m = !(mode_bits & BM_FLAGS) ? MODE_FLUID : (mode_bits & BM_HALFWAY) ? MODE_HALF : MODE_REF;
+    return ((mode_bits & BM_PERIODIC) ? 6 : 0) + m * 2 + ((mode_bits & BM_SR) ? 1 : 0);
+}
+
+}  // namespace cs
+
+using namespace cs;
+
+struct cs_batch {
+    cs_ctx* ctx = nullptr;
+    int n = 0, storage = CS_F32;
+    size_t esize = 4;
+    std::vector<int> w, h, pitch;
+    std::vector<int64_t> plane, off, node_off;
+    std::vector<cs_params> params;
+    std::vector<uint8_t> has_flags;
+    int64_t arena_elems = 0, node_elems = 0;
+    void* arena = nullptr;
+    uint32_t* nodes = nullptr;
+    void* d_lats = nullptr;
+    int* d_order = nullptr;
+    int* d_queue = nullptr;
+    bool lats_dirty = true;
+    int cur = 0;
+    uint64_t steps = 0;
+    cs_batch_tuning tuning{256, 2, 1, CS_ORDER_LARGEST_FIRST};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace cs {
+
+template <class F>
+static void dispatch_batch_storage(int storage, F&& f)
+{
+    switch (storage) {
+    case CS_F32: f.template operator()<StF32>(); break;
+    case CS_F64: f.template operator()<StF64>(); break;
+    case CS_F32DEV: f.template operator()<StF32Dev>(); break;
+    case CS_BF16DEV: f.template operator()<StBF16>(); break;
+    case CS_F16DEV: f.template operator()<StF16>(); break;
+    default: throw Error{CS_ERR_ARG, "bad storage"};
+    }
+}
+
+template <class T>
+static Consts<T> batch_consts(const cs_batch* b, int i);
+template <>
+Consts<float> batch_consts<float>(const cs_batch* b, int i)
+{
+    return is_dev_storage(b->storage) ? make_consts_dev_f32(b->params[i]) : make_consts_f32(b->params[i]);
+}
+template <>
+Consts<double> batch_consts<double>(const cs_batch* b, int i)
+{
+    return make_consts_f64(b->params[i]);
+}
+
+template <class T>
+static BatchLat<T> describe(const cs_batch* b, int i)
+{
+    BatchLat<T> L{};
+    L.w = b->w[i];
+    L.h = b->h[i];
+    L.pitch = b->pitch[i];
+    const cs_params& p = b->params[i];
+    L.mode = (p.edge == CS_EDGE_PERIODIC ? BM_PERIODIC : 0) | (p.boundary == CS_BB_HALFWAY ? BM_HALFWAY : 0) |
+             (b->has_flags[i] ? BM_FLAGS : 0) | (p.rounding == CS_ROUND_STOCHASTIC ? BM_SR : 0);
+    L.plane = b->plane[i];
+    L.off = b->off[i];
+    L.node_off = b->node_off[i];
+    L.seed = p.seed;
+    L.k = batch_consts<T>(b, i);
+    return L;
+}
+
+static void check_batch(const cs_batch* b, int index)
+{
+    CS_REQUIRE(b != nullptr, CS_ERR_ARG, "null batch");
+    CS_REQUIRE(index >= 0 && index < b->n, CS_ERR_ARG, "lattice index out of range");
+}
+
+static void upload_descriptors(cs_batch* b)
+{
+    if (!b->lats_dirty) return;
+    dispatch_batch_storage(b->storage, [&]<class St>() {
+        using T = typename St::T;
+        std::vector<BatchLat<T>> host((size_t)b->n);
+        for (int i = 0; i < b->n; ++i) host[(size_t)i] = describe<T>(b, i);
+        if (!b->d_lats) CS_HIP(hipMalloc(&b->d_lats, sizeof(BatchLat<T>) * (size_t)b->n));
+        CS_HIP(hipMemcpyAsync(b->d_lats, host.data(), sizeof(BatchLat<T>) * (size_t)b->n, hipMemcpyHostToDevice,
+                              b->stream));
+    });
+    b->lats_dirty = false;
+}
+
+// Lattice order (by the tuning's queue order), grouped by boundary mode;
+// returns (group, first, count) triples into d_order.
+static std::vector<std::array<int, 3>> upload_order(cs_batch* b)
+{
+    std::vector<int> ord((size_t)b->n);
+    std::iota(ord.begin(), ord.end(), 0);
+    auto cells = [&](int i) { return (int64_t)b->w[(size_t)i] * b->h[(size_t)i]; };
+    if (b->tuning.order == CS_ORDER_LARGEST_FIRST)
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return cells(a) > cells(c); });
+    else if (b->tuning.order == CS_ORDER_SMALLEST_FIRST)
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return cells(a) < cells(c); });
+    std::vector<int> grp((size_t)b->n);
+    for (int i = 0; i < b->n; ++i) {
+        const cs_params& p = b->params[(size_t)i];
+        const int bits = (p.edge == CS_EDGE_PERIODIC ? BM_PERIODIC : 0) | (p.boundary == CS_BB_HALFWAY ? BM_HALFWAY : 0) |
+                         (b->has_flags[(size_t)i] ? BM_FLAGS : 0) |
+                         (p.rounding == CS_ROUND_STOCHASTIC && b->esize == 2 ? BM_SR : 0);
+        grp[(size_t)i] = mode_group(bits);
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return grp[(size_t)a] < grp[(size_t)c]; });
+    std::vector<std::array<int, 3>> groups;
+    for (int k = 0; k < b->n; ++k) {
+        const int g = grp[(size_t)ord[(size_t)k]];
+        if (groups.empty() || groups.back()[0] != g) groups.push_back({g, k, 0});
+        groups.back()[2] += 1;
+    }
+    CS_HIP(hipMemcpyAsync(b->d_order, ord.data(), sizeof(int) * (size_t)b->n, hipMemcpyHostToDevice, b->stream));
+    return groups;
+}
+
+template <class St, int CPT>
+static void launch_group(cs_batch* b, int group, int first, int count, int steps, int* queue, int grid)
+{
+    using T = typename St::T;
+    const auto* lats = (const BatchLat<T>*)b->d_lats;
+    const int* order = b->d_order + first;
+    const dim3 block((unsigned)b->tuning.block_threads);
+    auto go = [&]<int EDGE, int MODE, bool SR>() {
+        hipLaunchKernelGGL((k_batch<St, EDGE, MODE, SR, CPT>), dim3((unsigned)grid), block, 0, b->stream, lats, order,
+                           count, queue, b->arena, b->nodes, steps, b->cur, b->steps);
+    };
+    auto go_sr = [&]<int EDGE, int MODE>() {
+        if constexpr (is_16bit<St>()) {
+            if (group & 1) {
+                go.template operator()<EDGE, MODE, true>();
+                return;
+            }
+        }
+        go.template operator()<EDGE, MODE, false>();
+    };
+    auto go_mode = [&]<int EDGE>() {
+        const int m = (group % 6) / 2;
+        if (m == MODE_FLUID) go_sr.template operator()<EDGE, MODE_FLUID>();
+        else if (m == MODE_HALF) go_sr.template operator()<EDGE, MODE_HALF>();
+        else go_sr.template operator()<EDGE, MODE_REF>();
+    };
+    if (group >= 6) go_mode.template operator()<CS_EDGE_PERIODIC>();
+    else go_mode.template operator()<CS_EDGE_ZERO>();
+}
+
+static void launch_batch(cs_batch* b, int steps)
+{
+    upload_descriptors(b);
+    const auto groups = upload_order(b);
+    CS_REQUIRE(groups.size() <= 16, CS_ERR_STATE, "too many boundary-mode groups");
+    CS_HIP(hipMemsetAsync(b->d_queue, 0, sizeof(int) * 16, b->stream));
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const auto& g = groups[gi];
+        const int grid = std::max(1, std::min(g[2], b->ctx->prop.multiProcessorCount * b->tuning.workgroups_per_cu));
+        dispatch_batch_storage(b->storage, [&]<class St>() {
+            if (b->tuning.cells_per_thread >= 2) launch_group<St, 2>(b, g[0], g[1], g[2], steps, b->d_queue + gi, grid);
+            else launch_group<St, 1>(b, g[0], g[1], g[2], steps, b->d_queue + gi, grid);
+        });
+    }
+    CS_HIP(hipGetLastError());
+    b->cur = (b->cur + steps) & 1;
+    b->steps += (uint64_t)steps;
+}
+
+// One k_step launch per lattice per step (single-lattice code path).
+static void launch_per_lattice(cs_batch* b, int steps)
+{
+    dispatch_batch_storage(b->storage, [&]<class St>() {
+        using T = typename St::T;
+        std::vector<BatchLat<T>> L((size_t)b->n);
+        for (int i = 0; i < b->n; ++i) L[(size_t)i] = describe<T>(b, i);
+        for (int s = 0; s < steps; ++s) {
+            const int cur = (b->cur + s) & 1;
+            for (int i = 0; i < b->n; ++i) {
+                const BatchLat<T>& d = L[(size_t)i];
+                StepArgs<T> a;
+                typename St::S* base = (typename St::S*)b->arena + d.off;
+                a.fin = base + (int64_t)cur * 9 * d.plane;
+                a.fout = base + (int64_t)(cur ^ 1) * 9 * d.plane;
+                a.node = b->nodes + d.node_off;
+                a.w = d.w;
+                a.h = d.h;
+                a.pitch = d.pitch;
+                a.plane = d.plane;
+                a.tiles_x = (d.w + 63) / 64;
+                a.sr_key = sr_key(d.seed, b->steps + (uint64_t)s);
+                a.cell_base = 0;
+                a.k = d.k;
+                const int nblocks = a.tiles_x * ((d.h + 3) / 4);
+                const bool periodic = d.mode & BM_PERIODIC;
+                const int mode = !(d.mode & BM_FLAGS) ? MODE_FLUID : (d.mode & BM_HALFWAY) ? MODE_HALF : MODE_REF;
+                const bool sr = (d.mode & BM_SR) && is_16bit<St>();
+                auto go = [&]<int EDGE, int MODE, bool SR>() {
+                    hipLaunchKernelGGL((k_step<St, EDGE, MODE, 1, SR, uint32_t>), dim3((unsigned)nblocks), dim3(256), 0,
+                                       b->stream, a);
+                };
+                auto go_sr = [&]<int EDGE, int MODE>() {
+                    if constexpr (is_16bit<St>()) {
+                        if (sr) {
+                            go.template operator()<EDGE, MODE, true>();
+                            return;
+                        }
+                    }
+                    go.template operator()<EDGE, MODE, false>();
+                };
+                auto go_mode = [&]<int EDGE>() {
+                    if (mode == MODE_FLUID) go_sr.template operator()<EDGE, MODE_FLUID>();
+                    else if (mode == MODE_HALF) go_sr.template operator()<EDGE, MODE_HALF>();
+                    else go_sr.template operator()<EDGE, MODE_REF>();
+                };
+                if (periodic) go_mode.template operator()<CS_EDGE_PERIODIC>();
+                else go_mode.template operator()<CS_EDGE_ZERO>();
+            }
+        }
+    });
+    CS_HIP(hipGetLastError());
+    b->cur = (b->cur + steps) & 1;
+    b->steps += (uint64_t)steps;
+}
+
+static void* lattice_buf(cs_batch* b, int i) { return (char*)b->arena + (size_t)b->off[(size_t)i] * b->esize + (size_t)b->cur * 9 * b->plane[(size_t)i] * b->esize; }
+
+}  // namespace cs
+
+extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const int32_t* h, int storage,
+                               cs_batch** out)
+{
+    return guarded([&] {
+        CS_REQUIRE(ctx && w && h && out && count >= 1, CS_ERR_ARG, "bad argument");
+        CS_REQUIRE(storage_bytes(storage) != 0, CS_ERR_ARG, "bad storage");
+        CS_HIP(hipSetDevice(ctx->device));
+        auto* b = new cs_batch;
+        b->ctx = ctx;
+        b->n = count;
+        b->storage = storage;
+        b->esize = storage_bytes(storage);
+        const int align = (int)(256 / b->esize);
+        int64_t acc = 0, nacc = 0;
+        for (int i = 0; i < count; ++i) {
+            if (!(w[i] >= 1 && h[i] >= 1 && (int64_t)w[i] * h[i] <= ((int64_t)1 << 31))) {
+                delete b;
+                throw Error{CS_ERR_ARG, "bad lattice size"};
+            }
+            const int p = (w[i] + align - 1) / align * align;
+            b->w.push_back(w[i]);
+            b->h.push_back(h[i]);
+            b->pitch.push_back(p);
+            b->plane.push_back((int64_t)p * h[i]);
+            b->off.push_back(acc);
+            b->node_off.push_back(nacc);
+            acc += 2 * 9 * (int64_t)p * h[i];
+            nacc += ((int64_t)p * h[i] + 63) / 64 * 64;
+        }
+        b->arena_elems = acc;
+        b->node_elems = nacc;
+        cs_params p{};
+        p.tau = 1.0;
+        p.delta_x = 1.0;
+        p.delta_t = 1.0;
+        p.outlet_rho = 1.0;
+        p.rho0 = 1.0;
+        b->params.assign((size_t)count, p);
+        b->has_flags.assign((size_t)count, 0);
+        try {
+            CS_HIP(hipMalloc(&b->arena, (size_t)acc * b->esize));
+            CS_HIP(hipMemset(b->arena, 0, (size_t)acc * b->esize));
+            CS_HIP(hipMalloc(&b->nodes, sizeof(uint32_t) * (size_t)nacc));
+            CS_HIP(hipMemset(b->nodes, 0, sizeof(uint32_t) * (size_t)nacc));
+            CS_HIP(hipMalloc(&b->d_order, sizeof(int) * (size_t)count));
+            CS_HIP(hipMalloc(&b->d_queue, sizeof(int) * 16));
+            CS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+            CS_HIP(hipEventCreate(&b->ev0));
+            CS_HIP(hipEventCreate(&b->ev1));
+        } catch (...) {
+            cs_batch_destroy(b);
+            throw;
+        }
+        *out = b;
+    });
+}
+
+extern "C" int cs_batch_destroy(cs_batch* b)
+{
+    return guarded([&] {
+        if (!b) return;
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        for (void* p : {b->arena, (void*)b->nodes, b->d_lats, (void*)b->d_order, (void*)b->d_queue})
+            if (p) (void)hipFree(p);
+        if (b->ev0) (void)hipEventDestroy(b->ev0);
+        if (b->ev1) (void)hipEventDestroy(b->ev1);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+    });
+}
+
+extern "C" int cs_batch_count(const cs_batch* b, int* count)
+{
+    return guarded([&] {
+        CS_REQUIRE(b && count, CS_ERR_ARG, "null argument");
+        *count = b->n;
+    });
+}
+
+extern "C" int cs_batch_set_params(cs_batch* b, int index, const cs_params* p)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        CS_REQUIRE(p != nullptr, CS_ERR_ARG, "null params");
+        CS_REQUIRE(p->edge == CS_EDGE_ZERO || p->edge == CS_EDGE_PERIODIC, CS_ERR_ARG, "bad edge mode");
+        CS_REQUIRE(p->boundary == CS_BB_REFERENCE || p->boundary == CS_BB_HALFWAY, CS_ERR_ARG, "bad boundary mode");
+        CS_REQUIRE(p->tau > 0 && p->delta_x > 0 && p->delta_t > 0 && p->outlet_rho > 0, CS_ERR_ARG,
+                   "tau, delta_x, delta_t, outlet_rho must be > 0");
+        CS_REQUIRE(p->rounding == CS_ROUND_NEAREST || p->rounding == CS_ROUND_STOCHASTIC, CS_ERR_ARG,
+                   "bad rounding mode");
+        CS_REQUIRE(p->rho0 == b->params[(size_t)index].rho0 || !is_dev_storage(b->storage), CS_ERR_UNSUPPORTED,
+                   "rho0 of a deviation-storage batch is fixed at creation (1.0)");
+        const bool edge_changed = p->edge != b->params[(size_t)index].edge;
+        CS_REQUIRE(!(edge_changed && b->has_flags[(size_t)index]), CS_ERR_STATE,
+                   "set the edge mode before the flags of a batch lattice");
+        b->params[(size_t)index] = *p;
+        b->lats_dirty = true;
+    });
+}
+
+extern "C" int cs_batch_set_flags(cs_batch* b, int index, const uint8_t* flags)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        const size_t i = (size_t)index;
+        if (!flags) {
+            b->has_flags[i] = 0;
+            b->lats_dirty = true;
+            return;
+        }
+        const int64_t cells = (int64_t)b->w[i] * b->h[i];
+        for (int64_t d = 0; d < cells; ++d) CS_REQUIRE(flags[d] <= CS_OUTLET, CS_ERR_ARG, "flag value out of range");
+        uint8_t* tmp = nullptr;
+        CS_HIP(hipMalloc(&tmp, (size_t)b->plane[i]));
+        CS_HIP(hipMemcpy2DAsync(tmp, (size_t)b->pitch[i], flags, (size_t)b->w[i], (size_t)b->w[i], (size_t)b->h[i],
+                                hipMemcpyHostToDevice, b->stream));
+        dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
+        hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, b->stream, tmp, b->nodes + b->node_off[i], b->w[i], b->h[i],
+                           b->pitch[i], (int)b->params[i].edge);
+        hipError_t e = hipGetLastError();
+        hipError_t e2 = hipStreamSynchronize(b->stream);
+        (void)hipFree(tmp);
+        CS_HIP(e);
+        CS_HIP(e2);
+        b->has_flags[i] = 1;
+        b->lats_dirty = true;
+    });
+}
+
+extern "C" int cs_batch_init_equilibrium(cs_batch* b, int index, const float* rho, const float* ux, const float* uy)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        CS_REQUIRE(rho && ux && uy, CS_ERR_ARG, "null input");
+        const size_t i = (size_t)index;
+        const int64_t n = (int64_t)b->w[i] * b->h[i];
+        dispatch_batch_storage(b->storage, [&]<class St>() {
+            using T = typename St::T;
+            std::vector<T> host((size_t)(3 * n));
+            for (int64_t d = 0; d < n; ++d) {
+                host[(size_t)d] = (T)rho[d];
+                host[(size_t)(n + d)] = (T)ux[d];
+                host[(size_t)(2 * n + d)] = (T)uy[d];
+            }
+            void* dev = nullptr;
+            CS_HIP(hipMalloc(&dev, sizeof(T) * 3 * (size_t)n));
+            CS_HIP(hipMemcpyAsync(dev, host.data(), sizeof(T) * 3 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+            const T* p = (const T*)dev;
+            dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
+            hipLaunchKernelGGL((k_init_eq<St>), g, dim3(256), 0, b->stream, p, p + n, p + 2 * n,
+                               (typename St::S*)lattice_buf(b, index), b->w[i], b->h[i], b->pitch[i], b->plane[i],
+                               batch_consts<T>(b, index));
+            hipError_t e = hipGetLastError();
+            hipError_t e2 = hipStreamSynchronize(b->stream);
+            (void)hipFree(dev);
+            CS_HIP(e);
+            CS_HIP(e2);
+        });
+    });
+}
+
+extern "C" int cs_batch_upload_f32(cs_batch* b, int index, const float* soa)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        CS_REQUIRE(soa, CS_ERR_ARG, "null data");
+        const size_t i = (size_t)index;
+        float* tmp = nullptr;
+        CS_HIP(hipMalloc(&tmp, sizeof(float) * 9 * (size_t)b->plane[i]));
+        CS_HIP(hipMemcpy2DAsync(tmp, (size_t)b->pitch[i] * 4, soa, (size_t)b->w[i] * 4, (size_t)b->w[i] * 4,
+                                (size_t)9 * b->h[i], hipMemcpyHostToDevice, b->stream));
+        dispatch_batch_storage(b->storage, [&]<class St>() {
+            dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
+            hipLaunchKernelGGL((k_from_f32<St>), g, dim3(256), 0, b->stream, (const float*)tmp,
+                               (typename St::S*)lattice_buf(b, index), b->w[i], b->h[i], b->pitch[i], b->plane[i],
+                               batch_consts<typename St::T>(b, index));
+        });
+        hipError_t e = hipGetLastError();
+        hipError_t e2 = hipStreamSynchronize(b->stream);
+        (void)hipFree(tmp);
+        CS_HIP(e);
+        CS_HIP(e2);
+    });
+}
+
+extern "C" int cs_batch_download_f32(cs_batch* b, int index, float* soa)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        CS_REQUIRE(soa, CS_ERR_ARG, "null data");
+        const size_t i = (size_t)index;
+        float* tmp = nullptr;
+        CS_HIP(hipMalloc(&tmp, sizeof(float) * 9 * (size_t)b->plane[i]));
+        dispatch_batch_storage(b->storage, [&]<class St>() {
+            dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
+            hipLaunchKernelGGL((k_to_f32<St>), g, dim3(256), 0, b->stream,
+                               (const typename St::S*)lattice_buf(b, index), tmp, b->w[i], b->h[i], b->pitch[i],
+                               b->plane[i], batch_consts<typename St::T>(b, index));
+        });
+        hipError_t e = hipGetLastError();
+        hipError_t e1 = hipMemcpy2DAsync(soa, (size_t)b->w[i] * 4, tmp, (size_t)b->pitch[i] * 4, (size_t)b->w[i] * 4,
+                                         (size_t)9 * b->h[i], hipMemcpyDeviceToHost, b->stream);
+        hipError_t e2 = hipStreamSynchronize(b->stream);
+        (void)hipFree(tmp);
+        CS_HIP(e);
+        CS_HIP(e1);
+        CS_HIP(e2);
+    });
+}
+
+extern "C" int cs_batch_download_raw(cs_batch* b, int index, void* data)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        CS_REQUIRE(data, CS_ERR_ARG, "null data");
+        const size_t i = (size_t)index;
+        CS_HIP(hipMemcpy2DAsync(data, (size_t)b->w[i] * b->esize, lattice_buf(b, index), (size_t)b->pitch[i] * b->esize,
+                                (size_t)b->w[i] * b->esize, (size_t)9 * b->h[i], hipMemcpyDeviceToHost, b->stream));
+        CS_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+extern "C" int cs_batch_velocity(cs_batch* b, int index, float* ux, float* uy)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        CS_REQUIRE(ux && uy, CS_ERR_ARG, "null output");
+        const size_t i = (size_t)index;
+        const int64_t n = (int64_t)b->w[i] * b->h[i];
+        double* dev = nullptr;
+        CS_HIP(hipMalloc(&dev, sizeof(double) * 2 * (size_t)n));
+        dispatch_batch_storage(b->storage, [&]<class St>() {
+            dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
+            hipLaunchKernelGGL((k_moments<St>), g, dim3(256), 0, b->stream,
+                               (const typename St::S*)lattice_buf(b, index), b->w[i], b->h[i], b->pitch[i],
+                               b->plane[i], batch_consts<typename St::T>(b, index), (unsigned)(M_UX | M_UY), dev);
+        });
+        std::vector<double> host((size_t)(2 * n));
+        hipError_t e = hipGetLastError();
+        hipError_t e1 = hipMemcpyAsync(host.data(), dev, sizeof(double) * 2 * (size_t)n, hipMemcpyDeviceToHost,
+                                       b->stream);
+        hipError_t e2 = hipStreamSynchronize(b->stream);
+        (void)hipFree(dev);
+        CS_HIP(e);
+        CS_HIP(e1);
+        CS_HIP(e2);
+        for (int64_t d = 0; d < n; ++d) {
+            ux[d] = (float)host[(size_t)d];
+            uy[d] = (float)host[(size_t)(n + d)];
+        }
+    });
+}
+
+extern "C" int cs_batch_set_tuning(cs_batch* b, const cs_batch_tuning* t)
+{
+    return guarded([&] {
+        CS_REQUIRE(b && t, CS_ERR_ARG, "null argument");
+        CS_REQUIRE(t->block_threads == 256 || t->block_threads == 512 || t->block_threads == 1024, CS_ERR_ARG,
+                   "block_threads must be 256, 512 or 1024");
+        CS_REQUIRE(t->workgroups_per_cu >= 1 && t->workgroups_per_cu <= 16, CS_ERR_ARG,
+                   "workgroups_per_cu must be 1..16");
+        CS_REQUIRE(t->cells_per_thread == 1 || t->cells_per_thread == 2, CS_ERR_ARG, "cells_per_thread must be 1 or 2");
+        CS_REQUIRE(t->order >= CS_ORDER_LARGEST_FIRST && t->order <= CS_ORDER_SMALLEST_FIRST, CS_ERR_ARG, "bad order");
+        b->tuning = *t;
+    });
+}
+
+extern "C" int cs_batch_get_tuning(const cs_batch* b, cs_batch_tuning* t)
+{
+    return guarded([&] {
+        CS_REQUIRE(b && t, CS_ERR_ARG, "null argument");
+        *t = b->tuning;
+    });
+}
+
+static void timed(cs_batch* b, float* ms, const std::function<void()>& body)
+{
+    CS_HIP(hipEventRecord(b->ev0, b->stream));
+    body();
+    CS_HIP(hipEventRecord(b->ev1, b->stream));
+    CS_HIP(hipEventSynchronize(b->ev1));
+    float t = 0;
+    CS_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    if (ms) *ms = t;
+}
+
+extern "C" int cs_batch_step(cs_batch* b, int steps, float* ms)
+{
+    return guarded([&] {
+        CS_REQUIRE(b && steps >= 0, CS_ERR_ARG, "bad argument");
+        if (steps == 0) return;
+        timed(b, ms, [&] { launch_batch(b, steps); });
+    });
+}
+
+extern "C" int cs_batch_step_per_lattice(cs_batch* b, int steps, float* ms)
+{
+    return guarded([&] {
+        CS_REQUIRE(b && steps >= 0, CS_ERR_ARG, "bad argument");
+        if (steps == 0) return;
+        timed(b, ms, [&] { launch_per_lattice(b, steps); });
+    });
+}
+
+extern "C" int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* best, float* best_ms)
+{
+    return guarded([&] {
+        CS_REQUIRE(b && trial_steps >= 1, CS_ERR_ARG, "bad argument");
+        // Keep the state: trials run on the live arena and are undone afterwards.
+        void* backup = nullptr;
+        const size_t bytes = (size_t)b->arena_elems * b->esize;
+        CS_HIP(hipMalloc(&backup, bytes));
+        CS_HIP(hipMemcpyAsync(backup, b->arena, bytes, hipMemcpyDeviceToDevice, b->stream));
+        const int cur0 = b->cur;
+        const uint64_t steps0 = b->steps;
+        const cs_batch_tuning keep = b->tuning;
+        cs_batch_tuning win = keep;
+        float win_ms = INFINITY;
+        try {
+            for (int blk : {256, 512, 1024})
+                for (int wpc : {1, 2, 4, 8})
+                    for (int cpt : {1, 2})
+                        for (int ord : {CS_ORDER_LARGEST_FIRST, CS_ORDER_GIVEN}) {
+                            b->tuning = cs_batch_tuning{blk, wpc, cpt, ord};
+                            float t = 0;
+                            timed(b, &t, [&] { launch_batch(b, trial_steps); });
+                            if (t < win_ms) {
+                                win_ms = t;
+                                win = b->tuning;
+                            }
+                        }
+        } catch (...) {
+            b->tuning = keep;
+            (void)hipMemcpyAsync(b->arena, backup, bytes, hipMemcpyDeviceToDevice, b->stream);
+            (void)hipStreamSynchronize(b->stream);
+            (void)hipFree(backup);
+            throw;
+        }
+        hipError_t e1 = hipMemcpyAsync(b->arena, backup, bytes, hipMemcpyDeviceToDevice, b->stream);
+        hipError_t e2 = hipStreamSynchronize(b->stream);
+        (void)hipFree(backup);
+        CS_HIP(e1);
+        CS_HIP(e2);
+        b->cur = cur0;
+        b->steps = steps0;
+        b->tuning = win;
+        if (best) *best = win;
+        if (best_ms) *best_ms = win_ms;
+    });
+}

@@ -74,6 +74,9 @@ struct Consts {
 
 Consts<float> make_consts_f32(const cs_params& p);
 Consts<double> make_consts_f64(const cs_params& p);
+// Constants of the deviation storages (A, B, C rounded once from f64).
+Consts<float> make_consts_dev_f32(const cs_params& p);
+bool is_dev_storage(int storage);
 
 // Node info word (one per cell, built on the device from the flags):
 //   bits 0..8   source cell of direction i is solid (wall or lid)

@@ -68,8 +68,11 @@ __device__ __forceinline__ uint32_t xorshift32(uint32_t x)
     x ^= x << 5;
     return x;
 }
-// Per-step key, computed on the host: lowbias32(seed ^ step * phi).
-inline uint32_t sr_key(uint32_t seed, uint64_t step) { return lowbias32(seed ^ ((uint32_t)step * 0x9e3779b9u)); }
+// Per-step key: lowbias32(seed ^ step * phi).
+__host__ __device__ __forceinline__ uint32_t sr_key(uint32_t seed, uint64_t step)
+{
+    return lowbias32(seed ^ ((uint32_t)step * 0x9e3779b9u));
+}
 
 // S: stored element type.  T: arithmetic type.  DEV: the stored value is the
 // deviation g_i = f_i - w_i*rho0 and the step runs in deviation arithmetic.
@@ -119,6 +122,20 @@ struct StF16 {
 };
 template <class St>
 constexpr bool is_16bit() { return sizeof(typename St::S) == 2; }
+
+// Element e of a buffer addressed by an offset of type Off.  With Off =
+// uint32_t the byte offset is formed in 32 bits (the buffer is < 4 GiB), so
+// the access is a wave-uniform base plus a 32-bit per-lane offset.
+template <class Off, class S>
+__device__ __forceinline__ const S* at(const S* base, Off e)
+{
+    return (const S*)((const char*)base + (Off)(e * (Off)sizeof(S)));
+}
+template <class Off, class S>
+__device__ __forceinline__ S* at(S* base, Off e)
+{
+    return (S*)((char*)base + (Off)(e * (Off)sizeof(S)));
+}
 
 // Boundary handling variants of the step kernel.
 enum StepMode : int {
@@ -258,8 +275,9 @@ __device__ __forceinline__ void zou_he_outlet(T (&B)[9], T rho)
 
 // Gathers the nine post-stream, post-boundary populations of cell (r, c).
 // Out-of-range (r, c) lanes read a clamped in-range address and are masked by
-// the caller.  Returns the cell's own flag (0 without flags).
-template <class St, int EDGE, int MODE>
+// the caller.  Returns the cell's own flag (0 without flags).  Off is the
+// offset type (uint32_t when the 9-plane buffer is < 4 GiB, else int64_t).
+template <class St, int EDGE, int MODE, class Off = int64_t>
 __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, const uint32_t* __restrict__ node,
                                       int w, int h, int pitch, int64_t P, const Consts<typename St::T>& k, int r,
                                       int c, typename St::T (&B)[9])
@@ -286,7 +304,8 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
     rows[1] = r;
     cols[1] = c;
     vr[1] = vc[1] = true;
-    const int64_t own = (int64_t)r * pitch + c;
+    const Off Pl = (Off)P;
+    const Off own = (Off)r * (Off)pitch + (Off)c;
 
     uint32_t info = 0;
     int type = CS_FLUID;
@@ -299,7 +318,7 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         // Normal pull source of direction i.
-        int64_t off = (int64_t)i * P + (int64_t)rows[CX[i] + 1] * pitch + cols[1 - CY[i]];
+        Off off = (Off)i * Pl + (Off)rows[CX[i] + 1] * (Off)pitch + (Off)cols[1 - CY[i]];
         bool valid = vr[CX[i] + 1] && vc[1 - CY[i]];
         bool add_lid = false;
         if (MODE == MODE_REF) {
@@ -307,21 +326,21 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
             // (lbm.rs:745), i.e. the normal source of direction opp(i).
             if (solid) {
                 const int j = OPP[i];
-                off = (int64_t)j * P + (int64_t)rows[CX[j] + 1] * pitch + cols[1 - CY[j]];
+                off = (Off)j * Pl + (Off)rows[CX[j] + 1] * (Off)pitch + (Off)cols[1 - CY[j]];
                 valid = vr[CX[j] + 1] && vc[1 - CY[j]];
             }
         } else if (MODE == MODE_HALF) {
             if (solid) {
-                off = (int64_t)i * P + own;  // carried unchanged
+                off = (Off)i * Pl + own;  // carried unchanged (copied raw in finish)
                 valid = true;
             } else if ((info >> i) & 1u) {
-                off = (int64_t)OPP[i] * P + own;  // half-way bounce-back
+                off = (Off)OPP[i] * Pl + own;  // half-way bounce-back
                 valid = true;
                 add_lid = (info >> (9 + i)) & 1u;
             }
         }
         // A population from outside a zero-filled edge is f = 0, i.e. g = -w_i rho0.
-        T v = St::ld(fin, off);
+        T v = St::ld(at<Off>(fin, off), 0);
         v = valid ? v : (St::DEV ? -k.base[i] : T(0));
         if (MODE == MODE_HALF) v = add_lid ? v + k.lid[i] : v;
         B[i] = v;
@@ -341,15 +360,16 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
 // Collides and stores cell (r, c).  Solid cells of the half-way mode are
 // carried unchanged: their stored values are copied bit for bit.  With SR the
 // 16-bit narrowing is stochastic, keyed by (sr_key, cell index).
-template <class St, int MODE, bool SR>
+template <class St, int MODE, bool SR, class Off = int64_t>
 __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, typename St::S* __restrict__ fout,
                                        int pitch, int64_t P, const Consts<typename St::T>& k, int r, int c,
                                        int type, typename St::T (&B)[9], uint32_t sr_key, uint32_t cell)
 {
-    const int64_t own = (int64_t)r * pitch + c;
+    const Off Pl = (Off)P;
+    const Off own = (Off)r * (Off)pitch + (Off)c;
     if (MODE == MODE_HALF && (type == CS_WALL || type == CS_LID)) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) fout[(int64_t)i * P + own] = fin[(int64_t)i * P + own];
+        for (int i = 0; i < 9; ++i) *at<Off>(fout, (Off)i * Pl + own) = *at<Off>(fin, (Off)i * Pl + own);
         return;
     }
     if constexpr (St::DEV) bgk_collide_dev(k, B);
@@ -358,21 +378,21 @@ __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, t
         uint32_t h = lowbias32(cell ^ sr_key);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            St::st_sr(fout, (int64_t)(2 * j) * P + own, B[2 * j], h);
-            St::st_sr(fout, (int64_t)(2 * j + 1) * P + own, B[2 * j + 1], h << 16);
+            St::st_sr(at<Off>(fout, (Off)(2 * j) * Pl + own), 0, B[2 * j], h);
+            St::st_sr(at<Off>(fout, (Off)(2 * j + 1) * Pl + own), 0, B[2 * j + 1], h << 16);
             h = xorshift32(h);
         }
-        St::st_sr(fout, (int64_t)8 * P + own, B[8], h);
+        St::st_sr(at<Off>(fout, (Off)8 * Pl + own), 0, B[8], h);
     } else {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) St::st(fout, (int64_t)i * P + own, B[i]);
+        for (int i = 0; i < 9; ++i) St::st(at<Off>(fout, (Off)i * Pl + own), 0, B[i]);
     }
 }
 
 // Fused step.  A block is 4 waves; wave y handles row 4*ty + y, lane l the
 // columns 64*CPT*tx + l + 64*j, j < CPT.  All loads of a thread's CPT cells
 // are issued before any arithmetic.
-template <class St, int EDGE, int MODE, int CPT, bool SR>
+template <class St, int EDGE, int MODE, int CPT, bool SR, class Off>
 __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
 {
     using S = typename St::S;
@@ -391,14 +411,14 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
     for (int j = 0; j < CPT; ++j) {
         const int c = c0 + 64 * j;
         const int cc = c < a.w ? c : a.w - 1;
-        type[j] = gather<St, EDGE, MODE>(fin, a.node, a.w, a.h, a.pitch, a.plane, a.k, r, cc, B[j]);
+        type[j] = gather<St, EDGE, MODE, Off>(fin, a.node, a.w, a.h, a.pitch, a.plane, a.k, r, cc, B[j]);
     }
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         const int c = c0 + 64 * j;
         if (c < a.w)
-            finish<St, MODE, SR>(fin, fout, a.pitch, a.plane, a.k, r, c, type[j], B[j], a.sr_key,
-                                 a.cell_base + (uint32_t)(r * a.w + c));
+            finish<St, MODE, SR, Off>(fin, fout, a.pitch, a.plane, a.k, r, c, type[j], B[j], a.sr_key,
+                                      a.cell_base + (uint32_t)(r * a.w + c));
     }
 }
 

@@ -186,6 +186,42 @@ int cs_lattice_residual_f64(cs_lattice* lat, double* residual);
 int cs_lattice_run_to_residual(cs_lattice* lat, double tol, int interval, int64_t max_steps,
                                int64_t* steps, double* residual);
 
+/* ---- batch of independent lattices (extension) --------------------------
+ * Many lattices of uneven sizes stepped by ONE persistent launch: each
+ * workgroup takes whole lattices from a device-side queue and runs all steps
+ * on them with block-level barriers between steps.  Results equal one
+ * cs_lattice_step launch per lattice per step, bit for bit.  Each lattice has
+ * its own cs_params and flags; all share one storage kind. */
+enum cs_batch_order { CS_ORDER_LARGEST_FIRST = 0, CS_ORDER_GIVEN = 1, CS_ORDER_SMALLEST_FIRST = 2 };
+typedef struct cs_batch_tuning {
+    int32_t block_threads;     /* 256, 512 or 1024 */
+    int32_t workgroups_per_cu; /* persistent grid = CUs x this (capped at the lattice count) */
+    int32_t cells_per_thread;  /* 1 or 2 */
+    int32_t order;             /* enum cs_batch_order: queue order */
+} cs_batch_tuning;
+
+int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const int32_t* h, int storage, cs_batch** out);
+int cs_batch_destroy(cs_batch* b);
+int cs_batch_count(const cs_batch* b, int* count);
+int cs_batch_set_params(cs_batch* b, int index, const cs_params* params);
+int cs_batch_set_flags(cs_batch* b, int index, const uint8_t* flags);
+int cs_batch_init_equilibrium(cs_batch* b, int index, const float* rho, const float* ux, const float* uy);
+int cs_batch_upload_f32(cs_batch* b, int index, const float* soa);
+int cs_batch_download_f32(cs_batch* b, int index, float* soa);
+int cs_batch_download_raw(cs_batch* b, int index, void* data);
+int cs_batch_velocity(cs_batch* b, int index, float* ux, float* uy);
+int cs_batch_set_tuning(cs_batch* b, const cs_batch_tuning* tuning);
+int cs_batch_get_tuning(const cs_batch* b, cs_batch_tuning* tuning);
+/* `steps` steps of every lattice in one persistent launch; synchronises and
+ * returns the elapsed milliseconds (HIP events) if ms is not null. */
+int cs_batch_step(cs_batch* b, int steps, float* ms);
+/* The same steps as one fused launch per lattice per step (the comparison
+ * path of the batch). */
+int cs_batch_step_per_lattice(cs_batch* b, int steps, float* ms);
+/* Times every tuning candidate for trial_steps steps (state restored after),
+ * keeps the fastest and returns it. */
+int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* best, float* best_ms);
+
 /* ---- measurement helpers ---------------------------------------------- */
 /* Plain device-to-device copy kernel of `bytes` bytes, `iters` times; returns
  * the average milliseconds per copy (HIP events).  Calibrates bandwidth and

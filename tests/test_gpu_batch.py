@@ -1,0 +1,121 @@
+"""Batched persistent launch: every lattice equals the oracle's run of that
+lattice alone, and the one-launch batch equals one launch per lattice per step,
+bit for bit."""
+import numpy as np
+import pytest
+
+from chemsim_amd import Params, capi
+from chemsim_amd import workloads as W
+from chemsim_amd.batch import Batch
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def build_mixed(ctx, storage="f32"):
+    """Lattices of uneven sizes and every boundary mode."""
+    rng = np.random.default_rng(5)
+    specs = []
+    for i, (w, h) in enumerate([(64, 64), (100, 37), (33, 80), (128, 128), (72, 40), (40, 72), (96, 24),
+                                (17, 19), (150, 64), (64, 150)]):
+        kind = i % 5
+        if kind == 0:      # lid-driven cavity, half-way
+            flags = W.cavity_flags(w, h)
+            op = O.params(w, h, tau=0.55 + 0.05 * i, boundary=O.BB_HALFWAY, lid=W.lid_velocity(0.02 + 0.01 * i))
+        elif kind == 1:    # periodic, no flags
+            flags = None
+            op = O.params(w, h, tau=0.8, edge=O.EDGE_PERIODIC)
+        elif kind == 2:    # reference mode with a random mask, zero edges
+            flags = (rng.random((h, w)) < 0.1).astype(np.uint8)
+            op = O.params(w, h, tau=0.7)
+        elif kind == 3:    # channel with Zou-He
+            ch = W.Channel(w=w, h=h, circles=[(h // 2, w // 3, 4)], re=20.0, u_in=0.04)
+            flags = ch.flags()
+            op = O.params(w, h, tau=ch.tau, boundary=O.BB_HALFWAY, inlet=ch.inlet(), outlet_rho=1.0)
+        else:              # periodic half-way with a moving wall strip
+            flags = np.zeros((h, w), np.uint8)
+            flags[h // 2, :] = O.LID
+            op = O.params(w, h, tau=0.9, edge=O.EDGE_PERIODIC, boundary=O.BB_HALFWAY, lid=(0.0, 0.03))
+        rho = (1 + 0.01 * rng.standard_normal((h, w))).astype(np.float32)
+        ux = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+        uy = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+        specs.append((w, h, op, flags, (rho, ux, uy)))
+    b = Batch(ctx, [(s[0], s[1]) for s in specs], storage)
+    for i, (w, h, op, flags, init) in enumerate(specs):
+        b.set_params(i, Params(tau=op.tau, edge=op.edge, boundary=op.boundary, lid_ux=op.lid_ux, lid_uy=op.lid_uy,
+                               inlet_ux=op.inlet_ux, inlet_uy=op.inlet_uy, outlet_rho=op.outlet_rho,
+                               rounding=capi.CS_ROUND_STOCHASTIC, seed=77 + i))
+        b.set_flags(i, flags)
+        b.init_equilibrium(i, *init)
+    return b, specs
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32dev"])
+def test_batch_matches_oracle(ctx, storage):
+    b, specs = build_mixed(ctx, storage)
+    with b:
+        b.step(7)
+        b.step(6)
+        for i, (w, h, op, flags, init) in enumerate(specs):
+            if storage == "f32":
+                ref = O.run(op, O.equilibrium(op, *init), 13, flags)
+            else:
+                ref = O.from_dev(op, O.run_devf32(op, O.equilibrium_dev(op, *init), 13, flags))
+            np.testing.assert_array_equal(b.download(i), ref, err_msg=f"lattice {i} ({w}x{h})")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_batch_equals_per_lattice_launches(ctx, storage):
+    tunings = [dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0),
+               dict(block_threads=1024, workgroups_per_cu=2, cells_per_thread=2, order=1),
+               dict(block_threads=512, workgroups_per_cu=8, cells_per_thread=2, order=2)]
+    a, _ = build_mixed(ctx, storage)
+    p, _ = build_mixed(ctx, storage)
+    with a, p:
+        p.step_per_lattice(9)
+        for t in tunings:
+            a.tuning = t
+            a.step(3)
+        for i in range(len(a)):
+            np.testing.assert_array_equal(a.download_raw(i), p.download_raw(i), err_msg=f"lattice {i}")
+
+
+def test_batch_16bit_stochastic_matches_oracle(ctx):
+    b, specs = build_mixed(ctx, "f16")
+    with b:
+        b.step(5)
+        for i, (w, h, op, flags, init) in enumerate(specs):
+            d0 = O.to_f16(O.equilibrium_dev(op, *init))
+            ref = O.run_dev16(op, O.F16, d0, 5, flags, rounding=1, seed=77 + i)
+            np.testing.assert_array_equal(b.download_raw(i), ref, err_msg=f"lattice {i}")
+
+
+def test_autotune_keeps_state(ctx):
+    a, _ = build_mixed(ctx, "f32")
+    p, _ = build_mixed(ctx, "f32")
+    with a, p:
+        a.step(4)
+        p.step(4)
+        tuning, ms = a.autotune(3)
+        assert ms > 0 and tuning["block_threads"] in (256, 512, 1024)
+        a.step(5)
+        p.step(5)
+        for i in range(len(a)):
+            np.testing.assert_array_equal(a.download(i), p.download(i))
+
+
+def test_cavity_batch_small(ctx):
+    """A reduced version of the 512-cavity workload: 24 cavities, sides 64..160."""
+    specs = W.batch_cavities(seed=3, count=24, lo=64, hi=160)
+    b = Batch(ctx, [(s.side, s.side) for s in specs], "f32")
+    with b:
+        for i, s in enumerate(specs):
+            b.set_params(i, Params(tau=s.tau, boundary=capi.CS_BB_HALFWAY, lid_uy=s.u_lid))
+            b.set_flags(i, W.cavity_flags(s.side, s.side))
+            b.init_equilibrium(i, *W.rest_state(s.side, s.side))
+        b.step(50)
+        for i in (0, 7, 23):
+            s = specs[i]
+            op = O.params(s.side, s.side, tau=s.tau, boundary=O.BB_HALFWAY, lid=W.lid_velocity(s.u_lid))
+            ref = O.run(op, O.equilibrium(op, *W.rest_state(s.side, s.side)), 50, W.cavity_flags(s.side, s.side))
+            np.testing.assert_array_equal(b.download(i), ref)
