@@ -32,7 +32,14 @@ __device__ __forceinline__ uint16_t f32_to_bf16_bits(float x)
 }
 
 __device__ __forceinline__ float f16_bits_to_f32(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
-__device__ __forceinline__ uint16_t f32_to_f16_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+// Narrowing of a value that is first rounded to f32.  The empty constraint
+// keeps the compiler from fusing the producing multiply into the conversion
+// (v_fma_mixlo_f16 rounds once, not twice, and so differs on rare ties).
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float x)
+{
+    asm volatile("" : "+v"(x));
+    return __builtin_bit_cast(uint16_t, (_Float16)x);
+}
 
 // Stochastic rounding with gfx950's v_cvt_sr_{bf16,f16}_f32: the magnitude
 // plus random bits below the kept mantissa (top 16 bits of r for bf16, top 13

@@ -178,6 +178,23 @@ def test_dev16_matches_oracle(ctx, kind, halfway):
 
 
 @pytest.mark.parametrize("kind", ["bf16", "f16"])
+def test_dev16_init_rounds_once_to_f32_then_16bit(ctx, kind):
+    """Random (rho, u) give ~2^-13 of the values on a rounding tie; the stored
+    16-bit value must be the f32 equilibrium rounded to nearest even (a fused
+    multiply-convert would round once and differ on those ties)."""
+    rng = np.random.default_rng(19)
+    h, w = 200, 300
+    rho = (1 + 0.01 * rng.standard_normal((h, w))).astype(np.float32)
+    ux = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+    uy = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+    op = O.params(w, h, tau=0.7)
+    okind = O.BF16 if kind == "bf16" else O.F16
+    with Lattice(ctx, w, h, kind, gpu_params(op)) as lat:
+        lat.init_equilibrium(rho, ux, uy)
+        np.testing.assert_array_equal(lat.download_raw(), narrow16(okind, O.equilibrium_dev(op, rho, ux, uy)))
+
+
+@pytest.mark.parametrize("kind", ["bf16", "f16"])
 @pytest.mark.parametrize("halfway", [False, True])
 def test_dev16_stochastic_rounding_matches_oracle(ctx, kind, halfway):
     n, m = 40, 64
