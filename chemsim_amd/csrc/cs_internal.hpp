@@ -61,15 +61,36 @@ constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 // Per-run constants in the working precision T (float or double):
 // K = -dt/tau (lbm.rs:357); A, B, C of compute_equilibrium (lbm.rs:54-66);
 // weights (lbm.rs:209-219); Ladd lid terms 6 w_i (c_i . u_lid); deviation
-// storage bases w_i * rho0 and rho0; Zou-He boundary values.
+// storage bases w_i * rho0 and rho0; Zou-He boundary values.  Only distinct
+// values are stored (three weight classes; the lid terms of directions
+// 3, 4, 7, 8 are the negatives of 1, 2, 5, 6), so a kernel keeps them all in
+// scalar registers.
 template <class T>
 struct Consts {
     T K, A, B, C, cs2;
-    T w[9];
-    T lid[9];
-    T base[9];
+    T wq[3];     // w of the rest, axis and diagonal directions
+    T baseq[3];  // w * rho0 of the same classes
+    T lidq[4];   // lid term of directions 1, 2, 5, 6
     T in_ux, in_uy, out_rho;
     T rho0;
+
+    __host__ __device__ static constexpr int wclass(int i) { return i == 0 ? 0 : (i < 5 ? 1 : 2); }
+    __host__ __device__ T w(int i) const { return wq[wclass(i)]; }
+    __host__ __device__ T base(int i) const { return baseq[wclass(i)]; }
+    __host__ __device__ T lid(int i) const
+    {
+        switch (i) {
+        case 1: return lidq[0];
+        case 2: return lidq[1];
+        case 3: return -lidq[0];
+        case 4: return -lidq[1];
+        case 5: return lidq[2];
+        case 6: return lidq[3];
+        case 7: return -lidq[2];
+        case 8: return -lidq[3];
+        default: return T(0);
+        }
+    }
 };
 
 Consts<float> make_consts_f32(const cs_params& p);

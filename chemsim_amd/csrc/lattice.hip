@@ -54,13 +54,18 @@ static Consts<T> make_consts(const cs_params& p)
     k.K = -dt / tau;
     volatile T n0 = 16, n1 = 4, n5 = 1, d = 36;
     const T w0 = n0 / d, w1 = n1 / d, w5 = n5 / d;
-    const T ws[9] = {w0, w1, w1, w1, w1, w5, w5, w5, w5};
-    for (int i = 0; i < 9; ++i) {
-        k.w[i] = ws[i];
-        // Ladd moving wall: f_i = f*_opp(i) + 2 w_i rho_w (c_i . u_w) / cs^2,
-        // rho_w = 1, cs^2 = 1/3 (extension).
-        k.lid[i] = (T)(6.0 * W_EXACT[i] * (CX[i] * p.lid_ux + CY[i] * p.lid_uy));
-        k.base[i] = ws[i] * (T)p.rho0;
+    const T ws[3] = {w0, w1, w5};
+    for (int q = 0; q < 3; ++q) {
+        k.wq[q] = ws[q];
+        k.baseq[q] = ws[q] * (T)p.rho0;
+    }
+    // Ladd moving wall: f_i = f*_opp(i) + 2 w_i rho_w (c_i . u_w) / cs^2,
+    // rho_w = 1, cs^2 = 1/3 (extension).  Directions 1, 2, 5, 6; the opposite
+    // ones carry the exact negatives.
+    const int dirs[4] = {1, 2, 5, 6};
+    for (int q = 0; q < 4; ++q) {
+        const int i = dirs[q];
+        k.lidq[q] = (T)(6.0 * W_EXACT[i] * (CX[i] * p.lid_ux + CY[i] * p.lid_uy));
     }
     k.in_ux = (T)p.inlet_ux;
     k.in_uy = (T)p.inlet_uy;

@@ -45,7 +45,7 @@ __device__ __forceinline__ void equilibrium(const Consts<T>& k, T rho, T ux, T u
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const T s = ((T(1) + vc[i] * k.A) + (vc[i] * vc[i]) * k.B) + v2 * k.C;
-        F[i] = (rho * k.w[i]) * s;
+        F[i] = (rho * k.w(i)) * s;
     }
 }
 
@@ -60,7 +60,7 @@ __device__ __forceinline__ void equilibrium_dev(const Consts<T>& k, T rho, T ux,
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const T s = (vc[i] * k.A + (vc[i] * vc[i]) * k.B) + v2 * k.C;
-        G[i] = k.w[i] * (drho + rho * s);
+        G[i] = k.w(i) * (drho + rho * s);
     }
 }
 
@@ -101,7 +101,7 @@ __global__ void k_from_f32(const float* __restrict__ src, typename St::S* __rest
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const typename St::T v = (typename St::T)src[i * P + own];
-        St::st(dst, i * P + own, St::DEV ? v - k.base[i] : v);
+        St::st(dst, i * P + own, St::DEV ? v - k.base(i) : v);
     }
 }
 
@@ -116,7 +116,7 @@ __global__ void k_to_f32(const typename St::S* __restrict__ src, float* __restri
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const typename St::T v = St::ld(src, i * P + own);
-        dst[i * P + own] = (float)(St::DEV ? v + k.base[i] : v);
+        dst[i * P + own] = (float)(St::DEV ? v + k.base(i) : v);
     }
 }
 

@@ -192,7 +192,7 @@ __device__ __forceinline__ void bgk_collide(const Consts<T>& k, T (&B)[9])
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const T s = ((T(1) + vc[i] * k.A) + (vc[i] * vc[i]) * k.B) + v2 * k.C;
-        const T feq = (rho * k.w[i]) * s;
+        const T feq = (rho * k.w(i)) * s;
         B[i] = B[i] + (B[i] - feq) * k.K;
     }
 }
@@ -217,7 +217,7 @@ __device__ __forceinline__ void bgk_collide_dev(const Consts<T>& k, T (&B)[9])
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const T s = (vc[i] * k.A + (vc[i] * vc[i]) * k.B) + v2 * k.C;
-        const T geq = k.w[i] * (drho + rho * s);
+        const T geq = k.w(i) * (drho + rho * s);
         B[i] = B[i] + (B[i] - geq) * k.K;
     }
 }
@@ -348,8 +348,8 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
         }
         // A population from outside a zero-filled edge is f = 0, i.e. g = -w_i rho0.
         T v = St::ld(at<Off>(fin, off), 0);
-        v = valid ? v : (St::DEV ? -k.base[i] : T(0));
-        if (MODE == MODE_HALF) v = add_lid ? v + k.lid[i] : v;
+        v = valid ? v : (St::DEV ? -k.base(i) : T(0));
+        if (MODE == MODE_HALF) v = add_lid ? v + k.lid(i) : v;
         B[i] = v;
     }
     if (MODE == MODE_HALF) {
