@@ -74,6 +74,23 @@ class Batch:
         call("cs_batch_velocity", self.handle, int(i), _ptr(ux), _ptr(uy))
         return ux, uy
 
+    def set_active(self, i: int, active: bool) -> None:
+        call("cs_batch_set_active", self.handle, int(i), 1 if active else 0)
+
+    def snapshot_velocity(self) -> None:
+        call("cs_batch_snapshot_velocity", self.handle)
+
+    def residuals(self) -> np.ndarray:
+        out = np.empty(len(self.sizes), np.float64)
+        call("cs_batch_residuals", self.handle, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def moments(self, i: int):
+        w, h = self.sizes[i]
+        outs = [np.empty((h, w), np.float32) for _ in range(3)]
+        call("cs_batch_moments", self.handle, int(i), *[_ptr(o) for o in outs])
+        return tuple(outs)
+
     @property
     def tuning(self) -> dict:
         t = capi.cs_batch_tuning()

@@ -131,6 +131,23 @@ SIGNATURES = {
     "cs_batch_step": [_P, _I, _FP],
     "cs_batch_step_per_lattice": [_P, _I, _FP],
     "cs_batch_autotune": [_P, _I, C.POINTER(cs_batch_tuning), _FP],
+    "cs_batch_set_active": [_P, _I, _I],
+    "cs_batch_snapshot_velocity": [_P],
+    "cs_batch_residuals": [_P, _DP],
+    "cs_batch_moments": [_P, _I, _P, _P, _P],
+    "cs_cnn_create": [_P, _I, _I, _I, _I, C.c_uint32, C.POINTER(_P)],
+    "cs_cnn_destroy": [_P],
+    "cs_cnn_param_count": [_P, _I64P],
+    "cs_cnn_get_params": [_P, _P],
+    "cs_cnn_set_params": [_P, _P],
+    "cs_cnn_get_grads": [_P, _P],
+    "cs_cnn_forward": [_P, _I, _P, _P],
+    "cs_cnn_loss_grad": [_P, _I, _P, _P, _P, _FP],
+    "cs_cnn_adam_step": [_P, _F, _F, _F, _F],
+    "cs_cnn_set_dataset": [_P, _I, _P, _P, _P],
+    "cs_cnn_train_epoch": [_P, _P, _I, _F, _FP],
+    "cs_cnn_time_step": [_P, _I, _FP, _DP],
+    "cs_gemm_bf16": [_P, _I, _I, _I, _P, _P, _P, _FP],
     "cs_bench_copy": [_P, _I64, _I, _FP],
     "cs_set_cells_per_thread": [_I],
 }

@@ -98,6 +98,76 @@ __global__ __launch_bounds__(1024) void k_batch(const BatchLat<typename St::T>* 
     }
 }
 
+// Per-lattice velocity snapshot (one block per lattice).
+template <class St>
+__global__ __launch_bounds__(256) void k_batch_snapshot(const BatchLat<typename St::T>* __restrict__ lats, int n,
+                                                        const void* __restrict__ arena, int cur, float* __restrict__ snap,
+                                                        const int64_t* __restrict__ snap_off)
+{
+    using T = typename St::T;
+    const int li = blockIdx.x;
+    if (li >= n) return;
+    const BatchLat<T> L = lats[li];
+    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)cur * 9 * L.plane;
+    const int64_t cells = (int64_t)L.w * L.h;
+    float* s = snap + snap_off[li];
+    for (int64_t d = threadIdx.x; d < cells; d += 256) {
+        const int r = (int)(d / L.w), c = (int)(d - (int64_t)r * L.w);
+        const int64_t own = (int64_t)r * L.pitch + c;
+        T B[9];
+        load_own<St>(f, L.plane, own, B);
+        T rho, mx, my, ux, uy;
+        moments<St>(L.k, B, rho, mx, my, ux, uy);
+        s[d] = (float)ux;
+        s[cells + d] = (float)uy;
+    }
+}
+
+// Per-lattice residual ||u - u_snap|| / ||u|| over non-solid cells, f64.
+template <class St>
+__global__ __launch_bounds__(256) void k_batch_residual(const BatchLat<typename St::T>* __restrict__ lats, int n,
+                                                        const void* __restrict__ arena, int cur,
+                                                        const uint32_t* __restrict__ nodes,
+                                                        const float* __restrict__ snap,
+                                                        const int64_t* __restrict__ snap_off, double* __restrict__ out)
+{
+    using T = typename St::T;
+    __shared__ double sh[2][256];
+    const int li = blockIdx.x;
+    if (li >= n) return;
+    const BatchLat<T> L = lats[li];
+    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)cur * 9 * L.plane;
+    const int64_t cells = (int64_t)L.w * L.h;
+    const float* s = snap + snap_off[li];
+    double num = 0.0, den = 0.0;
+    for (int64_t d = threadIdx.x; d < cells; d += 256) {
+        const int r = (int)(d / L.w), c = (int)(d - (int64_t)r * L.w);
+        const int64_t own = (int64_t)r * L.pitch + c;
+        if (L.mode & BM_FLAGS) {
+            const int t = (int)(nodes[L.node_off + own] >> NODE_TYPE_SHIFT);
+            if (t == CS_WALL || t == CS_LID) continue;
+        }
+        T B[9];
+        load_own<St>(f, L.plane, own, B);
+        T rho, mx, my, ux, uy;
+        moments<St>(L.k, B, rho, mx, my, ux, uy);
+        const double dx = (double)(float)ux - (double)s[d], dy = (double)(float)uy - (double)s[cells + d];
+        num += dx * dx + dy * dy;
+        den += (double)ux * (double)ux + (double)uy * (double)uy;
+    }
+    sh[0][threadIdx.x] = num;
+    sh[1][threadIdx.x] = den;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + k];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[li] = sh[1][0] > 0 ? sqrt(sh[0][0] / sh[1][0]) : (sh[0][0] > 0 ? INFINITY : 0.0);
+}
+
 // Mode group of a lattice: periodic, mode (fluid/ref/half), stochastic rounding.
 inline int mode_group(int mode_bits)
 {
@@ -117,6 +187,11 @@ struct cs_batch {
     std::vector<int64_t> plane, off, node_off;
     std::vector<cs_params> params;
     std::vector<uint8_t> has_flags;
+    std::vector<uint8_t> active;
+    std::vector<int64_t> snap_off;    // offset of each lattice's (ux, uy) snapshot
+    int64_t snap_elems = 0;
+    float* snap = nullptr;
+    double* d_res = nullptr;
     int64_t arena_elems = 0, node_elems = 0;
     void* arena = nullptr;
     uint32_t* nodes = nullptr;
@@ -201,8 +276,9 @@ static void upload_descriptors(cs_batch* b)
 // returns (group, first, count) triples into d_order.
 static std::vector<std::array<int, 3>> upload_order(cs_batch* b)
 {
-    std::vector<int> ord((size_t)b->n);
-    std::iota(ord.begin(), ord.end(), 0);
+    std::vector<int> ord;
+    for (int i = 0; i < b->n; ++i)
+        if (b->active[(size_t)i]) ord.push_back(i);
     auto cells = [&](int i) { return (int64_t)b->w[(size_t)i] * b->h[(size_t)i]; };
     if (b->tuning.order == CS_ORDER_LARGEST_FIRST)
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return cells(a) > cells(c); });
@@ -218,12 +294,13 @@ static std::vector<std::array<int, 3>> upload_order(cs_batch* b)
     }
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return grp[(size_t)a] < grp[(size_t)c]; });
     std::vector<std::array<int, 3>> groups;
-    for (int k = 0; k < b->n; ++k) {
+    for (int k = 0; k < (int)ord.size(); ++k) {
         const int g = grp[(size_t)ord[(size_t)k]];
         if (groups.empty() || groups.back()[0] != g) groups.push_back({g, k, 0});
         groups.back()[2] += 1;
     }
-    CS_HIP(hipMemcpyAsync(b->d_order, ord.data(), sizeof(int) * (size_t)b->n, hipMemcpyHostToDevice, b->stream));
+    if (!ord.empty())
+        CS_HIP(hipMemcpyAsync(b->d_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, b->stream));
     return groups;
 }
 
@@ -286,6 +363,7 @@ static void launch_per_lattice(cs_batch* b, int steps)
         for (int s = 0; s < steps; ++s) {
             const int cur = (b->cur + s) & 1;
             for (int i = 0; i < b->n; ++i) {
+                if (!b->active[(size_t)i]) continue;
                 const BatchLat<T>& d = L[(size_t)i];
                 StepArgs<T> a;
                 typename St::S* base = (typename St::S*)b->arena + d.off;
@@ -375,6 +453,11 @@ extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const i
         p.rho0 = 1.0;
         b->params.assign((size_t)count, p);
         b->has_flags.assign((size_t)count, 0);
+        b->active.assign((size_t)count, 1);
+        for (int i = 0; i < count; ++i) {
+            b->snap_off.push_back(b->snap_elems);
+            b->snap_elems += 2 * (int64_t)w[i] * h[i];
+        }
         try {
             CS_HIP(hipMalloc(&b->arena, (size_t)acc * b->esize));
             CS_HIP(hipMemset(b->arena, 0, (size_t)acc * b->esize));
@@ -398,7 +481,8 @@ extern "C" int cs_batch_destroy(cs_batch* b)
     return guarded([&] {
         if (!b) return;
         if (b->stream) (void)hipStreamSynchronize(b->stream);
-        for (void* p : {b->arena, (void*)b->nodes, b->d_lats, (void*)b->d_order, (void*)b->d_queue})
+        for (void* p : {b->arena, (void*)b->nodes, b->d_lats, (void*)b->d_order, (void*)b->d_queue, (void*)b->snap,
+                        (void*)b->d_res})
             if (p) (void)hipFree(p);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -684,5 +768,85 @@ extern "C" int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* 
         b->tuning = win;
         if (best) *best = win;
         if (best_ms) *best_ms = win_ms;
+    });
+}
+
+extern "C" int cs_batch_set_active(cs_batch* b, int index, int active)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        b->active[(size_t)index] = active ? 1 : 0;
+    });
+}
+
+extern "C" int cs_batch_snapshot_velocity(cs_batch* b)
+{
+    return guarded([&] {
+        CS_REQUIRE(b, CS_ERR_ARG, "null batch");
+        if (!b->snap) {
+            CS_HIP(hipMalloc(&b->snap, sizeof(float) * (size_t)b->snap_elems));
+            CS_HIP(hipMalloc(&b->d_res, sizeof(double) * (size_t)b->n + sizeof(int64_t) * (size_t)b->n));
+            CS_HIP(hipMemcpy((char*)b->d_res + sizeof(double) * (size_t)b->n, b->snap_off.data(),
+                             sizeof(int64_t) * (size_t)b->n, hipMemcpyHostToDevice));
+        }
+        upload_descriptors(b);
+        const int64_t* soff = (const int64_t*)((char*)b->d_res + sizeof(double) * (size_t)b->n);
+        dispatch_batch_storage(b->storage, [&]<class St>() {
+            hipLaunchKernelGGL((k_batch_snapshot<St>), dim3((unsigned)b->n), dim3(256), 0, b->stream,
+                               (const BatchLat<typename St::T>*)b->d_lats, b->n, b->arena, b->cur, b->snap, soff);
+        });
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+extern "C" int cs_batch_residuals(cs_batch* b, double* residuals)
+{
+    return guarded([&] {
+        CS_REQUIRE(b && residuals, CS_ERR_ARG, "null argument");
+        CS_REQUIRE(b->snap != nullptr, CS_ERR_STATE, "residuals requested before a velocity snapshot");
+        upload_descriptors(b);
+        const int64_t* soff = (const int64_t*)((char*)b->d_res + sizeof(double) * (size_t)b->n);
+        dispatch_batch_storage(b->storage, [&]<class St>() {
+            hipLaunchKernelGGL((k_batch_residual<St>), dim3((unsigned)b->n), dim3(256), 0, b->stream,
+                               (const BatchLat<typename St::T>*)b->d_lats, b->n, b->arena, b->cur, b->nodes, b->snap,
+                               soff, b->d_res);
+        });
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipMemcpyAsync(residuals, b->d_res, sizeof(double) * (size_t)b->n, hipMemcpyDeviceToHost, b->stream));
+        CS_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+extern "C" int cs_batch_moments(cs_batch* b, int index, float* rho, float* ux, float* uy)
+{
+    return guarded([&] {
+        check_batch(b, index);
+        CS_REQUIRE(rho && ux && uy, CS_ERR_ARG, "null output");
+        const size_t i = (size_t)index;
+        const int64_t n = (int64_t)b->w[i] * b->h[i];
+        double* dev = nullptr;
+        CS_HIP(hipMalloc(&dev, sizeof(double) * 3 * (size_t)n));
+        dispatch_batch_storage(b->storage, [&]<class St>() {
+            dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
+            hipLaunchKernelGGL((k_moments<St>), g, dim3(256), 0, b->stream,
+                               (const typename St::S*)lattice_buf(b, index), b->w[i], b->h[i], b->pitch[i],
+                               b->plane[i], batch_consts<typename St::T>(b, index), (unsigned)(M_RHO | M_UX | M_UY),
+                               dev);
+        });
+        std::vector<double> host((size_t)(3 * n));
+        hipError_t e = hipGetLastError();
+        hipError_t e1 = hipMemcpyAsync(host.data(), dev, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost,
+                                       b->stream);
+        hipError_t e2 = hipStreamSynchronize(b->stream);
+        (void)hipFree(dev);
+        CS_HIP(e);
+        CS_HIP(e1);
+        CS_HIP(e2);
+        for (int64_t d = 0; d < n; ++d) {
+            rho[d] = (float)host[(size_t)d];
+            ux[d] = (float)host[(size_t)(n + d)];
+            uy[d] = (float)host[(size_t)(2 * n + d)];
+        }
     });
 }

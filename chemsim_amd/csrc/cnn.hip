@@ -1,0 +1,985 @@
+// Warm-start network: a 4-level convolutional encoder-decoder that maps an
+// obstacle mask, the Reynolds number and position channels to the steady
+// (rho, u) field of a channel flow.  Every FLOP runs in the kernels of
+// cnn_kernels.hpp and the small element-wise kernels below (BF16 operands,
+// FP32 accumulation, FP32 master weights with Adam).
+//
+// Layers (base channels c, input 8 channels, all convolutions 3x3, pad 1):
+//   e1: 8->c->c          pool
+//   e2: c->2c->2c        pool
+//   e3: 2c->4c->4c       pool
+//   b : 4c->8c->8c       up, concat e3
+//   d3: 12c->4c->4c      up, concat e2
+//   d2: 6c->2c->2c       up, concat e1
+//   d1: 3c->c->c
+//   head: c->8 (channels 0..2 used), no activation
+// ReLU after every convolution except the head; 2x2 average pooling; nearest
+// 2x up-sampling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "cnn_kernels.hpp"
+#include "cs_internal.hpp"
+
+namespace cs {
+namespace cnn {
+
+// ------------------------------------------------------- element-wise ----
+__global__ void k_f32_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = bf16_rne(x[i]);
+}
+
+__global__ void k_bf16_to_f32(const uint16_t* __restrict__ x, float* __restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = bf16_f32(x[i]);
+}
+
+// Weights [cout][kpad] f32 -> bf16 copy and the transposed copy used by the
+// data gradient: wd[ci][tap*cout + co] = w[co][tap*cin + ci].
+__global__ void k_prepare_weights(const float* __restrict__ w, uint16_t* __restrict__ wb, uint16_t* __restrict__ wd,
+                                  int cin, int cout, int kpad, int kpad_d)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)cout * kpad) return;
+    const int co = (int)(i / kpad), k = (int)(i % kpad);
+    const uint16_t v = bf16_rne(w[i]);
+    wb[i] = v;
+    if (k < 9 * cin && wd) {
+        const int tap = k / cin, ci = k - tap * cin;
+        wd[(int64_t)ci * kpad_d + tap * cout + co] = v;
+    }
+}
+
+// 2x2 average pooling (NHWC).
+__global__ void k_pool(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int N, int H, int W, int C)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t n_out = (int64_t)N * Ho * Wo * C;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const int c = (int)(i % C);
+    int64_t t = i / C;
+    const int xo = (int)(t % Wo);
+    t /= Wo;
+    const int yo = (int)(t % Ho);
+    const int64_t n = t / Ho;
+    const int64_t b = ((n * H + 2 * yo) * W + 2 * xo) * C + c;
+    const float s = (bf16_f32(x[b]) + bf16_f32(x[b + C])) + (bf16_f32(x[b + (int64_t)W * C]) +
+                                                               bf16_f32(x[b + (int64_t)W * C + C]));
+    y[i] = bf16_rne(0.25f * s);
+}
+
+// Pooling backward: dx += dy / 4 on each of the four inputs (accumulates
+// into an f32 gradient buffer of the pooled tensor's input).
+__global__ void k_pool_back(const uint16_t* __restrict__ dy, float* __restrict__ dx, int N, int H, int W, int C)
+{
+    const int64_t n_in = (int64_t)N * H * W * C;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_in) return;
+    const int c = (int)(i % C);
+    int64_t t = i / C;
+    const int x = (int)(t % W);
+    t /= W;
+    const int y = (int)(t % H);
+    const int64_t n = t / H;
+    const int Ho = H / 2, Wo = W / 2;
+    dx[i] += 0.25f * bf16_f32(dy[((n * Ho + y / 2) * Wo + x / 2) * C + c]);
+}
+
+// Nearest 2x up-sampling of x (C1 channels) into channels [0, C1) of the
+// concatenation buffer y (Ct channels), and copy of the skip tensor s (C2
+// channels) into channels [C1, C1 + C2).
+__global__ void k_up_concat(const uint16_t* __restrict__ x, const uint16_t* __restrict__ s, uint16_t* __restrict__ y,
+                            int N, int H, int W, int C1, int C2)
+{
+    const int Ct = C1 + C2;
+    const int64_t n_out = (int64_t)N * H * W * Ct;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const int c = (int)(i % Ct);
+    const int64_t p = i / Ct;
+    if (c < C1) {
+        const int xx = (int)(p % W);
+        const int64_t t = p / W;
+        const int yy = (int)(t % H);
+        const int64_t n = t / H;
+        y[i] = x[((n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C1 + c];
+    } else {
+        y[i] = s[p * C2 + (c - C1)];
+    }
+}
+
+// Backward of k_up_concat: dx (f32, coarse) = sum of the 4 children of the
+// first C1 channels; ds (f32) += the last C2 channels.
+__global__ void k_up_concat_back(const uint16_t* __restrict__ dy, float* __restrict__ dx, float* __restrict__ ds,
+                                 int N, int H, int W, int C1, int C2)
+{
+    const int Ct = C1 + C2;
+    const int Hc = H / 2, Wc = W / 2;
+    const int64_t n_c = (int64_t)N * Hc * Wc * C1;
+    const int64_t n_s = (int64_t)N * H * W * C2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_c) {
+        const int c = (int)(i % C1);
+        int64_t t = i / C1;
+        const int xc = (int)(t % Wc);
+        t /= Wc;
+        const int yc = (int)(t % Hc);
+        const int64_t n = t / Hc;
+        float s = 0.f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                s += bf16_f32(dy[((n * H + 2 * yc + a) * W + 2 * xc + b) * Ct + c]);
+        dx[i] = s;
+    }
+    if (i < n_s) {
+        const int c = (int)(i % C2);
+        const int64_t p = i / C2;
+        ds[i] += bf16_f32(dy[p * Ct + C1 + c]);
+    }
+}
+
+// dz = dy * (y > 0) for a ReLU output y; dy is f32, dz bf16.  Also used with
+// relu = 0 to convert f32 gradients to bf16.
+__global__ void k_relu_back(const float* __restrict__ dy, const uint16_t* __restrict__ y, uint16_t* __restrict__ dz,
+                            int64_t n, int relu)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float g = dy[i];
+    if (relu && !(bf16_f32(y[i]) > 0.f)) g = 0.f;
+    dz[i] = bf16_rne(g);
+}
+
+// Bias gradient: db[c] = sum_p dz[p][c] (one block per channel, fixed order).
+__global__ void k_bias_grad(const uint16_t* __restrict__ dz, float* __restrict__ db, int64_t P, int C)
+{
+    __shared__ float sh[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (int64_t p = threadIdx.x; p < P; p += 256) s += bf16_f32(dz[p * C + c]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = sh[0];
+}
+
+// Sum of split-K partials in fixed order.
+__global__ void k_reduce_splits(const float* __restrict__ part, float* __restrict__ out, int64_t n, int splits)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < splits; ++z) s += part[(int64_t)z * n + i];
+    out[i] = s;
+}
+
+// Weighted MSE over the first `used` channels of the f32 head output y
+// ([P][8]): loss partials per block and dy = 2 w (y - t) / norm.
+__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ w,
+                      float* __restrict__ dy, float* __restrict__ partial, int64_t P, int used, float inv_norm)
+{
+    __shared__ float sh[256];
+    float s = 0.f;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < P; p += (int64_t)gridDim.x * 256) {
+        const float wt = w[p];
+        for (int c = 0; c < 8; ++c) {
+            float g = 0.f;
+            if (c < used) {
+                const float d = y[p * 8 + c] - t[p * used + c];
+                s += wt * d * d;
+                g = 2.f * wt * d * inv_norm;
+            }
+            dy[p * 8 + c] = g;
+        }
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int64_t n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Gathers minibatch samples from the device dataset (inputs bf16 [S][HW][8],
+// targets f32 [S][HW][3], weights f32 [S][HW]) by index list.
+__global__ void k_gather_batch(const uint16_t* __restrict__ din, const float* __restrict__ dt,
+                               const float* __restrict__ dw, const int* __restrict__ idx, int n, int64_t hw,
+                               uint16_t* __restrict__ bin, float* __restrict__ bt, float* __restrict__ bw)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * hw) return;
+    const int b = (int)(i / hw);
+    const int64_t p = i % hw;
+    const int64_t s = (int64_t)idx[b] * hw + p;
+    *(u32x4*)(bin + i * 8) = *(const u32x4*)(din + s * 8);
+    for (int c = 0; c < 3; ++c) bt[i * 3 + c] = dt[s * 3 + c];
+    bw[i] = dw[s];
+}
+
+}  // namespace cnn
+}  // namespace cs
+
+using namespace cs;
+using namespace cs::cnn;
+
+namespace {
+
+inline int pad32(int k) { return (k + 31) / 32 * 32; }
+inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+struct Conv {
+    int cin = 0, cout = 0, kpad = 0, kpad_d = 0;
+    bool relu = true;
+    float *w = nullptr, *b = nullptr, *gw = nullptr, *gb = nullptr;
+    float *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;
+    uint16_t *wb = nullptr, *wd = nullptr;
+    int64_t nw() const { return (int64_t)cout * kpad; }
+};
+
+struct Act {  // NHWC bf16 activation and its shape
+    uint16_t* d = nullptr;
+    int N = 0, H = 0, W = 0, C = 0;
+    int64_t P() const { return (int64_t)N * H * W; }
+    int64_t n() const { return P() * C; }
+};
+
+}  // namespace
+
+struct cs_cnn {
+    cs_ctx* ctx = nullptr;
+    int H = 0, W = 0, base = 0, batch = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Conv> convs;          // 19 convolutions in forward order
+    std::vector<void*> allocs;
+    // activations: inputs of every conv and every conv output
+    std::vector<Act> conv_in, conv_out;
+    Act x_in, p1, p2, p3, cat3, cat2, cat1;
+    float* head_out = nullptr;        // [P][8] f32
+    // scratch
+    uint16_t* col = nullptr;
+    int64_t col_elems = 0;
+    float* part = nullptr;
+    int64_t part_elems = 0;
+    float* dy32 = nullptr;            // two f32 gradient buffers of act_max elements each
+    int64_t act_max = 0;
+    float* skip32[3] = {nullptr, nullptr, nullptr};  // accumulated gradients of e1, e2, e3 outputs
+    uint16_t* dz = nullptr;           // bf16 gradient (largest activation)
+    float* loss_part = nullptr;
+    float* targets = nullptr;
+    float* weights = nullptr;
+    int adam_t = 0;
+    // device dataset
+    uint16_t* ds_in = nullptr;
+    float* ds_t = nullptr;
+    float* ds_w = nullptr;
+    int ds_count = 0;
+    int* d_idx = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double gemm_flops = 0.0;          // FLOPs of the last forward/backward
+};
+
+namespace {
+
+template <class T>
+T* dalloc(cs_cnn* net, int64_t n)
+{
+    void* p = nullptr;
+    CS_HIP(hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+    net->allocs.push_back(p);
+    return (T*)p;
+}
+
+void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, const uint16_t* B, int ldb, void* C, int ldc,
+          const float* bias, int M, int N, int K, int splits = 1)
+{
+    GemmArgs g;
+    g.A = A;
+    g.B = B;
+    g.C = C;
+    g.bias = bias;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.ldc = ldc;
+    g.k_per_split = (K / splits + BK - 1) / BK * BK;
+    g.split_stride = (int64_t)M * ldc;
+    dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN), (unsigned)splits);
+    net->gemm_flops += 2.0 * M * N * K;
+#define CS_GEMM(LA, LB, EPI) hipLaunchKernelGGL((k_gemm<LA, LB, EPI>), grid, dim3(256), 0, net->stream, g)
+    if (la == KMAJOR && lb == KMAJOR) {
+        if (epi == EPI_F32) CS_GEMM(KMAJOR, KMAJOR, EPI_F32);
+        else if (epi == EPI_BF16) CS_GEMM(KMAJOR, KMAJOR, EPI_BF16);
+        else CS_GEMM(KMAJOR, KMAJOR, EPI_BF16_RELU);
+    } else if (la == RMAJOR && lb == RMAJOR && epi == EPI_F32) {
+        CS_GEMM(RMAJOR, RMAJOR, EPI_F32);
+    } else {
+        throw Error{CS_ERR_ARG, "unsupported gemm layout"};
+    }
+#undef CS_GEMM
+    CS_HIP(hipGetLastError());
+}
+
+void im2col(cs_cnn* net, const Act& x, int kpad, int sign)
+{
+    const int64_t groups = x.P() * (kpad / 8);
+    CS_REQUIRE(x.P() * kpad <= net->col_elems, CS_ERR_STATE, "im2col scratch too small");
+    hipLaunchKernelGGL(k_im2col, dim3(blocks_for(groups)), dim3(256), 0, net->stream, x.d, net->col, x.N, x.H, x.W,
+                       x.C, kpad, sign);
+    CS_HIP(hipGetLastError());
+}
+
+void conv_forward(cs_cnn* net, int li, const Act& x, const Act& y, float* y32 = nullptr)
+{
+    Conv& c = net->convs[(size_t)li];
+    im2col(net, x, c.kpad, +1);
+    if (y32) gemm(net, KMAJOR, KMAJOR, EPI_F32, net->col, c.kpad, c.wb, c.kpad, y32, 8, c.b, (int)x.P(), c.cout, c.kpad);
+    else gemm(net, KMAJOR, KMAJOR, c.relu ? EPI_BF16_RELU : EPI_BF16, net->col, c.kpad, c.wb, c.kpad, y.d, c.cout, c.b,
+              (int)x.P(), c.cout, c.kpad);
+}
+
+// Backward of conv li given the f32 gradient dy of its output (post-ReLU).
+// Writes dW, dB and, when dx32 is not null, the f32 input gradient.
+void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float* dy, float* dx32)
+{
+    Conv& c = net->convs[(size_t)li];
+    const int64_t P = x.P();
+    hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(P * c.cout)), dim3(256), 0, net->stream, dy, y.d, net->dz,
+                       P * c.cout, c.relu ? 1 : 0);
+    CS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)c.cout), dim3(256), 0, net->stream, net->dz, c.gb, P, c.cout);
+    CS_HIP(hipGetLastError());
+    // weight gradient: dW[co][k] = sum_p dz[p][co] col[p][k], split over p
+    im2col(net, x, c.kpad, +1);
+    int splits = (int)std::max<int64_t>(1, std::min<int64_t>(64, P / 2048));
+    while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
+    gemm(net, RMAJOR, RMAJOR, EPI_F32, net->dz, c.cout, net->col, c.kpad, net->part, c.kpad, nullptr, c.cout, c.kpad,
+         (int)P, splits);
+    hipLaunchKernelGGL(k_reduce_splits, dim3(blocks_for(c.nw())), dim3(256), 0, net->stream, net->part, c.gw, c.nw(),
+                       splits);
+    CS_HIP(hipGetLastError());
+    if (!dx32) return;
+    // data gradient: dx[p][ci] = sum_k colT(dz)[p][k] wd[ci][k]
+    Act dzact{net->dz, x.N, x.H, x.W, c.cout};
+    im2col(net, dzact, c.kpad_d, -1);
+    gemm(net, KMAJOR, KMAJOR, EPI_F32, net->col, c.kpad_d, c.wd, c.kpad_d, dx32, c.cin, nullptr, (int)P, c.cin,
+         c.kpad_d);
+}
+
+void forward(cs_cnn* net, int n)
+{
+    net->gemm_flops = 0;
+    auto resize = [&](Act& a, int H, int W) { a.N = n; a.H = H; a.W = W; };
+    const int H = net->H, W = net->W;
+    for (auto& a : net->conv_in) a.N = n;
+    for (auto& a : net->conv_out) a.N = n;
+    resize(net->x_in, H, W);
+    resize(net->p1, H / 2, W / 2);
+    resize(net->p2, H / 4, W / 4);
+    resize(net->p3, H / 8, W / 8);
+    resize(net->cat3, H / 4, W / 4);
+    resize(net->cat2, H / 2, W / 2);
+    resize(net->cat1, H, W);
+    auto& ci = net->conv_in;
+    auto& co = net->conv_out;
+    hipStream_t s = net->stream;
+    // encoder
+    conv_forward(net, 0, ci[0], co[0]);
+    conv_forward(net, 1, ci[1], co[1]);
+    hipLaunchKernelGGL(k_pool, dim3(blocks_for(net->p1.n())), dim3(256), 0, s, co[1].d, net->p1.d, n, H, W, co[1].C);
+    conv_forward(net, 2, ci[2], co[2]);
+    conv_forward(net, 3, ci[3], co[3]);
+    hipLaunchKernelGGL(k_pool, dim3(blocks_for(net->p2.n())), dim3(256), 0, s, co[3].d, net->p2.d, n, H / 2, W / 2,
+                       co[3].C);
+    conv_forward(net, 4, ci[4], co[4]);
+    conv_forward(net, 5, ci[5], co[5]);
+    hipLaunchKernelGGL(k_pool, dim3(blocks_for(net->p3.n())), dim3(256), 0, s, co[5].d, net->p3.d, n, H / 4, W / 4,
+                       co[5].C);
+    conv_forward(net, 6, ci[6], co[6]);
+    conv_forward(net, 7, ci[7], co[7]);
+    // decoder
+    hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(net->cat3.n())), dim3(256), 0, s, co[7].d, co[5].d, net->cat3.d, n,
+                       H / 4, W / 4, co[7].C, co[5].C);
+    conv_forward(net, 8, ci[8], co[8]);
+    conv_forward(net, 9, ci[9], co[9]);
+    hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(net->cat2.n())), dim3(256), 0, s, co[9].d, co[3].d, net->cat2.d, n,
+                       H / 2, W / 2, co[9].C, co[3].C);
+    conv_forward(net, 10, ci[10], co[10]);
+    conv_forward(net, 11, ci[11], co[11]);
+    hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(net->cat1.n())), dim3(256), 0, s, co[11].d, co[1].d, net->cat1.d,
+                       n, H, W, co[11].C, co[1].C);
+    conv_forward(net, 12, ci[12], co[12]);
+    conv_forward(net, 13, ci[13], co[13]);
+    conv_forward(net, 14, ci[14], co[14], net->head_out);
+    CS_HIP(hipGetLastError());
+}
+
+// Backward from the f32 gradient of the head output (net->dy32 holds it on entry).
+void backward(cs_cnn* net, int n)
+{
+    auto& ci = net->conv_in;
+    auto& co = net->conv_out;
+    hipStream_t s = net->stream;
+    const int H = net->H, W = net->W;
+    float* g = net->dy32;       // gradient of the current conv output
+    float* gin = nullptr;       // gradient of the current conv input
+    float* buf[2] = {net->dy32, net->dy32 + net->act_max};
+    int cur = 0;
+    auto step = [&](int li, bool need_dx) {
+        gin = need_dx ? buf[cur ^ 1] : nullptr;
+        conv_backward(net, li, ci[(size_t)li], co[(size_t)li], g, gin);
+        cur ^= 1;
+        g = gin;
+    };
+    // head and d1
+    step(14, true);
+    step(13, true);
+    step(12, true);  // gradient of cat1 (f32, channels co[11].C + co[1].C)
+    // split cat1: coarse part -> gradient of co[11] (f32), skip part -> e1 skip accumulator
+    CS_HIP(hipMemsetAsync(net->skip32[0], 0, sizeof(float) * (size_t)co[1].n(), s));
+    {
+        float* dcoarse = buf[cur ^ 1];
+        Act cat1 = net->cat1;
+        // k_up_concat_back reads bf16 gradients: convert the f32 gradient first
+        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(cat1.n())), dim3(256), 0, s, g, cat1.d, net->dz, cat1.n(), 0);
+        const int64_t nmax = std::max<int64_t>((int64_t)n * (H / 2) * (W / 2) * co[11].C, co[1].n());
+        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, net->dz, dcoarse, net->skip32[0],
+                           n, H, W, co[11].C, co[1].C);
+        CS_HIP(hipGetLastError());
+        cur ^= 1;
+        g = dcoarse;
+    }
+    step(11, true);
+    step(10, true);  // gradient of cat2
+    CS_HIP(hipMemsetAsync(net->skip32[1], 0, sizeof(float) * (size_t)co[3].n(), s));
+    {
+        float* dcoarse = buf[cur ^ 1];
+        Act cat2 = net->cat2;
+        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(cat2.n())), dim3(256), 0, s, g, cat2.d, net->dz, cat2.n(), 0);
+        const int64_t nmax = std::max<int64_t>((int64_t)n * (H / 4) * (W / 4) * co[9].C, co[3].n());
+        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, net->dz, dcoarse, net->skip32[1],
+                           n, H / 2, W / 2, co[9].C, co[3].C);
+        CS_HIP(hipGetLastError());
+        cur ^= 1;
+        g = dcoarse;
+    }
+    step(9, true);
+    step(8, true);  // gradient of cat3
+    CS_HIP(hipMemsetAsync(net->skip32[2], 0, sizeof(float) * (size_t)co[5].n(), s));
+    {
+        float* dcoarse = buf[cur ^ 1];
+        Act cat3 = net->cat3;
+        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(cat3.n())), dim3(256), 0, s, g, cat3.d, net->dz, cat3.n(), 0);
+        const int64_t nmax = std::max<int64_t>((int64_t)n * (H / 8) * (W / 8) * co[7].C, co[5].n());
+        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, net->dz, dcoarse, net->skip32[2],
+                           n, H / 4, W / 4, co[7].C, co[5].C);
+        CS_HIP(hipGetLastError());
+        cur ^= 1;
+        g = dcoarse;
+    }
+    step(7, true);
+    step(6, true);  // gradient of p3 (f32)
+    // pool backward into the e3 skip accumulator -> total gradient of co[5]
+    {
+        Act p3 = net->p3;
+        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(p3.n())), dim3(256), 0, s, g, p3.d, net->dz, p3.n(), 0);
+        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for(co[5].n())), dim3(256), 0, s, net->dz, net->skip32[2], n,
+                           H / 4, W / 4, co[5].C);
+        CS_HIP(hipGetLastError());
+        g = net->skip32[2];
+    }
+    step(5, true);
+    step(4, true);  // gradient of p2
+    {
+        Act p2 = net->p2;
+        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(p2.n())), dim3(256), 0, s, g, p2.d, net->dz, p2.n(), 0);
+        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for(co[3].n())), dim3(256), 0, s, net->dz, net->skip32[1], n,
+                           H / 2, W / 2, co[3].C);
+        CS_HIP(hipGetLastError());
+        g = net->skip32[1];
+    }
+    step(3, true);
+    step(2, true);  // gradient of p1
+    {
+        Act p1 = net->p1;
+        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(p1.n())), dim3(256), 0, s, g, p1.d, net->dz, p1.n(), 0);
+        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for(co[1].n())), dim3(256), 0, s, net->dz, net->skip32[0], n, H,
+                           W, co[1].C);
+        CS_HIP(hipGetLastError());
+        g = net->skip32[0];
+    }
+    step(1, true);
+    step(0, false);
+}
+
+void prepare_weights(cs_cnn* net)
+{
+    for (auto& c : net->convs) {
+        hipLaunchKernelGGL(k_prepare_weights, dim3(blocks_for(c.nw())), dim3(256), 0, net->stream, c.w, c.wb, c.wd,
+                           c.cin, c.cout, c.kpad, c.kpad_d);
+        CS_HIP(hipGetLastError());
+    }
+}
+
+void check_net(const cs_cnn* n) { CS_REQUIRE(n != nullptr, CS_ERR_ARG, "null network"); }
+
+void upload_inputs(cs_cnn* net, int n, const float* input)
+{
+    const int64_t cnt = (int64_t)n * net->H * net->W * 8;
+    float* tmp = (float*)net->dy32;  // large enough f32 staging
+    CS_HIP(hipMemcpyAsync(tmp, input, sizeof(float) * (size_t)cnt, hipMemcpyHostToDevice, net->stream));
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(blocks_for(cnt)), dim3(256), 0, net->stream, tmp, net->x_in.d, cnt);
+    CS_HIP(hipGetLastError());
+}
+
+float loss_and_grad(cs_cnn* net, int n, const float* targets_dev, const float* weights_dev)
+{
+    const int64_t P = (int64_t)n * net->H * net->W;
+    // loss = sum w (y - t)^2 / (3 P)
+    const int blocks = 256;
+    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, net->stream, net->head_out, targets_dev, weights_dev,
+                       net->dy32, net->loss_part, P, 3, 1.0f / (float)(P * 3));
+    CS_HIP(hipGetLastError());
+    std::vector<float> part(blocks);
+    CS_HIP(hipMemcpyAsync(part.data(), net->loss_part, sizeof(float) * blocks, hipMemcpyDeviceToHost, net->stream));
+    backward(net, n);
+    CS_HIP(hipStreamSynchronize(net->stream));
+    double s = 0;
+    for (float v : part) s += v;
+    return (float)(s / (double)(P * 3));
+}
+
+void adam(cs_cnn* net, float lr, float b1, float b2, float eps)
+{
+    net->adam_t += 1;
+    const float c1 = 1.f - std::pow(b1, (float)net->adam_t), c2 = 1.f - std::pow(b2, (float)net->adam_t);
+    for (auto& c : net->convs) {
+        hipLaunchKernelGGL(k_adam, dim3(blocks_for(c.nw())), dim3(256), 0, net->stream, c.w, c.gw, c.mw, c.vw, c.nw(),
+                           lr, b1, b2, eps, c1, c2);
+        hipLaunchKernelGGL(k_adam, dim3(blocks_for(c.cout)), dim3(256), 0, net->stream, c.b, c.gb, c.mb, c.vb,
+                           (int64_t)c.cout, lr, b1, b2, eps, c1, c2);
+    }
+    CS_HIP(hipGetLastError());
+    prepare_weights(net);
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+
+extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int batch, uint32_t seed, cs_cnn** out)
+{
+    return guarded([&] {
+        CS_REQUIRE(ctx && out, CS_ERR_ARG, "null argument");
+        CS_REQUIRE(height > 0 && width > 0 && height % 8 == 0 && width % 8 == 0, CS_ERR_ARG,
+                   "height and width must be positive multiples of 8");
+        CS_REQUIRE(base >= 8 && base % 8 == 0 && base <= 256, CS_ERR_ARG, "base channels must be a multiple of 8");
+        CS_REQUIRE(batch >= 1 && batch <= 256, CS_ERR_ARG, "batch must be 1..256");
+        CS_HIP(hipSetDevice(ctx->device));
+        auto* net = new cs_cnn;
+        try {
+            net->ctx = ctx;
+            net->H = height;
+            net->W = width;
+            net->base = base;
+            net->batch = batch;
+            CS_HIP(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking));
+            CS_HIP(hipEventCreate(&net->ev0));
+            CS_HIP(hipEventCreate(&net->ev1));
+            const int c = base;
+            // (cin, cout, relu) in forward order
+            const int spec[15][3] = {{8, c, 1},          {c, c, 1},          {c, 2 * c, 1},     {2 * c, 2 * c, 1},
+                                     {2 * c, 4 * c, 1},  {4 * c, 4 * c, 1},  {4 * c, 8 * c, 1}, {8 * c, 8 * c, 1},
+                                     {12 * c, 4 * c, 1}, {4 * c, 4 * c, 1},  {6 * c, 2 * c, 1}, {2 * c, 2 * c, 1},
+                                     {3 * c, c, 1},      {c, c, 1},          {c, 8, 0}};
+            std::mt19937 rng(seed);
+            for (auto& sp : spec) {
+                Conv cv;
+                cv.cin = sp[0];
+                cv.cout = sp[1];
+                cv.relu = sp[2] != 0;
+                cv.kpad = pad32(9 * cv.cin);
+                cv.kpad_d = pad32(9 * cv.cout);
+                cv.w = dalloc<float>(net, cv.nw());
+                cv.b = dalloc<float>(net, cv.cout);
+                cv.gw = dalloc<float>(net, cv.nw());
+                cv.gb = dalloc<float>(net, cv.cout);
+                cv.mw = dalloc<float>(net, cv.nw());
+                cv.vw = dalloc<float>(net, cv.nw());
+                cv.mb = dalloc<float>(net, cv.cout);
+                cv.vb = dalloc<float>(net, cv.cout);
+                cv.wb = dalloc<uint16_t>(net, cv.nw());
+                cv.wd = dalloc<uint16_t>(net, (int64_t)cv.cin * cv.kpad_d);
+                CS_HIP(hipMemset(cv.wd, 0, sizeof(uint16_t) * (size_t)cv.cin * cv.kpad_d));
+                for (float* p : {cv.mw, cv.vw}) CS_HIP(hipMemset(p, 0, sizeof(float) * (size_t)cv.nw()));
+                for (float* p : {cv.mb, cv.vb, cv.b}) CS_HIP(hipMemset(p, 0, sizeof(float) * (size_t)cv.cout));
+                // He initialisation over the 9*cin fan-in; padding columns zero.
+                std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (9.f * cv.cin)));
+                std::vector<float> hw((size_t)cv.nw(), 0.f);
+                for (int o = 0; o < cv.cout; ++o)
+                    for (int k = 0; k < 9 * cv.cin; ++k) hw[(size_t)o * cv.kpad + k] = nd(rng) * (cv.relu ? 1.f : 0.5f);
+                CS_HIP(hipMemcpy(cv.w, hw.data(), sizeof(float) * hw.size(), hipMemcpyHostToDevice));
+                net->convs.push_back(cv);
+            }
+            // activations at the maximum batch
+            const int n = batch, H = height, W = width;
+            auto act = [&](int h, int w, int ch) {
+                Act a{dalloc<uint16_t>(net, (int64_t)n * h * w * ch), n, h, w, ch};
+                return a;
+            };
+            net->x_in = act(H, W, 8);
+            const int chans[15] = {c, c, 2 * c, 2 * c, 4 * c, 4 * c, 8 * c, 8 * c, 4 * c, 4 * c, 2 * c, 2 * c, c, c, 8};
+            const int scale[15] = {1, 1, 2, 2, 4, 4, 8, 8, 4, 4, 2, 2, 1, 1, 1};
+            for (int i = 0; i < 15; ++i) net->conv_out.push_back(act(H / scale[i], W / scale[i], chans[i]));
+            net->p1 = act(H / 2, W / 2, c);
+            net->p2 = act(H / 4, W / 4, 2 * c);
+            net->p3 = act(H / 8, W / 8, 4 * c);
+            net->cat3 = act(H / 4, W / 4, 12 * c);
+            net->cat2 = act(H / 2, W / 2, 6 * c);
+            net->cat1 = act(H, W, 3 * c);
+            const Act ins[15] = {net->x_in,        net->conv_out[0], net->p1,          net->conv_out[2], net->p2,
+                                 net->conv_out[4], net->p3,          net->conv_out[6], net->cat3,        net->conv_out[8],
+                                 net->cat2,        net->conv_out[10], net->cat1,       net->conv_out[12], net->conv_out[13]};
+            for (int i = 0; i < 15; ++i) net->conv_in.push_back(ins[i]);
+            net->head_out = dalloc<float>(net, (int64_t)n * H * W * 8);
+            // scratch: im2col of the largest (pixels x kpad) over forward and data-gradient gathers
+            int64_t col = 0, part = 0;
+            for (int i = 0; i < 15; ++i) {
+                const int64_t P = net->conv_in[(size_t)i].P();
+                col = std::max(col, P * std::max(net->convs[(size_t)i].kpad, net->convs[(size_t)i].kpad_d));
+                part = std::max(part, (int64_t)64 * net->convs[(size_t)i].nw());
+                part = std::max(part, P * net->convs[(size_t)i].cin);
+            }
+            net->col = dalloc<uint16_t>(net, col);
+            net->col_elems = col;
+            net->part = dalloc<float>(net, part);
+            net->part_elems = part;
+            const int64_t amax = (int64_t)n * H * W * 96;  // >= every activation (3c <= 96 for c <= 32)
+            const int64_t amax2 = std::max<int64_t>(amax, (int64_t)n * H * W * 3 * c);
+            net->act_max = amax2;
+            net->dy32 = dalloc<float>(net, 2 * amax2);
+            net->dz = dalloc<uint16_t>(net, amax2);
+            net->skip32[0] = dalloc<float>(net, net->conv_out[1].n());
+            net->skip32[1] = dalloc<float>(net, net->conv_out[3].n());
+            net->skip32[2] = dalloc<float>(net, net->conv_out[5].n());
+            net->loss_part = dalloc<float>(net, 256);
+            net->targets = dalloc<float>(net, (int64_t)n * H * W * 3);
+            net->weights = dalloc<float>(net, (int64_t)n * H * W);
+            net->d_idx = dalloc<int>(net, n);
+            prepare_weights(net);
+            CS_HIP(hipStreamSynchronize(net->stream));
+        } catch (...) {
+            cs_cnn_destroy(net);
+            throw;
+        }
+        *out = net;
+    });
+}
+
+extern "C" int cs_cnn_destroy(cs_cnn* net)
+{
+    return guarded([&] {
+        if (!net) return;
+        if (net->stream) (void)hipStreamSynchronize(net->stream);
+        for (void* p : net->allocs) (void)hipFree(p);
+        for (void* p : {(void*)net->ds_in, (void*)net->ds_t, (void*)net->ds_w})
+            if (p) (void)hipFree(p);
+        if (net->ev0) (void)hipEventDestroy(net->ev0);
+        if (net->ev1) (void)hipEventDestroy(net->ev1);
+        if (net->stream) (void)hipStreamDestroy(net->stream);
+        delete net;
+    });
+}
+
+// Parameters in a fixed order: for each convolution, weights [cout][9*cin]
+// (tap-major, ci fastest) then bias [cout].
+static int64_t param_count(const cs_cnn* net)
+{
+    int64_t n = 0;
+    for (const auto& c : net->convs) n += (int64_t)c.cout * 9 * c.cin + c.cout;
+    return n;
+}
+
+extern "C" int cs_cnn_param_count(const cs_cnn* net, int64_t* n)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(n, CS_ERR_ARG, "null output");
+        *n = param_count(net);
+    });
+}
+
+static void copy_params(cs_cnn* net, float* host, bool to_host, bool grads)
+{
+    int64_t off = 0;
+    for (auto& c : net->convs) {
+        float* w = grads ? c.gw : c.w;
+        float* b = grads ? c.gb : c.b;
+        const int k9 = 9 * c.cin;
+        if (to_host) {
+            std::vector<float> tmp((size_t)c.nw());
+            CS_HIP(hipMemcpy(tmp.data(), w, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost));
+            for (int o = 0; o < c.cout; ++o)
+                std::memcpy(host + off + (int64_t)o * k9, tmp.data() + (size_t)o * c.kpad, sizeof(float) * k9);
+            CS_HIP(hipMemcpy(host + off + (int64_t)c.cout * k9, b, sizeof(float) * c.cout, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp((size_t)c.nw(), 0.f);
+            for (int o = 0; o < c.cout; ++o)
+                std::memcpy(tmp.data() + (size_t)o * c.kpad, host + off + (int64_t)o * k9, sizeof(float) * k9);
+            CS_HIP(hipMemcpy(w, tmp.data(), sizeof(float) * tmp.size(), hipMemcpyHostToDevice));
+            CS_HIP(hipMemcpy(b, host + off + (int64_t)c.cout * k9, sizeof(float) * c.cout, hipMemcpyHostToDevice));
+        }
+        off += (int64_t)c.cout * k9 + c.cout;
+    }
+}
+
+extern "C" int cs_cnn_get_params(cs_cnn* net, float* out)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(out, CS_ERR_ARG, "null output");
+        CS_HIP(hipStreamSynchronize(net->stream));
+        copy_params(net, out, true, false);
+    });
+}
+
+extern "C" int cs_cnn_set_params(cs_cnn* net, const float* in)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(in, CS_ERR_ARG, "null input");
+        CS_HIP(hipStreamSynchronize(net->stream));
+        copy_params(net, const_cast<float*>(in), false, false);
+        prepare_weights(net);
+        CS_HIP(hipStreamSynchronize(net->stream));
+    });
+}
+
+extern "C" int cs_cnn_get_grads(cs_cnn* net, float* out)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(out, CS_ERR_ARG, "null output");
+        CS_HIP(hipStreamSynchronize(net->stream));
+        copy_params(net, out, true, true);
+    });
+}
+
+// input: [n][H][W][8] f32; output: [n][H][W][3] f32.
+extern "C" int cs_cnn_forward(cs_cnn* net, int n, const float* input, float* output)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(n >= 1 && n <= net->batch && input && output, CS_ERR_ARG, "bad argument");
+        upload_inputs(net, n, input);
+        forward(net, n);
+        const int64_t P = (int64_t)n * net->H * net->W;
+        std::vector<float> tmp((size_t)(P * 8));
+        CS_HIP(hipMemcpyAsync(tmp.data(), net->head_out, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost,
+                              net->stream));
+        CS_HIP(hipStreamSynchronize(net->stream));
+        for (int64_t p = 0; p < P; ++p)
+            for (int c = 0; c < 3; ++c) output[p * 3 + c] = tmp[(size_t)(p * 8 + c)];
+    });
+}
+
+// Forward + backward on host data: loss = sum w (y - t)^2 / (3 n H W);
+// gradients are left in the network (cs_cnn_get_grads, cs_cnn_adam_step).
+extern "C" int cs_cnn_loss_grad(cs_cnn* net, int n, const float* input, const float* target, const float* weight,
+                                float* loss)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(n >= 1 && n <= net->batch && input && target && weight && loss, CS_ERR_ARG, "bad argument");
+        upload_inputs(net, n, input);
+        const int64_t P = (int64_t)n * net->H * net->W;
+        CS_HIP(hipMemcpyAsync(net->targets, target, sizeof(float) * (size_t)(P * 3), hipMemcpyHostToDevice,
+                              net->stream));
+        CS_HIP(hipMemcpyAsync(net->weights, weight, sizeof(float) * (size_t)P, hipMemcpyHostToDevice, net->stream));
+        forward(net, n);
+        *loss = loss_and_grad(net, n, net->targets, net->weights);
+    });
+}
+
+extern "C" int cs_cnn_adam_step(cs_cnn* net, float lr, float beta1, float beta2, float eps)
+{
+    return guarded([&] {
+        check_net(net);
+        adam(net, lr, beta1, beta2, eps);
+        CS_HIP(hipStreamSynchronize(net->stream));
+    });
+}
+
+extern "C" int cs_cnn_set_dataset(cs_cnn* net, int count, const float* inputs, const float* targets,
+                                  const float* weights)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(count >= 1 && inputs && targets && weights, CS_ERR_ARG, "bad argument");
+        const int64_t hw = (int64_t)net->H * net->W;
+        for (void* p : {(void*)net->ds_in, (void*)net->ds_t, (void*)net->ds_w})
+            if (p) (void)hipFree(p);
+        net->ds_in = nullptr;
+        net->ds_t = nullptr;
+        net->ds_w = nullptr;
+        CS_HIP(hipMalloc(&net->ds_in, sizeof(uint16_t) * (size_t)(count * hw * 8)));
+        CS_HIP(hipMalloc(&net->ds_t, sizeof(float) * (size_t)(count * hw * 3)));
+        CS_HIP(hipMalloc(&net->ds_w, sizeof(float) * (size_t)(count * hw)));
+        // inputs converted on the device, one sample at a time through dy32
+        for (int s = 0; s < count; ++s) {
+            CS_HIP(hipMemcpyAsync(net->dy32, inputs + (int64_t)s * hw * 8, sizeof(float) * (size_t)(hw * 8),
+                                  hipMemcpyHostToDevice, net->stream));
+            hipLaunchKernelGGL(k_f32_to_bf16, dim3(blocks_for(hw * 8)), dim3(256), 0, net->stream, net->dy32,
+                               net->ds_in + (int64_t)s * hw * 8, hw * 8);
+            CS_HIP(hipGetLastError());
+            CS_HIP(hipStreamSynchronize(net->stream));
+        }
+        CS_HIP(hipMemcpy(net->ds_t, targets, sizeof(float) * (size_t)(count * hw * 3), hipMemcpyHostToDevice));
+        CS_HIP(hipMemcpy(net->ds_w, weights, sizeof(float) * (size_t)(count * hw), hipMemcpyHostToDevice));
+        net->ds_count = count;
+    });
+}
+
+// One pass over `count` dataset samples in the given order, minibatches of
+// the network's batch size, Adam after each; returns the mean loss.
+extern "C" int cs_cnn_train_epoch(cs_cnn* net, const int32_t* order, int count, float lr, float* mean_loss)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(order && mean_loss && count >= 1 && net->ds_count > 0, CS_ERR_ARG, "bad argument");
+        for (int i = 0; i < count; ++i)
+            CS_REQUIRE(order[i] >= 0 && order[i] < net->ds_count, CS_ERR_ARG, "sample index out of range");
+        const int64_t hw = (int64_t)net->H * net->W;
+        double total = 0;
+        int batches = 0;
+        for (int s = 0; s + net->batch <= count; s += net->batch) {
+            const int n = net->batch;
+            CS_HIP(hipMemcpyAsync(net->d_idx, order + s, sizeof(int) * n, hipMemcpyHostToDevice, net->stream));
+            hipLaunchKernelGGL(k_gather_batch, dim3(blocks_for(n * hw)), dim3(256), 0, net->stream, net->ds_in,
+                               net->ds_t, net->ds_w, net->d_idx, n, hw, net->x_in.d, net->targets, net->weights);
+            CS_HIP(hipGetLastError());
+            forward(net, n);
+            total += loss_and_grad(net, n, net->targets, net->weights);
+            adam(net, lr, 0.9f, 0.999f, 1e-8f);
+            batches += 1;
+        }
+        CS_HIP(hipStreamSynchronize(net->stream));
+        *mean_loss = batches ? (float)(total / batches) : 0.f;
+    });
+}
+
+// Times `iters` forward+backward passes at the full batch on the current
+// input buffers; returns ms per pass and the GEMM FLOPs of one pass.
+extern "C" int cs_cnn_time_step(cs_cnn* net, int iters, float* ms, double* flops)
+{
+    return guarded([&] {
+        check_net(net);
+        CS_REQUIRE(iters >= 1 && ms && flops, CS_ERR_ARG, "bad argument");
+        const int n = net->batch;
+        CS_HIP(hipMemsetAsync(net->weights, 0, sizeof(float) * (size_t)n * net->H * net->W, net->stream));
+        CS_HIP(hipMemsetAsync(net->targets, 0, sizeof(float) * (size_t)n * net->H * net->W * 3, net->stream));
+        CS_HIP(hipEventRecord(net->ev0, net->stream));
+        double f = 0;
+        for (int i = 0; i < iters; ++i) {
+            forward(net, n);
+            f = net->gemm_flops;
+            hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, net->stream, net->head_out, net->targets, net->weights,
+                               net->dy32, net->loss_part, (int64_t)n * net->H * net->W, 3, 1.0f);
+            net->gemm_flops = 0;
+            backward(net, n);
+            f += net->gemm_flops;
+        }
+        CS_HIP(hipEventRecord(net->ev1, net->stream));
+        CS_HIP(hipEventSynchronize(net->ev1));
+        float t = 0;
+        CS_HIP(hipEventElapsedTime(&t, net->ev0, net->ev1));
+        *ms = t / iters;
+        *flops = f;
+    });
+}
+
+// Plain BF16 GEMM C[M][N] (f32) = A[M][K] B[N][K]^T, both K-contiguous, K a
+// multiple of 32; exposes the MFMA tile kernel for tests and timing.
+extern "C" int cs_gemm_bf16(cs_ctx* ctx, int M, int N, int K, const uint16_t* A, const uint16_t* B, float* C,
+                            float* ms)
+{
+    return guarded([&] {
+        CS_REQUIRE(ctx && A && B && C && M > 0 && N > 0 && K > 0 && K % 32 == 0, CS_ERR_ARG, "bad argument");
+        CS_HIP(hipSetDevice(ctx->device));
+        void *da = nullptr, *db = nullptr, *dc = nullptr;
+        hipStream_t s;
+        CS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        auto cleanup = [&] {
+            for (void* p : {da, db, dc})
+                if (p) (void)hipFree(p);
+            (void)hipStreamDestroy(s);
+        };
+        try {
+            CS_HIP(hipMalloc(&da, sizeof(uint16_t) * (size_t)M * K));
+            CS_HIP(hipMalloc(&db, sizeof(uint16_t) * (size_t)N * K));
+            CS_HIP(hipMalloc(&dc, sizeof(float) * (size_t)M * N));
+            CS_HIP(hipMemcpy(da, A, sizeof(uint16_t) * (size_t)M * K, hipMemcpyHostToDevice));
+            CS_HIP(hipMemcpy(db, B, sizeof(uint16_t) * (size_t)N * K, hipMemcpyHostToDevice));
+            GemmArgs g;
+            g.A = (const uint16_t*)da;
+            g.B = (const uint16_t*)db;
+            g.C = dc;
+            g.bias = nullptr;
+            g.M = M;
+            g.N = N;
+            g.K = K;
+            g.lda = K;
+            g.ldb = K;
+            g.ldc = N;
+            g.k_per_split = K;
+            g.split_stride = 0;
+            dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN), 1);
+            hipEvent_t e0, e1;
+            CS_HIP(hipEventCreate(&e0));
+            CS_HIP(hipEventCreate(&e1));
+            hipLaunchKernelGGL((k_gemm<KMAJOR, KMAJOR, EPI_F32>), grid, dim3(256), 0, s, g);
+            CS_HIP(hipEventRecord(e0, s));
+            const int iters = 10;
+            for (int i = 0; i < iters; ++i) hipLaunchKernelGGL((k_gemm<KMAJOR, KMAJOR, EPI_F32>), grid, dim3(256), 0, s, g);
+            CS_HIP(hipEventRecord(e1, s));
+            CS_HIP(hipEventSynchronize(e1));
+            float t = 0;
+            CS_HIP(hipEventElapsedTime(&t, e0, e1));
+            if (ms) *ms = t / iters;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            CS_HIP(hipMemcpy(C, dc, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}

@@ -40,6 +40,7 @@ extern "C" {
 typedef struct cs_ctx cs_ctx;         /* one GPU (af::init + set_backend, main.rs:331-332) */
 typedef struct cs_lattice cs_lattice; /* State<D2Q9> with BGK (lbm.rs:670-676) */
 typedef struct cs_batch cs_batch;     /* many independent lattices, one launch (extension) */
+typedef struct cs_cnn cs_cnn;         /* warm-start network (extension) */
 
 enum cs_status {
     CS_OK = 0,
@@ -221,6 +222,45 @@ int cs_batch_step_per_lattice(cs_batch* b, int steps, float* ms);
 /* Times every tuning candidate for trial_steps steps (state restored after),
  * keeps the fastest and returns it. */
 int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* best, float* best_ms);
+/* Inactive lattices are skipped by the steps (their state is kept). */
+int cs_batch_set_active(cs_batch* b, int index, int active);
+/* Velocity snapshot of every lattice; cs_batch_residuals then returns, per
+ * lattice, ||u - u_snapshot||_2 / ||u||_2 over non-solid cells in f64. */
+int cs_batch_snapshot_velocity(cs_batch* b);
+int cs_batch_residuals(cs_batch* b, double* residuals);
+/* Moments of one lattice (h*w planes), working precision rounded to f32. */
+int cs_batch_moments(cs_batch* b, int index, float* rho, float* ux, float* uy);
+
+/* ---- learned warm start (extension; no reference counterpart) ----------
+ * A 4-level convolutional encoder-decoder (3x3 convolutions, ReLU, 2x2
+ * average pooling, nearest up-sampling, skip concatenations; base channel
+ * count c) predicting the steady (rho, u) of a channel flow from 8 input
+ * channels per cell.  BF16 MFMA implicit-GEMM kernels with FP32 accumulation,
+ * FP32 master weights, Adam.  Tensors are [n][height][width][channels] f32 on
+ * the host; height and width are multiples of 8.  Parameters are ordered per
+ * convolution: weights [cout][3][3][cin] then bias [cout]. */
+int cs_cnn_create(cs_ctx* ctx, int height, int width, int base_channels, int batch, uint32_t seed, cs_cnn** out);
+int cs_cnn_destroy(cs_cnn* net);
+int cs_cnn_param_count(const cs_cnn* net, int64_t* count);
+int cs_cnn_get_params(cs_cnn* net, float* params);
+int cs_cnn_set_params(cs_cnn* net, const float* params);
+/* Gradients of the last cs_cnn_loss_grad / training minibatch. */
+int cs_cnn_get_grads(cs_cnn* net, float* grads);
+/* input [n][h][w][8] -> output [n][h][w][3]; n <= batch. */
+int cs_cnn_forward(cs_cnn* net, int n, const float* input, float* output);
+/* loss = sum weight * (output - target)^2 / (3 n h w); forward + backward. */
+int cs_cnn_loss_grad(cs_cnn* net, int n, const float* input, const float* target, const float* weight, float* loss);
+int cs_cnn_adam_step(cs_cnn* net, float lr, float beta1, float beta2, float eps);
+/* Device-resident training set: inputs [count][h][w][8], targets
+ * [count][h][w][3], weights [count][h][w]. */
+int cs_cnn_set_dataset(cs_cnn* net, int count, const float* inputs, const float* targets, const float* weights);
+/* One pass in the given sample order, minibatches of `batch`, Adam per batch. */
+int cs_cnn_train_epoch(cs_cnn* net, const int32_t* order, int count, float lr, float* mean_loss);
+/* Average ms of `iters` forward+backward passes and the GEMM FLOPs of one. */
+int cs_cnn_time_step(cs_cnn* net, int iters, float* ms, double* flops);
+/* C[M][N] (f32) = A[M][K] . B[N][K]^T in BF16 on the MFMA tile kernel
+ * (K a multiple of 32); returns the average kernel ms. */
+int cs_gemm_bf16(cs_ctx* ctx, int M, int N, int K, const uint16_t* A, const uint16_t* B, float* C, float* ms);
 
 /* ---- measurement helpers ---------------------------------------------- */
 /* Plain device-to-device copy kernel of `bytes` bytes, `iters` times; returns
