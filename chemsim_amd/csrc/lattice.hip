@@ -802,6 +802,34 @@ static void init_eq_impl(cs_lattice* l, const T* rho, const T* ux, const T* uy)
     });
 }
 
+extern "C" int cs_lattice_init_nonequilibrium(cs_lattice* l, const float* rho, const float* ux, const float* uy)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(rho && ux && uy, CS_ERR_ARG, "null input");
+        const int64_t n = (int64_t)l->w * l->h;
+        dispatch_storage(l->storage, [&]<class St>() {
+            using TT = typename St::T;
+            std::vector<TT> host((size_t)(3 * n));
+            for (int64_t d = 0; d < n; ++d) {
+                host[(size_t)d] = (TT)rho[d];
+                host[(size_t)(n + d)] = (TT)ux[d];
+                host[(size_t)(2 * n + d)] = (TT)uy[d];
+            }
+            DevBuf dev(sizeof(TT) * 3 * (size_t)n);
+            CS_HIP(hipMemcpyAsync(dev.p, host.data(), sizeof(TT) * 3 * (size_t)n, hipMemcpyHostToDevice, l->stream));
+            const TT* p = (const TT*)dev.p;
+            hipLaunchKernelGGL((k_init_neq<St>), cell_grid(l), dim3(256), 0, l->stream, p, p + n, p + 2 * n,
+                               l->has_flags ? (const uint8_t*)l->flags : nullptr, (typename St::S*)l->buf[l->cur],
+                               l->w, l->h, l->pitch, l->plane, consts_of<St>(l),
+                               NeqArgs<TT>{(TT)l->params.tau, (TT)l->params.lid_ux, (TT)l->params.lid_uy,
+                                           l->params.edge == CS_EDGE_PERIODIC ? 1 : 0});
+            CS_HIP(hipGetLastError());
+            CS_HIP(hipStreamSynchronize(l->stream));
+        });
+    });
+}
+
 extern "C" int cs_lattice_init_equilibrium(cs_lattice* l, const float* rho, const float* ux, const float* uy)
 {
     return guarded([&] {

@@ -138,6 +138,98 @@ __global__ void k_init_eq(const typename St::T* __restrict__ rho, const typename
     for (int i = 0; i < 9; ++i) St::st(f, i * P + own, F[i]);
 }
 
+// Derivative at a node from its two neighbours along one axis.  Each
+// neighbour is absent (NB_NONE: outside a non-periodic edge), a fluid node at
+// distance 1 (NB_FLUID, value v) or a wall (NB_WALL, wall velocity v) half-way
+// to the solid node -- where both bounce-back forms put it.  The wall cases
+// are the derivative of the quadratic through the three points.
+enum : int { NB_NONE = 0, NB_FLUID = 1, NB_WALL = 2 };
+template <class T>
+__host__ __device__ inline T neq_deriv(T u0, int kp, T vp, int km, T vm)
+{
+    const T third = T(1) / T(3), four_thirds = T(4) / T(3);
+    if (kp == NB_FLUID && km == NB_FLUID) return (vp - vm) * T(0.5);
+    if (kp == NB_FLUID && km == NB_WALL) return (vp * third + u0) - vm * four_thirds;
+    if (kp == NB_WALL && km == NB_FLUID) return (vp * four_thirds - u0) - vm * third;
+    if (kp == NB_WALL && km == NB_WALL) return vp - vm;
+    if (kp == NB_FLUID) return vp - u0;
+    if (km == NB_FLUID) return u0 - vm;
+    if (kp == NB_WALL) return (vp - u0) * T(2);
+    if (km == NB_WALL) return (u0 - vm) * T(2);
+    return T(0);
+}
+
+// f = f_eq(rho, u) + f_neq, with the first-order Chapman-Enskog
+// non-equilibrium part of the post-collision populations a lattice holds
+// between steps: f_neq_i = -3 (tau - 1) w_i rho (Q_i : S) (the pre-collision
+// part -3 tau w_i rho Q_i : S times 1 - 1/tau), Q_i = c_i c_i - I/3, S the
+// symmetric velocity gradient from neq_deriv (lattice units, dt = 1),
+// f_neq = 0 at solid cells.  Lattice-velocity
+// coordinates: X = -row, Y = column.  Walls move with the lid velocity at
+// CS_LID cells and rest at CS_WALL cells.
+template <class T>
+struct NeqArgs {
+    T tau, lid_ux, lid_uy;
+    int periodic;
+};
+
+template <class St>
+__global__ void k_init_neq(const typename St::T* __restrict__ rho, const typename St::T* __restrict__ ux,
+                           const typename St::T* __restrict__ uy, const uint8_t* __restrict__ flags,
+                           typename St::S* __restrict__ f, int w, int h, int pitch, int64_t P,
+                           Consts<typename St::T> k, NeqArgs<typename St::T> a)
+{
+    using T = typename St::T;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (c >= w) return;
+    const int64_t d = (int64_t)r * w + c, own = (int64_t)r * pitch + c;
+    // kind of node (rr, cc); on NB_FLUID/NB_WALL sets the dense index or the wall flag
+    auto kind = [&](int rr, int cc, int64_t& idx, int& t) {
+        if (a.periodic) {
+            rr = rr < 0 ? rr + h : (rr >= h ? rr - h : rr);
+            cc = cc < 0 ? cc + w : (cc >= w ? cc - w : cc);
+        } else if (rr < 0 || rr >= h || cc < 0 || cc >= w) {
+            return (int)NB_NONE;
+        }
+        idx = (int64_t)rr * w + cc;
+        t = flags ? (int)flags[(int64_t)rr * pitch + cc] : (int)CS_FLUID;
+        return (t == CS_WALL || t == CS_LID) ? (int)NB_WALL : (int)NB_FLUID;
+    };
+    T F[9];
+    if constexpr (St::DEV) equilibrium_dev<T>(k, rho[d], ux[d], uy[d], F);
+    else equilibrium<T>(k, rho[d], ux[d], uy[d], F);
+    int64_t i0;
+    int t0;
+    if (kind(r, c, i0, t0) == NB_FLUID) {
+        // +X is row r - 1, +Y is column c + 1
+        int64_t ip = 0, im = 0;
+        int tp = 0, tm = 0;
+        const int kpx = kind(r - 1, c, ip, tp), kmx = kind(r + 1, c, im, tm);
+        auto val = [&](int kk, int64_t idx, int t, const T* u, T lid) -> T {
+            if (kk == NB_FLUID) return u[idx];
+            if (kk == NB_WALL) return t == CS_LID ? lid : T(0);
+            return T(0);
+        };
+        const T dxux = neq_deriv<T>(ux[d], kpx, val(kpx, ip, tp, ux, a.lid_ux), kmx, val(kmx, im, tm, ux, a.lid_ux));
+        const T dxuy = neq_deriv<T>(uy[d], kpx, val(kpx, ip, tp, uy, a.lid_uy), kmx, val(kmx, im, tm, uy, a.lid_uy));
+        const int kpy = kind(r, c + 1, ip, tp), kmy = kind(r, c - 1, im, tm);
+        const T dyux = neq_deriv<T>(ux[d], kpy, val(kpy, ip, tp, ux, a.lid_ux), kmy, val(kmy, im, tm, ux, a.lid_ux));
+        const T dyuy = neq_deriv<T>(uy[d], kpy, val(kpy, ip, tp, uy, a.lid_uy), kmy, val(kmy, im, tm, uy, a.lid_uy));
+        const T sxy = T(0.5) * (dyux + dxuy);
+        const T third = T(1) / T(3);
+        const T pre = (T(-3) * (a.tau - T(1))) * rho[d];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const T cx = (T)CX[i], cy = (T)CY[i];
+            const T qs = ((cx * cx - third) * dxux + (T(2) * cx * cy) * sxy) + (cy * cy - third) * dyuy;
+            F[i] = F[i] + (pre * k.w(i)) * qs;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) St::st(f, i * P + own, F[i]);
+}
+
 enum : unsigned {
     M_RHO = 1u, M_MX = 2u, M_MY = 4u, M_UX = 8u, M_UY = 16u, M_SPEED = 32u, M_P = 64u
 };

@@ -144,6 +144,12 @@ class Lattice:
             arrs = [np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float32) for a in (rho, ux, uy)]
             call("cs_lattice_init_equilibrium", self.handle, *[_ptr(a) for a in arrs])
 
+    def init_nonequilibrium(self, rho, ux, uy) -> None:
+        """f = f_eq(rho, u) + first-order Chapman-Enskog f_neq from grad u."""
+        shape = (self.h, self.w)
+        arrs = [np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float32) for a in (rho, ux, uy)]
+        call("cs_lattice_init_nonequilibrium", self.handle, *[_ptr(a) for a in arrs])
+
     # -- stepping ------------------------------------------------------------
     def step(self, n: int = 1) -> None:
         call("cs_lattice_step", self.handle, int(n))

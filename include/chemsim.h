@@ -140,6 +140,14 @@ int cs_lattice_download_raw(cs_lattice* lat, void* data);
 int cs_lattice_init_equilibrium(cs_lattice* lat, const float* rho, const float* ux, const float* uy);
 int cs_lattice_init_equilibrium_f64(cs_lattice* lat, const double* rho, const double* ux,
                                     const double* uy);
+/* f = f_eq(rho, u) + f_neq, f_neq the first-order Chapman-Enskog
+ * non-equilibrium part of the post-collision populations a lattice holds
+ * between steps, -3 (tau - 1) w_i rho (c_i c_i - I/3) : S, S the symmetric
+ * gradient of u from neighbour differences (walls half-way to solid cells,
+ * moving with the lid velocity at CS_LID cells; f_neq = 0 at solid cells).
+ * A consistent start from a predicted field (extension; set the flags and
+ * params first). */
+int cs_lattice_init_nonequilibrium(cs_lattice* lat, const float* rho, const float* ux, const float* uy);
 
 /* ---- the step ------------------------------------------------------------ */
 /* n x State::step (lbm.rs:694-714): stream, bounce-back and BGK collide fused

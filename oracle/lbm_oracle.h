@@ -66,6 +66,16 @@ void orc_consts_f64_make(const orc_params* p, orc_consts_f64* out);
  * from the f64 evaluation rounded once (extension). */
 void orc_consts_devf32_make(const orc_params* p, orc_consts_f32* out);
 
+/* Consistent start (extension): F (an equilibrium, plain or deviation form)
+ * += the first-order Chapman-Enskog non-equilibrium part of post-collision
+ * populations at fluid cells, -3 (tau - 1) rho w_i (c_i c_i - I/3) : S, S from
+ * neighbour differences with walls half-way to solid nodes (flags dense h*w
+ * or NULL). */
+void orc_add_neq_f32(const orc_params* p, const uint8_t* flags, const float* rho, const float* ux,
+                     const float* uy, float* F);
+void orc_add_neq_f64(const orc_params* p, const uint8_t* flags, const double* rho, const double* ux,
+                     const double* uy, double* F);
+
 /* ---- Reference operations, one pass each, as lbm.rs runs them ---------- */
 /* State::stream (lbm.rs:716-729): Q_i[r][c] = P_i[r+cx_i][c-cy_i], zero outside
  * (edge = ZERO) or wrapped (edge = PERIODIC, extension). */

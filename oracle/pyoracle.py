@@ -75,6 +75,8 @@ for name, args, res in [
     ("orc_step_devf32", [_PP, _P, _P, _P], None),
     ("orc_run_devf32", [_PP, _P, _P, C.c_int], C.c_int),
     ("orc_equilibrium_devf32", [_PP, C.c_int, _P, _P, _P, _P], None),
+    ("orc_add_neq_f32", [_PP, _P, _P, _P, _P, _P], None),
+    ("orc_add_neq_f64", [_PP, _P, _P, _P, _P, _P], None),
     ("orc_f32_to_bf16_n", [_P, _P, C.c_long], None),
     ("orc_f32_to_f16_n", [_P, _P, C.c_long], None),
     ("orc_step_dev16", [_PP, C.c_int, _P, _P, _P], None),
@@ -274,6 +276,16 @@ def equilibrium_dev(p: Params, rho, ux, uy):
     out = np.empty((9,) + rho.shape, np.float32)
     lib.orc_equilibrium_devf32(C.byref(p), int(rho.size), _p(rho), _p(ux), _p(uy), _p(out))
     return out
+
+
+def nonequilibrium_init(p: Params, rho, ux, uy, flags=None, dev=False):
+    """f_eq(rho, u) (deviation form with dev=True) plus the first-order
+    Chapman-Enskog non-equilibrium part at fluid cells."""
+    dt = np.float32 if dev else _real(np.asarray(rho))
+    rho, ux, uy = (np.ascontiguousarray(a, dtype=dt) for a in (rho, ux, uy))
+    F = equilibrium_dev(p, rho, ux, uy) if dev else equilibrium(p, rho, ux, uy)
+    getattr(lib, "orc_add_neq_" + _sfx(dt))(C.byref(p), _p(_flags(flags, p)), _p(rho), _p(ux), _p(uy), _p(F))
+    return F
 
 
 def to_dev(p: Params, f):

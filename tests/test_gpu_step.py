@@ -371,3 +371,29 @@ def test_taylor_green_4096_fp32_vs_fp64(ctx):
     assert err["f32"] < 5e-4 and res["f32"][1] < 5e-5
     assert err["f32dev"] < 3e-5 and res["f32dev"][1] < 1e-6
     assert err["f32dev"] < err["f32"]
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64", "f32dev", "f16", "bf16"])
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+def test_init_nonequilibrium_matches_oracle(ctx, storage, edge):
+    """Consistent start f_eq + f_neq on random walls and lids: value for value
+    (16-bit: the f32 deviation rounded to nearest even)."""
+    rng = np.random.default_rng(23)
+    h, w = 45, 70
+    rho = (1 + 0.01 * rng.standard_normal((h, w))).astype(np.float32)
+    ux = (0.03 * rng.standard_normal((h, w))).astype(np.float32)
+    uy = (0.03 * rng.standard_normal((h, w))).astype(np.float32)
+    flags = random_flags(rng, h, w)
+    op = O.params(w, h, tau=0.7, edge=edge, boundary=O.BB_HALFWAY, lid=(0.01, -0.02))
+    with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+        lat.init_nonequilibrium(rho, ux, uy)
+        if storage in ("f32", "f64"):
+            dt = STORE[storage]
+            ref = O.nonequilibrium_init(op, *(a.astype(dt) for a in (rho, ux, uy)), flags)
+            np.testing.assert_array_equal(lat.download(dt), ref)
+        else:
+            g = O.nonequilibrium_init(op, rho, ux, uy, flags, dev=True)
+            if storage == "f32dev":
+                np.testing.assert_array_equal(lat.download_raw(), g)
+            else:
+                np.testing.assert_array_equal(lat.download_raw(), narrow16(O.F16 if storage == "f16" else O.BF16, g))
