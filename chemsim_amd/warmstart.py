@@ -2,12 +2,14 @@
 
 A network (chemsim_amd/csrc/cnn.hip) predicts the steady (rho, u) of a
 channel flow past circular obstacles from the obstacle mask, the Reynolds
-number and position channels.  The lattice is initialised at the predicted
+number, position channels, a lubrication-theory estimate of the flow and the
+distance to the nearest solid.  The lattice is initialised at the predicted
 equilibrium (compute_equilibrium, src/lbm.rs:43-71, as src/main.rs:257-267
-initialises a lattice) and the plain BGK step runs to the residual
-tolerance, so the LBM stays the judge of the prediction.  Training data are
-steady states computed by the GPU LBM itself (a batch of lattices in
-deviation arithmetic).
+initialises a lattice), optionally plus the Chapman-Enskog non-equilibrium
+part of the predicted velocity gradient, and the plain BGK step runs to the
+residual tolerance, so the LBM stays the judge of the prediction.  Training
+data are steady states computed by the GPU LBM itself (a batch of lattices
+in deviation arithmetic).
 """
 from __future__ import annotations
 
@@ -23,33 +25,83 @@ from .batch import Batch
 from .lattice import Context, Lattice, Params
 from . import workloads as W
 
-RHO_SCALE = 0.01          # target channel 0 = (rho - 1) / RHO_SCALE
 RE_MID, RE_HALF = 60.0, 40.0
+DIST_SCALE = 16.0         # signed distance feature = distance / DIST_SCALE, clipped to [-1, 1]
+
+
+def lubrication(ch: W.Channel):
+    """Lubrication-theory estimate of the steady flow, column by column: the
+    inlet flux Q = u_in (h - 2) splits over the open gaps of a column in
+    proportion to gap^3 (parallel Poiseuille resistances), each gap carries
+    a Poiseuille profile, and the pressure falls by 12 nu Q / sum(gap^3) per
+    column towards the outlet (p = 0 there; rho - 1 = 3 p).  Returns
+    (rho - 1, uy) as h x w arrays; it ignores inertia and the entrance length,
+    the network corrects it."""
+    h, w = ch.h, ch.w
+    solid = ch.flags() == W.WALL
+    fluid = ~solid
+    q = ch.u_in * (h - 2)
+    rows = np.arange(h)[:, None] * np.ones((1, w), np.int64)
+    # gaps: runs of fluid rows in a column, labelled uniquely over the lattice
+    starts = fluid & np.vstack([np.ones((1, w), bool), ~fluid[:-1]])
+    label = (np.cumsum(starts, axis=0) + (h + 1) * np.arange(w)[None, :]) * fluid
+    n_lab = (h + 1) * w + h + 1
+    g = np.bincount(label[fluid], minlength=n_lab).astype(np.float64)
+    first = np.full(n_lab, h, np.int64)
+    np.minimum.at(first, label[fluid], rows[fluid])
+    g3 = np.bincount(label[fluid] // (h + 1), weights=g[label[fluid]] ** 2, minlength=w)  # sum of gap^3
+    uy = np.zeros((h, w), np.float64)
+    gl = g[label[fluid]]
+    qk = q * gl ** 3 / g3[label[fluid] // (h + 1)]
+    x = (rows[fluid] - first[label[fluid]] + 0.5) / gl
+    uy[fluid] = 6.0 * (qk / gl) * x * (1 - x)
+    dpdy = np.where(g3 > 0, 12.0 * ch.nu * q / np.maximum(g3, 1e-30), 0.0)
+    p = np.concatenate([np.cumsum(dpdy[::-1])[::-1][1:], [0.0]])
+    return 3.0 * np.repeat(p[None, :], h, 0) * fluid, uy
+
+
+def rho_scale(ch: W.Channel) -> float:
+    """rho - 1 of the inlet of the empty channel by the lubrication estimate
+    (3 * 12 nu u_in L / (h - 2)^2): the density target is (rho - 1) / this."""
+    return 36.0 * ch.nu * ch.u_in * (ch.w - 1) / (ch.h - 2) ** 2
+
+
+def signed_distance(ch: W.Channel) -> np.ndarray:
+    """Distance to the nearest wall or obstacle surface (walls half-way
+    between the first fluid row and the wall row), negative inside solids."""
+    h, w = ch.h, ch.w
+    rr = np.arange(h, dtype=np.float64)[:, None] * np.ones((1, w))
+    cc = np.ones((h, 1)) * np.arange(w, dtype=np.float64)[None, :]
+    d = np.minimum(rr - 0.5, (h - 1.5) - rr)
+    for (r0, c0, rad) in ch.circles:
+        d = np.minimum(d, np.sqrt((rr - r0) ** 2 + (cc - c0) ** 2) - rad)
+    return d
 
 
 def features(ch: W.Channel) -> np.ndarray:
     """[h][w][8] network input: solid mask, normalised Re, row and column
-    coordinates in [-1, 1], the Poiseuille shape across the channel, inlet
-    and outlet columns, and zero."""
+    coordinates in [-1, 1], the lubrication estimate of uy / u_in and of
+    (rho - 1) / rho_scale, the signed distance to the nearest solid, and the
+    inlet (+1) / outlet (-1) columns."""
     h, w = ch.h, ch.w
     f = np.zeros((h, w, 8), np.float32)
     flags = ch.flags()
-    solid = (flags == W.WALL).astype(np.float32)
-    f[..., 0] = solid
+    f[..., 0] = flags == W.WALL
     f[..., 1] = (ch.re - RE_MID) / RE_HALF
-    r = (np.arange(h, dtype=np.float32) + 0.5) / h * 2 - 1
-    c = (np.arange(w, dtype=np.float32) + 0.5) / w * 2 - 1
-    f[..., 2] = r[:, None]
-    f[..., 3] = c[None, :]
-    s = np.clip((np.arange(h, dtype=np.float32) - 0.5) / (h - 2), 0, 1)
-    f[..., 4] = (4 * s * (1 - s))[:, None]
-    f[..., 5] = (flags == W.INLET)
-    f[..., 6] = (flags == W.OUTLET)
+    f[..., 2] = ((np.arange(h, dtype=np.float32) + 0.5) / h * 2 - 1)[:, None]
+    f[..., 3] = ((np.arange(w, dtype=np.float32) + 0.5) / w * 2 - 1)[None, :]
+    drho, uy = lubrication(ch)
+    f[..., 4] = uy / ch.u_in
+    f[..., 5] = drho / rho_scale(ch)
+    f[..., 6] = np.clip(signed_distance(ch) / DIST_SCALE, -1, 1)
+    f[..., 7] = (flags == W.INLET).astype(np.float32) - (flags == W.OUTLET)
     return f
 
 
 def encode_targets(ch: W.Channel, rho, ux, uy) -> tuple[np.ndarray, np.ndarray]:
-    t = np.stack([(rho - 1.0) / RHO_SCALE, ux / ch.u_in, uy / ch.u_in], axis=-1).astype(np.float32)
+    """Targets ((rho - 1) / rho_scale, ux / u_in, uy / u_in) and the loss
+    weight (1 at fluid cells, 0 at solid ones)."""
+    t = np.stack([(rho - 1.0) / rho_scale(ch), ux / ch.u_in, uy / ch.u_in], axis=-1).astype(np.float32)
     wgt = (ch.flags() != W.WALL).astype(np.float32)
     t[wgt == 0] = 0.0
     return t, wgt
@@ -57,13 +109,19 @@ def encode_targets(ch: W.Channel, rho, ux, uy) -> tuple[np.ndarray, np.ndarray]:
 
 def decode_prediction(ch: W.Channel, y: np.ndarray):
     solid = ch.flags() == W.WALL
-    rho = (1.0 + RHO_SCALE * y[..., 0]).astype(np.float32)
+    rho = (1.0 + rho_scale(ch) * y[..., 0]).astype(np.float32)
     ux = (ch.u_in * y[..., 1]).astype(np.float32)
     uy = (ch.u_in * y[..., 2]).astype(np.float32)
     rho[solid] = 1.0
     ux[solid] = 0.0
     uy[solid] = 0.0
     return rho, ux, uy
+
+
+def mirrored(ch: W.Channel) -> W.Channel:
+    """The channel reflected across its centre line (same flow, ux negated)."""
+    return W.Channel(w=ch.w, h=ch.h, circles=[(ch.h - 1 - r, c, rad) for (r, c, rad) in ch.circles], re=ch.re,
+                     u_in=ch.u_in)
 
 
 def cold_state(ch: W.Channel):

@@ -112,13 +112,22 @@ def summarize(rows, max_steps: int):
 def dataset(ctx: Context, chans, args):
     fields, steps, res = WS.steady_states(ctx, chans, tol=args.data_tol, interval=100, chunk=args.chunk,
                                           max_steps=args.data_max_steps, storage=args.storage, log=log)
-    xs, ts, ws = [], [], []
-    for c, f in zip(chans, fields):
-        t, w = WS.encode_targets(c, *f)
-        xs.append(WS.features(c))
-        ts.append(t)
-        ws.append(w)
-    return np.stack(xs), np.stack(ts), np.stack(ws), steps, res
+    n = len(chans) * (2 if args.mirror else 1)
+    h, w = chans[0].h, chans[0].w
+    x = np.empty((n, h, w, 8), np.float32)
+    t = np.empty((n, h, w, 3), np.float32)
+    wt = np.empty((n, h, w), np.float32)
+    for i, (c, f) in enumerate(zip(chans, fields)):
+        t[i], wt[i] = WS.encode_targets(c, *f)
+        x[i] = WS.features(c)
+        if args.mirror:
+            # reflection across the centre line: rows reversed, ux negated
+            j = len(chans) + i
+            x[j] = WS.features(WS.mirrored(c))
+            t[j] = t[i, ::-1]
+            t[j, ..., 1] *= -1
+            wt[j] = wt[i, ::-1]
+    return x, t, wt, steps, res
 
 
 def main(argv=None) -> int:
@@ -133,6 +142,7 @@ def main(argv=None) -> int:
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--train-seconds", type=float, default=1e9, help="stop training after this long")
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--mirror", type=int, default=1, help="add the mirrored channels to the training set")
     ap.add_argument("--storage", default="f32dev")
     ap.add_argument("--eval-storage", default="f32dev")
     ap.add_argument("--data-tol", type=float, default=1e-6)
@@ -164,11 +174,12 @@ def main(argv=None) -> int:
         lr_final = args.lr * 0.05
         rng = np.random.default_rng(args.seed_train)
         net.set_dataset(x, t, w)
+        n_samples = x.shape[0]
         del x, t, w
         for e in range(args.epochs):
             frac = e / max(1, args.epochs - 1)
             lr_e = lr_final + 0.5 * (args.lr - lr_final) * (1 + np.cos(np.pi * frac))
-            loss = net.train_epoch(rng.permutation(args.train).astype(np.int32), lr_e)
+            loss = net.train_epoch(rng.permutation(n_samples).astype(np.int32), lr_e)
             hist.append(loss)
             log(f"[train] epoch {e + 1}/{args.epochs} lr {lr_e:.2e} loss {loss:.6f} "
                 f"({time.perf_counter() - t1:.0f}s)")

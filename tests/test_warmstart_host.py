@@ -1,0 +1,59 @@
+"""Host-side logic of the learned warm start (no GPU): input features,
+target encoding, mirror augmentation and the lubrication estimate."""
+import numpy as np
+
+from chemsim_amd import warmstart as WS
+from chemsim_amd import workloads as W
+
+
+def test_lubrication_empty_channel_is_poiseuille():
+    ch = W.Channel(w=64, h=34, circles=[], re=40.0, u_in=0.04)
+    drho, uy = WS.lubrication(ch)
+    H = ch.h - 2
+    x = (np.arange(H) + 0.5) / H
+    np.testing.assert_allclose(uy[1:-1, 5], 6 * ch.u_in * x * (1 - x), rtol=1e-12)
+    assert np.all(uy[[0, -1]] == 0)
+    # pressure gradient 12 nu Q / H^3 per column, zero at the outlet column
+    np.testing.assert_allclose(drho[10, :], 36 * ch.nu * ch.u_in * H * (ch.w - 1 - np.arange(ch.w)) / H ** 3,
+                               rtol=1e-12)
+    np.testing.assert_allclose(drho[10, 0], WS.rho_scale(ch), rtol=1e-12)
+
+
+def test_lubrication_flux_split():
+    """An obstacle splits a column into gaps; the flux through the column is
+    the inlet flux and the gaps share it as gap^3."""
+    ch = W.Channel(w=80, h=66, circles=[(24, 40, 10)], re=30.0, u_in=0.05)
+    drho, uy = WS.lubrication(ch)
+    q = ch.u_in * (ch.h - 2)
+    # sum of the discrete Poiseuille profile (midpoint rule) is q (1 - 1/(g^2)) per gap
+    np.testing.assert_allclose(uy[:, 40].sum(), q, rtol=2e-3)
+    assert drho[5, 0] > drho[5, -20] > 0 and np.all(np.diff(drho[5]) <= 1e-15)
+    solid = ch.flags() == W.WALL
+    assert np.all(uy[solid] == 0) and np.all(drho[solid] == 0)
+
+
+def test_features_and_targets_mirror():
+    ch = WS.make_channels(11, 1)[0]
+    m = WS.mirrored(ch)
+    np.testing.assert_array_equal(m.flags(), ch.flags()[::-1])
+    f, fm = WS.features(ch), WS.features(m)
+    for k in (0, 1, 3, 4, 5, 6, 7):
+        np.testing.assert_allclose(fm[::-1, :, k], f[..., k], atol=1e-6)
+    np.testing.assert_allclose(fm[::-1, :, 2], -f[..., 2], atol=1e-6)
+    rng = np.random.default_rng(0)
+    rho = 1 + 0.01 * rng.standard_normal((ch.h, ch.w))
+    ux, uy = 0.05 * rng.standard_normal((2, ch.h, ch.w))
+    t, wgt = WS.encode_targets(ch, rho, ux, uy)
+    r2, x2, y2 = WS.decode_prediction(ch, t)
+    fluid = wgt > 0
+    np.testing.assert_allclose(r2[fluid], rho[fluid], rtol=1e-6)
+    np.testing.assert_allclose(x2[fluid], ux[fluid], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(y2[fluid], uy[fluid], rtol=1e-5, atol=1e-9)
+    assert np.all(r2[~fluid] == 1) and np.all(x2[~fluid] == 0)
+
+
+def test_signed_distance():
+    ch = W.Channel(w=40, h=30, circles=[(15, 20, 5)], re=30.0)
+    d = WS.signed_distance(ch)
+    assert abs(d[15, 20] + 5) < 1e-12 and abs(d[15, 28] - 3) < 1e-12
+    assert abs(d[1, 2] - 0.5) < 1e-12
