@@ -95,6 +95,7 @@ SIGNATURES = {
     "cs_lattice_init_equilibrium": [_P, _P, _P, _P],
     "cs_lattice_init_equilibrium_f64": [_P, _P, _P, _P],
     "cs_lattice_init_nonequilibrium": [_P, _P, _P, _P],
+    "cs_bgk_evaluate": [_P, C.c_int64, _P, _P, C.c_float, C.c_float, _P],
     "cs_lattice_step": [_P, _I],
     "cs_lattice_step_timed": [_P, _I, _FP],
     "cs_lattice_stream": [_P],

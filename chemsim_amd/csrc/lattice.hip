@@ -1136,6 +1136,36 @@ extern "C" int cs_lattice_run_to_residual(cs_lattice* l, double tol, int interva
     });
 }
 
+__global__ void k_bgk_evaluate(const float* __restrict__ f, const float* __restrict__ feq, float K,
+                               float* __restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f[i] + (f[i] - feq[i]) * K;
+}
+
+// BGK::evaluate (lbm.rs:350-364) on caller-supplied populations, for the
+// unfused plugin path: out = f + (f - f_eq) * K with K = (-dt) / tau in f32.
+extern "C" int cs_bgk_evaluate(cs_ctx* ctx, int64_t n, const float* f, const float* feq, float tau,
+                               float delta_t, float* out)
+{
+    return guarded([&] {
+        CS_REQUIRE(ctx && f && feq && out && n >= 0, CS_ERR_ARG, "bad argument");
+        CS_REQUIRE(tau > 0.f && delta_t > 0.f, CS_ERR_ARG, "tau and delta_t must be positive");
+        if (n == 0) return;
+        CS_HIP(hipSetDevice(ctx->device));
+        volatile float dt = delta_t, t = tau;
+        const float K = (-dt) / t;
+        DevBuf d((size_t)n * 3 * sizeof(float));
+        float* df = (float*)d.p;
+        CS_HIP(hipMemcpy(df, f, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+        CS_HIP(hipMemcpy(df + n, feq, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_bgk_evaluate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, df, df + n, K,
+                           df + 2 * n, n);
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipMemcpy(out, df + 2 * n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
 extern "C" int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy)
 {
     return guarded([&] {

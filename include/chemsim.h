@@ -180,6 +180,11 @@ int cs_lattice_pressure(cs_lattice* lat, float* p);                 /* :784-787 
 int cs_lattice_equilibrium(cs_lattice* lat, float* feq);            /* :156-160, 804-807 */
 int cs_lattice_non_equilibrium(cs_lattice* lat, float* fneq);       /* :162-173, 809-812 */
 int cs_lattice_is_unstable(cs_lattice* lat, int* unstable);         /* :814-818 */
+/* BGK::evaluate (lbm.rs:350-364) on caller-supplied host populations f and
+ * f_eq (n floats each, any layout): out = f + (f - f_eq) * K, K = (-dt)/tau in
+ * f32.  The unfused CollisionOperator path of a host layer; synchronous. */
+int cs_bgk_evaluate(cs_ctx* ctx, int64_t n, const float* f, const float* feq, float tau, float delta_t,
+                    float* out);
 /* Moments in the working precision, widened to f64 (any null output is skipped). */
 int cs_lattice_moments_f64(cs_lattice* lat, double* rho, double* ux, double* uy);
 

@@ -1,0 +1,60 @@
+"""The C++ drop-in layer (chemsim_amd/host/lbm.hpp), driven by
+tests/cpp/lbm_host_test.cpp the way main.rs drives lbm.rs, against the
+oracle: the fused BGK step, the three phases called one by one, a user
+collision operator on the plugin path, BGK::evaluate, the moment fields and a
+geometry edit between steps -- value for value."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "..", "chemsim_amd", "lib", "lbm_host_test")
+W_, H_ = 96, 64
+
+
+def test_host_layer_cpu_checks():
+    """Discretization, directions and stencils, Matrix errors, BGK viscosity
+    and geometry versions (no GPU call)."""
+    r = subprocess.run([BIN, "cpu"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def setup():
+    op = O.params(W_, H_, tau=0.6)
+    rho = np.ones((H_, W_), np.float32)
+    ux = np.full((H_, W_), 0.02, np.float32)
+    uy = np.zeros((H_, W_), np.float32)
+    f0 = O.equilibrium(op, rho, ux, uy)
+    y, x = np.mgrid[0:H_, 0:W_]
+    r = np.sqrt((x - W_ / 2.0) ** 2 + (y - H_ / 2.0) ** 2)
+    solid = (r < 9.0) | (x == 0) | (y == 0) | (x == W_ - 1) | (y == H_ - 1)
+    return op, f0, solid.astype(np.uint8)
+
+
+@pytest.mark.gpu
+def test_host_layer_matches_oracle(tmp_path):
+    print(f"[host layer] {BIN} {tmp_path}", flush=True)
+    r = subprocess.run([BIN, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    load = lambda name, shape=(9, H_, W_): np.fromfile(tmp_path / name, np.float32).reshape(shape)
+    op, f0, flags = setup()
+    ref = O.run(op, f0, 20, flags)
+    np.testing.assert_array_equal(load("fused.f32"), ref)
+    np.testing.assert_array_equal(load("phases.f32"), O.run_unfused(op, f0, 20, flags))
+    np.testing.assert_array_equal(load("plugin.f32"), ref)
+    feq0 = O.equilibrium(op, O.density(f0), *O.velocity(f0))
+    np.testing.assert_array_equal(load("bgk_eval.f32"), O.bgk(op, f0, feq0))
+    m = load("moments.f32", (5, H_, W_))
+    vx, vy = O.velocity(ref)
+    np.testing.assert_array_equal(m[0], O.density(ref))
+    np.testing.assert_array_equal(m[1], vx)
+    np.testing.assert_array_equal(m[2], vy)
+    np.testing.assert_array_equal(m[3], np.sqrt(vx * vx + vy * vy))
+    np.testing.assert_array_equal(m[4], O.density(ref) * np.float32(O.consts_f32(op).cs2))
+    flags2 = flags.copy()
+    flags2[10, 20:30] = 1
+    np.testing.assert_array_equal(load("edited.f32"), O.run(op, O.run(op, f0, 10, flags), 10, flags2))
