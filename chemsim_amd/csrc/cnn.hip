@@ -159,20 +159,61 @@ __global__ void k_relu_back(const float* __restrict__ dy, const uint16_t* __rest
     dz[i] = bf16_rne(g);
 }
 
-// Bias gradient: db[c] = sum_p dz[p][c] (one block per channel, fixed order).
-__global__ void k_bias_grad(const uint16_t* __restrict__ dz, float* __restrict__ db, int64_t P, int C)
+// Bias gradient, first stage: block b sums the rows [b*rows, (b+1)*rows) of
+// dz [P][C] per channel into part[b][C] (C divides 256; fixed order).  The
+// second stage is k_reduce_splits over the blocks.
+__global__ void k_bias_partial(const uint16_t* __restrict__ dz, float* __restrict__ part, int64_t P, int C,
+                               int64_t rows)
 {
     __shared__ float sh[256];
-    const int c = blockIdx.x;
+    const int t = threadIdx.x, step = 256 / C;
+    const int c = t % C, r0 = t / C;
+    const int64_t pb = (int64_t)blockIdx.x * rows, pe = min(P, pb + rows);
     float s = 0.f;
-    for (int64_t p = threadIdx.x; p < P; p += 256) s += bf16_f32(dz[p * C + c]);
-    sh[threadIdx.x] = s;
+    for (int64_t p = pb + r0; p < pe; p += step) s += bf16_f32(dz[p * C + c]);
+    sh[t] = s;
+    __syncthreads();
+    if (t < C) {
+        float v = 0.f;
+        for (int j = 0; j < step; ++j) v += sh[j * C + t];
+        part[(int64_t)blockIdx.x * C + t] = v;
+    }
+}
+
+// Column sums of part[rows][C] in a fixed order: one block per column.
+__global__ void k_colsum(const float* __restrict__ part, float* __restrict__ out, int rows, int C)
+{
+    __shared__ float sh[256];
+    const int c = blockIdx.x, t = threadIdx.x;
+    float s = 0.f;
+    for (int r = t; r < rows; r += 256) s += part[(int64_t)r * C + c];
+    sh[t] = s;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        if (t < k) sh[t] += sh[t + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[c] = sh[0];
+    if (t == 0) out[c] = sh[0];
+}
+
+// dst[c][r] = src[r][c] for a bf16 [R][C] matrix, 64 x 64 tiles through LDS.
+__global__ void k_transpose16(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int64_t R, int C)
+{
+    __shared__ uint16_t tile[64][65];
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {
+        const int64_t r = r0 + i;
+        const int c = c0 + tx;
+        tile[i][tx] = (r < R && c < C) ? src[r * C + c] : (uint16_t)0;
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int c = c0 + i;
+        const int64_t r = r0 + tx;
+        if (c < C && r < R) dst[(int64_t)c * R + r] = tile[tx][i];
+    }
 }
 
 // Sum of split-K partials in fixed order.
@@ -290,6 +331,7 @@ struct cs_cnn {
     int64_t act_max = 0;
     float* skip32[3] = {nullptr, nullptr, nullptr};  // accumulated gradients of e1, e2, e3 outputs
     uint16_t* dz = nullptr;           // bf16 gradient (largest activation)
+    uint16_t* dzT = nullptr;          // its transpose [C][P] for the weight gradient
     float* loss_part = nullptr;
     float* targets = nullptr;
     float* weights = nullptr;
@@ -316,9 +358,16 @@ T* dalloc(cs_cnn* net, int64_t n)
 }
 
 void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, const uint16_t* B, int ldb, void* C, int ldc,
-          const float* bias, int M, int N, int K, int splits = 1)
+          const float* bias, int M, int N, int K, int splits = 1, const Act* gather = nullptr, int sign = 1)
 {
-    GemmArgs g;
+    GemmArgs g{};
+    if (la == GATHER) {
+        CS_REQUIRE(gather && gather->C % 8 == 0 && (int64_t)M == gather->P(), CS_ERR_ARG, "bad gather operand");
+        g.gH = gather->H;
+        g.gW = gather->W;
+        g.gC = gather->C;
+        g.gsign = sign;
+    }
     g.A = A;
     g.B = B;
     g.C = C;
@@ -329,17 +378,21 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
     g.lda = lda;
     g.ldb = ldb;
     g.ldc = ldc;
-    g.k_per_split = (K / splits + BK - 1) / BK * BK;
+    CS_REQUIRE(K % 8 == 0, CS_ERR_ARG, "gemm K must be a multiple of 8");
+    g.k_per_split = ((K + splits - 1) / splits + BK - 1) / BK * BK;  // splits cover [0, K)
     g.split_stride = (int64_t)M * ldc;
-    dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN), (unsigned)splits);
+    const int cfg = pick_tile(M, N);
+    int bm, bn;
+    tile_dims(cfg, bm, bn);
+    dim3 grid((unsigned)((M + bm - 1) / bm), (unsigned)((N + bn - 1) / bn), (unsigned)splits);
     net->gemm_flops += 2.0 * M * N * K;
-#define CS_GEMM(LA, LB, EPI) hipLaunchKernelGGL((k_gemm<LA, LB, EPI>), grid, dim3(256), 0, net->stream, g)
-    if (la == KMAJOR && lb == KMAJOR) {
-        if (epi == EPI_F32) CS_GEMM(KMAJOR, KMAJOR, EPI_F32);
-        else if (epi == EPI_BF16) CS_GEMM(KMAJOR, KMAJOR, EPI_BF16);
-        else CS_GEMM(KMAJOR, KMAJOR, EPI_BF16_RELU);
-    } else if (la == RMAJOR && lb == RMAJOR && epi == EPI_F32) {
-        CS_GEMM(RMAJOR, RMAJOR, EPI_F32);
+#define CS_GEMM(LA, LB, EPI) launch_gemm<LA, LB, EPI>(cfg, grid, net->stream, g)
+    if (la == KMAJOR && lb == KMAJOR && epi == EPI_F32) {
+        CS_GEMM(KMAJOR, KMAJOR, EPI_F32);
+    } else if (la == GATHER && lb == KMAJOR) {
+        if (epi == EPI_F32) CS_GEMM(GATHER, KMAJOR, EPI_F32);
+        else if (epi == EPI_BF16) CS_GEMM(GATHER, KMAJOR, EPI_BF16);
+        else CS_GEMM(GATHER, KMAJOR, EPI_BF16_RELU);
     } else {
         throw Error{CS_ERR_ARG, "unsupported gemm layout"};
     }
@@ -347,22 +400,22 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
     CS_HIP(hipGetLastError());
 }
 
-void im2col(cs_cnn* net, const Act& x, int kpad, int sign)
+// Transposed gather for the weight gradient: col[k][p] (k < kpad, p < P).
+void im2col_t(cs_cnn* net, const Act& x, int kpad)
 {
-    const int64_t groups = x.P() * (kpad / 8);
-    CS_REQUIRE(x.P() * kpad <= net->col_elems, CS_ERR_STATE, "im2col scratch too small");
-    hipLaunchKernelGGL(k_im2col, dim3(blocks_for(groups)), dim3(256), 0, net->stream, x.d, net->col, x.N, x.H, x.W,
-                       x.C, kpad, sign);
+    const int64_t P = x.P();
+    CS_REQUIRE(P * kpad <= net->col_elems && P % 8 == 0, CS_ERR_STATE, "im2col scratch too small");
+    hipLaunchKernelGGL(k_im2col_t, dim3((unsigned)((P + 63) / 64), (unsigned)(kpad / 32)), dim3(256), 0, net->stream,
+                       x.d, net->col, x.N, x.H, x.W, x.C, kpad);
     CS_HIP(hipGetLastError());
 }
 
 void conv_forward(cs_cnn* net, int li, const Act& x, const Act& y, float* y32 = nullptr)
 {
     Conv& c = net->convs[(size_t)li];
-    im2col(net, x, c.kpad, +1);
-    if (y32) gemm(net, KMAJOR, KMAJOR, EPI_F32, net->col, c.kpad, c.wb, c.kpad, y32, 8, c.b, (int)x.P(), c.cout, c.kpad);
-    else gemm(net, KMAJOR, KMAJOR, c.relu ? EPI_BF16_RELU : EPI_BF16, net->col, c.kpad, c.wb, c.kpad, y.d, c.cout, c.b,
-              (int)x.P(), c.cout, c.kpad);
+    if (y32) gemm(net, GATHER, KMAJOR, EPI_F32, x.d, 0, c.wb, c.kpad, y32, 8, c.b, (int)x.P(), c.cout, c.kpad, 1, &x, 1);
+    else gemm(net, GATHER, KMAJOR, c.relu ? EPI_BF16_RELU : EPI_BF16, x.d, 0, c.wb, c.kpad, y.d, c.cout, c.b,
+              (int)x.P(), c.cout, c.kpad, 1, &x, 1);
 }
 
 // Backward of conv li given the f32 gradient dy of its output (post-ReLU).
@@ -374,13 +427,26 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
     hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(P * c.cout)), dim3(256), 0, net->stream, dy, y.d, net->dz,
                        P * c.cout, c.relu ? 1 : 0);
     CS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)c.cout), dim3(256), 0, net->stream, net->dz, c.gb, P, c.cout);
     CS_HIP(hipGetLastError());
-    // weight gradient: dW[co][k] = sum_p dz[p][co] col[p][k], split over p
-    im2col(net, x, c.kpad, +1);
+    CS_REQUIRE(c.cout <= 256 && 256 % c.cout == 0 && P % 8 == 0, CS_ERR_STATE, "unsupported layer shape");
+    {
+        const int64_t blocks = std::min<int64_t>(1024, (P + 255) / 256);
+        const int64_t rows = (P + blocks - 1) / blocks;
+        hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)blocks), dim3(256), 0, net->stream, net->dz, net->part, P,
+                           c.cout, rows);
+        hipLaunchKernelGGL(k_colsum, dim3((unsigned)c.cout), dim3(256), 0, net->stream, net->part, c.gb, (int)blocks,
+                           c.cout);
+        CS_HIP(hipGetLastError());
+    }
+    // weight gradient: dW[co][k] = sum_p dzT[co][p] colT[k][p], split over p;
+    // both operands transposed so that p (the reduction) is contiguous
+    hipLaunchKernelGGL(k_transpose16, dim3((unsigned)((P + 63) / 64), (unsigned)((c.cout + 63) / 64)), dim3(256), 0,
+                       net->stream, net->dz, net->dzT, P, c.cout);
+    CS_HIP(hipGetLastError());
+    im2col_t(net, x, c.kpad);
     int splits = (int)std::max<int64_t>(1, std::min<int64_t>(64, P / 2048));
     while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
-    gemm(net, RMAJOR, RMAJOR, EPI_F32, net->dz, c.cout, net->col, c.kpad, net->part, c.kpad, nullptr, c.cout, c.kpad,
+    gemm(net, KMAJOR, KMAJOR, EPI_F32, net->dzT, (int)P, net->col, (int)P, net->part, c.kpad, nullptr, c.cout, c.kpad,
          (int)P, splits);
     hipLaunchKernelGGL(k_reduce_splits, dim3(blocks_for(c.nw())), dim3(256), 0, net->stream, net->part, c.gw, c.nw(),
                        splits);
@@ -388,9 +454,8 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
     if (!dx32) return;
     // data gradient: dx[p][ci] = sum_k colT(dz)[p][k] wd[ci][k]
     Act dzact{net->dz, x.N, x.H, x.W, c.cout};
-    im2col(net, dzact, c.kpad_d, -1);
-    gemm(net, KMAJOR, KMAJOR, EPI_F32, net->col, c.kpad_d, c.wd, c.kpad_d, dx32, c.cin, nullptr, (int)P, c.cin,
-         c.kpad_d);
+    gemm(net, GATHER, KMAJOR, EPI_F32, net->dz, 0, c.wd, c.kpad_d, dx32, c.cin, nullptr, (int)P, c.cin, c.kpad_d, 1,
+         &dzact, -1);
 }
 
 void forward(cs_cnn* net, int n)
@@ -686,6 +751,7 @@ extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int b
             net->act_max = amax2;
             net->dy32 = dalloc<float>(net, 2 * amax2);
             net->dz = dalloc<uint16_t>(net, amax2);
+            net->dzT = dalloc<uint16_t>(net, amax2);
             net->skip32[0] = dalloc<float>(net, net->conv_out[1].n());
             net->skip32[1] = dalloc<float>(net, net->conv_out[3].n());
             net->skip32[2] = dalloc<float>(net, net->conv_out[5].n());
@@ -947,7 +1013,7 @@ extern "C" int cs_gemm_bf16(cs_ctx* ctx, int M, int N, int K, const uint16_t* A,
             CS_HIP(hipMalloc(&dc, sizeof(float) * (size_t)M * N));
             CS_HIP(hipMemcpy(da, A, sizeof(uint16_t) * (size_t)M * K, hipMemcpyHostToDevice));
             CS_HIP(hipMemcpy(db, B, sizeof(uint16_t) * (size_t)N * K, hipMemcpyHostToDevice));
-            GemmArgs g;
+            GemmArgs g{};
             g.A = (const uint16_t*)da;
             g.B = (const uint16_t*)db;
             g.C = dc;
@@ -960,14 +1026,17 @@ extern "C" int cs_gemm_bf16(cs_ctx* ctx, int M, int N, int K, const uint16_t* A,
             g.ldc = N;
             g.k_per_split = K;
             g.split_stride = 0;
-            dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN), 1);
+            const int cfg = pick_tile(M, N);
+            int bm, bn;
+            tile_dims(cfg, bm, bn);
+            dim3 grid((unsigned)((M + bm - 1) / bm), (unsigned)((N + bn - 1) / bn), 1);
             hipEvent_t e0, e1;
             CS_HIP(hipEventCreate(&e0));
             CS_HIP(hipEventCreate(&e1));
-            hipLaunchKernelGGL((k_gemm<KMAJOR, KMAJOR, EPI_F32>), grid, dim3(256), 0, s, g);
+            launch_gemm<KMAJOR, KMAJOR, EPI_F32>(cfg, grid, s, g);
             CS_HIP(hipEventRecord(e0, s));
             const int iters = 10;
-            for (int i = 0; i < iters; ++i) hipLaunchKernelGGL((k_gemm<KMAJOR, KMAJOR, EPI_F32>), grid, dim3(256), 0, s, g);
+            for (int i = 0; i < iters; ++i) launch_gemm<KMAJOR, KMAJOR, EPI_F32>(cfg, grid, s, g);
             CS_HIP(hipEventRecord(e1, s));
             CS_HIP(hipEventSynchronize(e1));
             float t = 0;

@@ -2,14 +2,14 @@
 //
 // Activations are NHWC bf16 (channels contiguous), weights are [Cout][K] with
 // K = 9*Cin ordered (tap, ci), tap = 3*(dy+1) + (dx+1).  A 3x3 convolution is
-// an implicit GEMM; here the gather is explicit (im2col into a [pixels][Kpad]
-// bf16 buffer) and one GEMM kernel serves the forward pass, the data gradient
-// and the weight gradient:
-//   forward  out[p][co]  = sum_k col(in)[p][k]  * W[co][k]            (NT)
-//   dgrad    din[p][ci]  = sum_k colT(dout)[p][k] * Wd[ci][k]         (NT)
-//   wgrad    dW[co][k]   = sum_p dout[p][co] * col(in)[p][k]          (TN, split over p)
-// GEMM tiles are 64x64x32 in LDS, 4 waves of 64 lanes, each wave a 32x32
-// quarter as 2x2 v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+// a GEMM over the gathered input col(in)[p][k]; one GEMM kernel serves the
+// forward pass, the data gradient and the weight gradient:
+//   forward  out[p][co] = sum_k col(in)[p][k] * W[co][k]    gather inside the GEMM
+//   dgrad    din[p][ci] = sum_k colT(dout)[p][k] * Wd[ci][k] gather (flipped taps) inside the GEMM
+//   wgrad    dW[co][k]  = sum_p doutT[co][p] * colT[k][p]   both transposed in memory, split over p
+// GEMM tiles (BM x BN x 32, chosen by shape) are double-buffered in LDS;
+// 4 waves of 64 lanes each own a sub-tile of v_mfma_f32_16x16x32_bf16
+// fragments with FP32 accumulation.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,7 +34,10 @@ __device__ __forceinline__ float bf16_f32(uint16_t b) { return __uint_as_float((
 
 // Operand layouts.  KMAJOR: element (r, k) at base[r*ld + k] (k contiguous).
 // RMAJOR: element (r, k) at base[k*ld + r] (the row index contiguous).
-enum Layout : int { KMAJOR = 0, RMAJOR = 1 };
+// GATHER: the A operand of a 3x3 convolution read straight from the NHWC
+// input (implicit im2col): row r = pixel, k = tap*C + c, zero outside the
+// image and for k >= 9C.
+enum Layout : int { KMAJOR = 0, RMAJOR = 1, GATHER = 2 };
 
 enum Epi : int {
     EPI_F32 = 0,        // C (+ bias) stored as f32
@@ -47,81 +50,209 @@ struct GemmArgs {
     const uint16_t* B;  // rows N
     void* C;            // [M][ldc] (or [splits][M][ldc] partials for EPI_F32 with split-K)
     const float* bias;  // [N] or null
-    int M, N, K;        // K: multiple of 32 (operands zero-padded)
+    int M, N, K;        // K: multiple of 8; a K range's tail beyond K (or the split's end) reads as zero
     int lda, ldb, ldc;
     int k_per_split;    // K range of one blockIdx.z (multiple of 32)
     int64_t split_stride;  // elements between the partial outputs of consecutive z
+    int gH, gW, gC, gsign;  // GATHER: input height, width, channels (multiple of 8), tap sign (+1 / -1)
 };
 
-constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int BK = 32;
 constexpr int LDS_PITCH = BK + 8;  // bf16 elements per LDS row (padding against bank conflicts)
 
-// Loads one 64x32 tile of an operand into LDS as [row][k].
-template <int L>
-__device__ __forceinline__ void load_tile(const uint16_t* __restrict__ src, int ld, int rows, int r0, int k0,
-                                          uint16_t (*lds)[LDS_PITCH])
-{
-    const int t = threadIdx.x;
-    if (L == KMAJOR) {
-        const int r = t >> 2, kg = t & 3;  // 64 rows x 4 groups of 8 k
-        u32x4 v = {0u, 0u, 0u, 0u};
-        if (r0 + r < rows) v = *(const u32x4*)(src + (int64_t)(r0 + r) * ld + k0 + 8 * kg);
-        *(u32x4*)(&lds[r][8 * kg]) = v;
-    } else {
-        const int kk = t >> 3, rg = t & 7;  // 32 k x 8 groups of 8 rows
-        u32x4 v = {0u, 0u, 0u, 0u};
-        const int r = r0 + 8 * rg;
-        if (r + 7 < rows) {
-            v = *(const u32x4*)(src + (int64_t)(k0 + kk) * ld + r);
-        } else {
-            uint16_t tmp[8];
-            for (int j = 0; j < 8; ++j) tmp[j] = (r + j < rows) ? src[(int64_t)(k0 + kk) * ld + r + j] : (uint16_t)0;
-            v = *(const u32x4*)tmp;
-        }
-        const uint16_t* e = (const uint16_t*)&v;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) lds[8 * rg + j][kk] = e[j];
-    }
-}
+// One BK-wide slice of an R-row operand tile: global -> registers (NV 16-byte
+// vectors per thread, zero outside `rows`), registers -> LDS as [row][k].
+template <int L, int R>
+struct TileLoader {
+    static constexpr int VECS = 4 * R;                  // R rows x BK/8 vectors
+    static constexpr int NV = (VECS + 255) / 256;
+    u32x4 v[NV];
 
-template <int LA, int LB, int EPI>
+    // k >= kend reads as zero (kend: end of this block's K range, a multiple of 8)
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ src, int ld, int rows, int r0, int k0, int kend)
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            v[i] = u32x4{0u, 0u, 0u, 0u};
+            if (VECS % 256 != 0 && e >= VECS) continue;
+            if (L == KMAJOR) {
+                const int r = e >> 2, kg = e & 3;
+                if (r0 + r < rows && k0 + 8 * kg < kend)
+                    v[i] = *(const u32x4*)(src + (int64_t)(r0 + r) * ld + k0 + 8 * kg);
+            } else {
+                const int kk = e / (R / 8), rg = e - kk * (R / 8);
+                const int r = r0 + 8 * rg;
+                const uint16_t* p = src + (int64_t)(k0 + kk) * ld + r;
+                if (k0 + kk >= kend) {
+                    // zero
+                } else if (r + 7 < rows) {
+                    v[i] = *(const u32x4*)p;
+                } else {
+                    uint16_t tmp[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) tmp[j] = (r + j < rows) ? p[j] : (uint16_t)0;
+                    v[i] = *(const u32x4*)tmp;
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            if (VECS % 256 != 0 && e >= VECS) continue;
+            if (L == KMAJOR) {
+                const int r = e >> 2, kg = e & 3;
+                *(u32x4*)(&lds[r][8 * kg]) = v[i];
+            } else {
+                const int kk = e / (R / 8), rg = e - kk * (R / 8);
+                const uint16_t* x = (const uint16_t*)&v[i];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) lds[8 * rg + j][kk] = x[j];
+            }
+        }
+    }
+};
+
+// The GATHER operand: each vector slot's pixel is fixed for the whole K loop,
+// so its coordinates are computed once; a K slice then only decodes the tap.
+template <int R>
+struct GatherLoader {
+    static constexpr int VECS = 4 * R;
+    static constexpr int NV = (VECS + 255) / 256;
+    u32x4 v[NV];
+    int y[NV], x[NV];
+    int64_t base[NV];  // element offset of channel 0 of the slot's pixel
+
+    __device__ __forceinline__ void init(const GemmArgs& g, int r0)
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            const int p = r0 + (e >> 2);
+            y[i] = -(1 << 28);  // marks an unused slot: every tap falls outside
+            x[i] = 0;
+            base[i] = 0;
+            if ((VECS % 256 == 0 || e < VECS) && p < g.M) {
+                x[i] = p % g.gW;
+                const int ny = p / g.gW;
+                y[i] = ny % g.gH;
+                base[i] = (int64_t)p * g.gC;
+            }
+        }
+    }
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ in, const GemmArgs& g, int k0, int kend)
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            const int k = k0 + 8 * (e & 3);
+            v[i] = u32x4{0u, 0u, 0u, 0u};
+            if (k < kend && k < 9 * g.gC) {
+                const int tap = k / g.gC, c = k - tap * g.gC;
+                const int dy = g.gsign * (tap / 3 - 1), dx = g.gsign * (tap % 3 - 1);
+                const int yy = y[i] + dy, xx = x[i] + dx;
+                if (yy >= 0 && yy < g.gH && xx >= 0 && xx < g.gW)
+                    v[i] = *(const u32x4*)(in + base[i] + (int64_t)(dy * g.gW + dx) * g.gC + c);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            if (VECS % 256 != 0 && e >= VECS) continue;
+            *(u32x4*)(&lds[e >> 2][8 * (e & 3)]) = v[i];
+        }
+    }
+};
+
+template <int L, int R>
+struct LoaderA {
+    TileLoader<L, R> l;
+    __device__ __forceinline__ void init(const GemmArgs&, int) {}
+    __device__ __forceinline__ void load(const GemmArgs& g, int r0, int k0, int kend) { l.load(g.A, g.lda, g.M, r0, k0, kend); }
+    __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const { l.store(lds); }
+};
+template <int R>
+struct LoaderA<GATHER, R> {
+    GatherLoader<R> l;
+    __device__ __forceinline__ void init(const GemmArgs& g, int r0) { l.init(g, r0); }
+    __device__ __forceinline__ void load(const GemmArgs& g, int, int k0, int kend) { l.load(g.A, g, k0, kend); }
+    __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const { l.store(lds); }
+};
+
+// C[M][N] = A B^T over one K range per blockIdx.z.  Block tile BM x BN,
+// 4 waves as WM x WN, each wave a (BM/WM) x (BN/WN) tile of 16x16 MFMA
+// fragments.  LDS is double-buffered: the next K slice is fetched into
+// registers while the MFMAs run on the current one, one barrier per slice.
+template <int LA, int LB, int EPI, int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 {
-    __shared__ __align__(16) uint16_t As[BM][LDS_PITCH];
-    __shared__ __align__(16) uint16_t Bs[BN][LDS_PITCH];
+    static_assert(WM * WN == 4, "4 waves");
+    constexpr int TM = BM / WM / 16, TN = BN / WN / 16;
+    static_assert(TM >= 1 && TN >= 1 && BM % 8 == 0 && BN % 8 == 0, "tile shape");
+    __shared__ __align__(16) uint16_t As[2][BM][LDS_PITCH];
+    __shared__ __align__(16) uint16_t Bs[2][BN][LDS_PITCH];
     const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
     const int kb = blockIdx.z * g.k_per_split;
     const int ke = min(g.K, kb + g.k_per_split);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave / WN) * (BM / WM), wn = (wave % WN) * (BN / WN);
     const int fr = lane & 15, fk = 8 * (lane >> 4);
-    f32x4 acc[2][2];
+    f32x4 acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    LoaderA<LA, BM> la;
+    TileLoader<LB, BN> lb;
+    la.init(g, m0);
+    if (kb < ke) {
+        la.load(g, m0, kb, ke);
+        lb.load(g.B, g.ldb, g.N, n0, kb, ke);
+        la.store(As[0]);
+        lb.store(Bs[0]);
+    }
+    __syncthreads();
+    int cur = 0;
     for (int k0 = kb; k0 < ke; k0 += BK) {
-        load_tile<LA>(g.A, g.lda, g.M, m0, k0, As);
-        load_tile<LB>(g.B, g.ldb, g.N, n0, k0, Bs);
+        const bool more = k0 + BK < ke;
+        if (more) {
+            la.load(g, m0, k0 + BK, ke);
+            lb.load(g.B, g.ldb, g.N, n0, k0 + BK, ke);
+        }
+        bf16x8 a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = *(const bf16x8*)(&As[cur][wm + 16 * i + fr][fk]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = *(const bf16x8*)(&Bs[cur][wn + 16 * j + fr][fk]);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (more) {
+            la.store(As[cur ^ 1]);
+            lb.store(Bs[cur ^ 1]);
+        }
         __syncthreads();
-        bf16x8 a[2], b[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8*)(&As[wm + 16 * i + fr][fk]);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8*)(&Bs[wn + 16 * j + fr][fk]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        __syncthreads();
+        cur ^= 1;
     }
 
     // D layout: lane holds rows 4*(lane/16) + 0..3 of column lane % 16.
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < TN; ++j) {
             const int col = n0 + wn + 16 * j + fr;
             if (col >= g.N) continue;
             const float bias = g.bias ? g.bias[col] : 0.f;
@@ -140,32 +271,81 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
         }
 }
 
-// im2col of an NHWC bf16 tensor: col[p][k], k = tap*C + c < 9C, zero beyond
-// (up to Kpad) and outside the image.  sign = +1 gathers in[y+dy][x+dx]
-// (forward), -1 gathers in[y-dy][x-dx] (data gradient).  One thread writes 8
-// consecutive k (C is a multiple of 8).
-__global__ void k_im2col(const uint16_t* __restrict__ in, uint16_t* __restrict__ col, int N, int H, int W, int C,
-                         int Kpad, int sign)
+// Tile shapes by the GEMM's N (columns of C): the network's GEMMs are
+// tall (M = pixels) with N = 8..256 output channels, or, for the weight
+// gradient, short (M = output channels) and wide (N = 9 x input channels).
+enum TileCfg : int { TILE_256x32 = 0, TILE_128x64 = 1, TILE_128x128 = 2, TILE_32x256 = 3, TILE_64x256 = 4 };
+
+inline int pick_tile(int M, int N)
 {
-    const int64_t groups = (int64_t)N * H * W * (Kpad / 8);
-    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= groups) return;
-    const int kg = (int)(gi % (Kpad / 8));
-    const int64_t p = gi / (Kpad / 8);
-    const int k = 8 * kg;
-    u32x4 v = {0u, 0u, 0u, 0u};
-    if (k < 9 * C) {
-        const int tap = k / C, c = k - tap * C;
-        const int dy = sign * (tap / 3 - 1), dx = sign * (tap % 3 - 1);
-        const int x = (int)(p % W);
-        const int64_t ny = p / W;
-        const int y = (int)(ny % H);
-        const int64_t n = ny / H;
-        const int yy = y + dy, xx = x + dx;
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-            v = *(const u32x4*)(in + ((n * H + yy) * W + xx) * C + c);
+    if (M <= 32 && N >= 128) return TILE_32x256;
+    if (M <= 64 && N >= 128) return TILE_64x256;
+    if (N <= 32) return TILE_256x32;
+    if (N <= 64) return TILE_128x64;
+    return TILE_128x128;
+}
+
+inline void tile_dims(int cfg, int& bm, int& bn)
+{
+    switch (cfg) {
+    case TILE_256x32: bm = 256; bn = 32; break;
+    case TILE_128x64: bm = 128; bn = 64; break;
+    case TILE_32x256: bm = 32; bn = 256; break;
+    case TILE_64x256: bm = 64; bn = 256; break;
+    default: bm = 128; bn = 128; break;
     }
-    *(u32x4*)(col + p * Kpad + k) = v;
+}
+
+template <int LA, int LB, int EPI>
+inline void launch_gemm(int cfg, dim3 grid, hipStream_t s, const GemmArgs& g)
+{
+    switch (cfg) {
+    case TILE_256x32: hipLaunchKernelGGL((k_gemm<LA, LB, EPI, 256, 32, 4, 1>), grid, dim3(256), 0, s, g); break;
+    case TILE_128x64: hipLaunchKernelGGL((k_gemm<LA, LB, EPI, 128, 64, 2, 2>), grid, dim3(256), 0, s, g); break;
+    case TILE_32x256: hipLaunchKernelGGL((k_gemm<LA, LB, EPI, 32, 256, 1, 4>), grid, dim3(256), 0, s, g); break;
+    case TILE_64x256: hipLaunchKernelGGL((k_gemm<LA, LB, EPI, 64, 256, 1, 4>), grid, dim3(256), 0, s, g); break;
+    default: hipLaunchKernelGGL((k_gemm<LA, LB, EPI, 128, 128, 2, 2>), grid, dim3(256), 0, s, g); break;
+    }
+}
+
+// Transposed im2col for the weight gradient: col[k][p] = element (p, k) of
+// the forward gather (k = tap*C + c < Kpad; zero for k >= 9C and outside the
+// image).  A block builds a 32 (k) x 64 (pixel) tile: 16-byte gathers along
+// the channels of each pixel, a transpose through LDS, then 16-byte stores
+// along the pixels of each k row.  P and C are multiples of 8.
+__global__ __launch_bounds__(256) void k_im2col_t(const uint16_t* __restrict__ in, uint16_t* __restrict__ col, int N,
+                                                  int H, int W, int C, int Kpad)
+{
+    __shared__ uint16_t tile[32][64 + 8];
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t p0 = (int64_t)blockIdx.x * 64;
+    const int k0 = blockIdx.y * 32;
+    const int t = threadIdx.x;
+    {
+        const int j = t >> 2, kv = t & 3;  // pixel p0 + j, channels k0 + 8kv .. +7
+        const int64_t p = p0 + j;
+        const int k = k0 + 8 * kv;
+        u32x4 v = {0u, 0u, 0u, 0u};
+        if (p < P && k < 9 * C) {
+            const int tap = k / C, c = k - tap * C;
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const int x = (int)(p % W);
+            const int64_t ny = p / W;
+            const int y = (int)(ny % H);
+            const int yy = y + dy, xx = x + dx;
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = *(const u32x4*)(in + (p + dy * W + dx) * C + c);
+        }
+        const uint16_t* e = (const uint16_t*)&v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) tile[8 * kv + i][j] = e[i];
+    }
+    __syncthreads();
+    {
+        const int row = t >> 3, q = t & 7;  // k0 + row, pixels p0 + 8q .. +7
+        const int64_t p = p0 + 8 * q;
+        if (k0 + row < Kpad && p < P)
+            *(u32x4*)(col + (int64_t)(k0 + row) * P + p) = *(const u32x4*)(&tile[row][8 * q]);
+    }
 }
 
 }  // namespace cnn

@@ -98,16 +98,30 @@ def features(ch: W.Channel) -> np.ndarray:
     return f
 
 
-def encode_targets(ch: W.Channel, rho, ux, uy) -> tuple[np.ndarray, np.ndarray]:
-    """Targets ((rho - 1) / rho_scale, ux / u_in, uy / u_in) and the loss
-    weight (1 at fluid cells, 0 at solid ones)."""
+def baseline(ch: W.Channel, feats: np.ndarray | None = None) -> np.ndarray:
+    """The lubrication estimate in target units, [h][w][3]: the network
+    predicts the difference to it (residual learning)."""
+    f = features(ch) if feats is None else feats
+    b = np.zeros((ch.h, ch.w, 3), np.float32)
+    b[..., 0] = f[..., 5]
+    b[..., 2] = f[..., 4]
+    return b
+
+
+def encode_targets(ch: W.Channel, rho, ux, uy, feats: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """Targets ((rho - 1) / rho_scale, ux / u_in, uy / u_in) minus the
+    lubrication baseline, and the loss weight (1 at fluid cells, 0 at solid
+    ones)."""
     t = np.stack([(rho - 1.0) / rho_scale(ch), ux / ch.u_in, uy / ch.u_in], axis=-1).astype(np.float32)
+    t -= baseline(ch, feats)
     wgt = (ch.flags() != W.WALL).astype(np.float32)
     t[wgt == 0] = 0.0
     return t, wgt
 
 
-def decode_prediction(ch: W.Channel, y: np.ndarray):
+def decode_prediction(ch: W.Channel, y: np.ndarray, feats: np.ndarray | None = None):
+    """(rho, ux, uy) from the network output (baseline added back)."""
+    y = y[..., :3] + baseline(ch, feats)
     solid = ch.flags() == W.WALL
     rho = (1.0 + rho_scale(ch) * y[..., 0]).astype(np.float32)
     ux = (ch.u_in * y[..., 1]).astype(np.float32)

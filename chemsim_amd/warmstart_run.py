@@ -51,7 +51,7 @@ def predict(net: Net, chans):
     t0 = time.perf_counter()
     y = net.forward(x)
     dt = (time.perf_counter() - t0) / len(chans)
-    return [WS.decode_prediction(c, y[i]) for i, c in enumerate(chans)], dt
+    return [WS.decode_prediction(c, y[i], x[i]) for i, c in enumerate(chans)], dt
 
 
 def evaluate(ctx: Context, net: Net, chans, storage: str = "f32dev", tol: float = 1e-6, interval: int = 100,
@@ -118,8 +118,8 @@ def dataset(ctx: Context, chans, args):
     t = np.empty((n, h, w, 3), np.float32)
     wt = np.empty((n, h, w), np.float32)
     for i, (c, f) in enumerate(zip(chans, fields)):
-        t[i], wt[i] = WS.encode_targets(c, *f)
         x[i] = WS.features(c)
+        t[i], wt[i] = WS.encode_targets(c, *f, feats=x[i])
         if args.mirror:
             # reflection across the centre line: rows reversed, ux negated
             j = len(chans) + i

@@ -45,10 +45,13 @@ def test_features_and_targets_mirror():
     ux, uy = 0.05 * rng.standard_normal((2, ch.h, ch.w))
     t, wgt = WS.encode_targets(ch, rho, ux, uy)
     r2, x2, y2 = WS.decode_prediction(ch, t)
+    # the mirrored channel's targets are the reflected ones with ux negated
+    tm, _ = WS.encode_targets(m, rho[::-1], -ux[::-1], uy[::-1])
+    np.testing.assert_allclose(tm[::-1] * [1, -1, 1], t, atol=1e-5)
     fluid = wgt > 0
     np.testing.assert_allclose(r2[fluid], rho[fluid], rtol=1e-6)
-    np.testing.assert_allclose(x2[fluid], ux[fluid], rtol=1e-5, atol=1e-9)
-    np.testing.assert_allclose(y2[fluid], uy[fluid], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(x2[fluid], ux[fluid], rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(y2[fluid], uy[fluid], rtol=1e-5, atol=1e-8)
     assert np.all(r2[~fluid] == 1) and np.all(x2[~fluid] == 0)
 
 
