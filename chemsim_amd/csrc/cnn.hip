@@ -444,7 +444,11 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
                        net->stream, net->dz, net->dzT, P, c.cout);
     CS_HIP(hipGetLastError());
     im2col_t(net, x, c.kpad);
-    int splits = (int)std::max<int64_t>(1, std::min<int64_t>(64, P / 2048));
+    // enough K ranges for ~1024 workgroups, each at least 1024 pixels long
+    int bm, bn;
+    tile_dims(pick_tile(c.cout, c.kpad), bm, bn);
+    const int64_t tiles = (int64_t)((c.cout + bm - 1) / bm) * ((c.kpad + bn - 1) / bn);
+    int splits = (int)std::max<int64_t>(1, std::min<int64_t>({256, P / 1024, (1024 + tiles - 1) / tiles}));
     while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
     gemm(net, KMAJOR, KMAJOR, EPI_F32, net->dzT, (int)P, net->col, (int)P, net->part, c.kpad, nullptr, c.cout, c.kpad,
          (int)P, splits);

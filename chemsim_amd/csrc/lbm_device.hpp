@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "cs_internal.hpp"
 
@@ -88,6 +89,8 @@ struct StF32 {
     using S = float;
     using T = float;
     static constexpr bool DEV = false;
+    __device__ static __forceinline__ T wid(S v) { return v; }
+    __device__ static __forceinline__ S nar(T v) { return v; }
     __device__ static __forceinline__ T ld(const S* p, int64_t o) { return p[o]; }
     __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(v, p + o); }
 };
@@ -95,6 +98,8 @@ struct StF64 {
     using S = double;
     using T = double;
     static constexpr bool DEV = false;
+    __device__ static __forceinline__ T wid(S v) { return v; }
+    __device__ static __forceinline__ S nar(T v) { return v; }
     __device__ static __forceinline__ T ld(const S* p, int64_t o) { return p[o]; }
     __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(v, p + o); }
 };
@@ -102,6 +107,8 @@ struct StF32Dev {
     using S = float;
     using T = float;
     static constexpr bool DEV = true;
+    __device__ static __forceinline__ T wid(S v) { return v; }
+    __device__ static __forceinline__ S nar(T v) { return v; }
     __device__ static __forceinline__ T ld(const S* p, int64_t o) { return p[o]; }
     __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(v, p + o); }
 };
@@ -109,6 +116,9 @@ struct StBF16 {
     using S = uint16_t;
     using T = float;
     static constexpr bool DEV = true;
+    __device__ static __forceinline__ T wid(S v) { return bf16_bits_to_f32(v); }
+    __device__ static __forceinline__ S nar(T v) { return f32_to_bf16_bits(v); }
+    __device__ static __forceinline__ S nar_sr(T v, uint32_t r) { return f32_to_bf16_sr_bits(v, r); }
     __device__ static __forceinline__ T ld(const S* p, int64_t o) { return bf16_bits_to_f32(p[o]); }
     __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(f32_to_bf16_bits(v), p + o); }
     __device__ static __forceinline__ void st_sr(S* p, int64_t o, T v, uint32_t r)
@@ -120,6 +130,9 @@ struct StF16 {
     using S = uint16_t;
     using T = float;
     static constexpr bool DEV = true;
+    __device__ static __forceinline__ T wid(S v) { return f16_bits_to_f32(v); }
+    __device__ static __forceinline__ S nar(T v) { return f32_to_f16_bits(v); }
+    __device__ static __forceinline__ S nar_sr(T v, uint32_t r) { return f32_to_f16_sr_bits(v, r); }
     __device__ static __forceinline__ T ld(const S* p, int64_t o) { return f16_bits_to_f32(p[o]); }
     __device__ static __forceinline__ void st(S* p, int64_t o, T v) { __builtin_nontemporal_store(f32_to_f16_bits(v), p + o); }
     __device__ static __forceinline__ void st_sr(S* p, int64_t o, T v, uint32_t r)
@@ -426,6 +439,154 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
         if (c < a.w)
             finish<St, MODE, SR, Off>(fin, fout, a.pitch, a.plane, a.k, r, c, type[j], B[j], a.sr_key,
                                       a.cell_base + (uint32_t)(r * a.w + c));
+    }
+}
+
+// ---------------------------------------------- vector step, fluid lattices --
+// Lattices without geometry: a lane owns V consecutive cells of a row, so
+// every population plane is read and written with one 8- or 16-byte access
+// per lane (non-temporal loads and stores: every element is used once).  The
+// populations that stream along the row (cy = +-1) come
+// from the lane's own vector shifted by one cell; the cell that crosses into
+// the neighbouring lane is exchanged with a lane shuffle, and the first and
+// last lane of a wave (and the last active lane of a row) read their outside
+// cell directly.  The per-cell arithmetic and the stored values are those of
+// gather/finish, so the results equal k_step bit for bit.
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+template <int VB>
+using RawVec = std::conditional_t<VB == 16, u32x4_t, std::conditional_t<VB == 8, u32x2_t, uint32_t>>;
+template <class S, int VB>
+union VecN {
+    RawVec<VB> raw;
+    S e[VB / sizeof(S)];
+};
+
+template <class S>
+__device__ __forceinline__ S shfl_elem(S v, int src_lane)
+{
+    if constexpr (sizeof(S) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, src_lane, 64);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), src_lane, 64);
+        return __builtin_bit_cast(S, ((uint64_t)hi << 32) | lo);
+    } else if constexpr (sizeof(S) == 4) {
+        return __builtin_bit_cast(S, __shfl(__builtin_bit_cast(int, v), src_lane, 64));
+    } else {
+        return (S)__shfl((int)v, src_lane, 64);
+    }
+}
+
+// One row per wave (ROWS rows per block), 64*V columns per wave, VB bytes
+// per lane and plane (V = VB / sizeof(S) cells).  w % V == 0.
+template <class St, int EDGE, bool SR, int VB = 16, int ROWS = 4>
+__global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T> a)
+{
+    using S = typename St::S;
+    using T = typename St::T;
+    constexpr int V = VB / (int)sizeof(S);
+    static_assert(V >= 2, "at least two cells per lane");
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = tile / a.tiles_x;
+    const int tx = tile - ty * a.tiles_x;
+    const int r = ty * ROWS + (threadIdx.x >> 6);
+    if (r >= a.h) return;  // uniform over the wave
+    const int lane = threadIdx.x & 63;
+    const int c0 = (tx * 64 + lane) * V;
+    const bool active = c0 < a.w;
+    const int cl = active ? c0 : a.w - V;  // in-range address for inactive lanes
+    const bool last = active && (lane == 63 || c0 + V >= a.w);
+    const S* __restrict__ fin = (const S*)a.fin;
+    S* __restrict__ fout = (S*)a.fout;
+    const uint32_t P = (uint32_t)a.plane;
+
+    int rows[3];
+    bool vr[3];
+    if (EDGE == CS_EDGE_PERIODIC) {
+        rows[0] = r == 0 ? a.h - 1 : r - 1;
+        rows[2] = r == a.h - 1 ? 0 : r + 1;
+        vr[0] = vr[2] = true;
+    } else {
+        vr[0] = r > 0;
+        vr[2] = r < a.h - 1;
+        rows[0] = vr[0] ? r - 1 : r;
+        rows[2] = vr[2] ? r + 1 : r;
+    }
+    rows[1] = r;
+    vr[1] = true;
+    // columns of the cells just outside this lane's vector
+    const int cleft = c0 - 1, cright = c0 + V;
+    const bool vleft = EDGE == CS_EDGE_PERIODIC || cleft >= 0;
+    const bool vright = EDGE == CS_EDGE_PERIODIC || cright < a.w;
+    const int cleft_w = cleft < 0 ? a.w - 1 : cleft;
+    const int cright_w = cright >= a.w ? 0 : cright;
+
+    T B[V][9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int rs = rows[CX[i] + 1];
+        const bool rvalid = vr[CX[i] + 1];
+        const uint32_t rowbase = (uint32_t)i * P + (uint32_t)rs * (uint32_t)a.pitch;
+        VecN<S, VB> v;
+        v.raw = __builtin_nontemporal_load((const RawVec<VB>*)at<uint32_t>(fin, rowbase + (uint32_t)cl));
+        const T outside = St::DEV ? -a.k.base(i) : T(0);
+        if (CY[i] == 0) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) B[j][i] = rvalid ? St::wid(v.e[j]) : outside;
+        } else if (CY[i] == 1) {
+            // source column c - 1: the previous cell, across the lane boundary for j = 0
+            S left = shfl_elem(v.e[V - 1], lane == 0 ? 0 : lane - 1);
+            if (lane == 0 && active) left = *at<uint32_t>(fin, rowbase + (uint32_t)cleft_w);
+            const bool lvalid = rvalid && (lane != 0 || vleft);
+            B[0][i] = lvalid ? St::wid(left) : outside;
+#pragma unroll
+            for (int j = 1; j < V; ++j) B[j][i] = rvalid ? St::wid(v.e[j - 1]) : outside;
+        } else {
+            // source column c + 1: the next cell, across the lane boundary for j = V - 1
+            S right = shfl_elem(v.e[0], lane == 63 ? 63 : lane + 1);
+            if (last) right = *at<uint32_t>(fin, rowbase + (uint32_t)cright_w);
+            const bool rtvalid = rvalid && (!last || vright);
+#pragma unroll
+            for (int j = 0; j < V - 1; ++j) B[j][i] = rvalid ? St::wid(v.e[j + 1]) : outside;
+            B[V - 1][i] = rtvalid ? St::wid(right) : outside;
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        if constexpr (St::DEV) bgk_collide_dev(a.k, B[j]);
+        else bgk_collide(a.k, B[j]);
+    }
+    const uint32_t own = (uint32_t)r * (uint32_t)a.pitch + (uint32_t)c0;
+    if constexpr (SR) {
+        // the bits of finish(): h = lowbias32(cell ^ key), two directions per word
+        S o[9][V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            uint32_t h = lowbias32((a.cell_base + (uint32_t)(r * a.w + c0 + j)) ^ a.sr_key);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                o[2 * q][j] = St::nar_sr(B[j][2 * q], h);
+                o[2 * q + 1][j] = St::nar_sr(B[j][2 * q + 1], h << 16);
+                h = xorshift32(h);
+            }
+            o[8][j] = St::nar_sr(B[j][8], h);
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            VecN<S, VB> v;
+#pragma unroll
+            for (int j = 0; j < V; ++j) v.e[j] = o[i][j];
+            __builtin_nontemporal_store(v.raw, (RawVec<VB>*)at<uint32_t>(fout, (uint32_t)i * P + own));
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            VecN<S, VB> v;
+#pragma unroll
+            for (int j = 0; j < V; ++j) v.e[j] = St::nar(B[j][i]);
+            __builtin_nontemporal_store(v.raw, (RawVec<VB>*)at<uint32_t>(fout, (uint32_t)i * P + own));
+        }
     }
 }
 

@@ -280,9 +280,10 @@ int cs_gemm_bf16(cs_ctx* ctx, int M, int N, int K, const uint16_t* A, const uint
  * the average milliseconds per copy (HIP events).  Calibrates bandwidth and
  * the HBM byte counters. */
 int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy);
-/* Launch shape of the step kernel: cells per thread (1, 2 or 4; 0 = the
- * built-in default).  Process-wide; used by the autotuner.  No reference
- * counterpart (ArrayFire picks its own launch shapes). */
+/* Launch shape of the step kernel: cells per thread of the scalar kernel (1,
+ * 2 or 4, which also disables the 16-byte vector kernel of lattices without
+ * geometry; 0 = the built-in default).  Process-wide; for tests and tuning.
+ * No reference counterpart (ArrayFire picks its own launch shapes). */
 int cs_set_cells_per_thread(int cells_per_thread);
 
 #ifdef __cplusplus
