@@ -71,10 +71,11 @@ def cpu_baseline(n, tau, u0, budget_s=12.0):
     return n * n * steps / dt / 1e9, used, steps
 
 
-def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20):
+def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
     """512 independent lid-driven cavities (sides U{64..512}, tau U[0.55,1],
-    lid U[0.02,0.1], seed 0), FP32, half-way walls: autotuned persistent launch
-    of `steps` steps vs one launch per lattice per step."""
+    lid U[0.02,0.1], seed 0), FP32, half-way walls: the autotuned batch launch
+    for `steps` steps, the other launch kind (persistent / tile list) for
+    `other_steps`, and one launch per lattice per step."""
     from chemsim_amd import Params, capi
     from chemsim_amd import workloads as W
     from chemsim_amd.batch import Batch, tune
@@ -88,10 +89,21 @@ def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20):
                            os.path.join(os.path.expanduser("~"), ".cache", "chemsim_amd", "batch_tuning.json"))
     rec = tune(ctx, b, cache, trial_steps=20)
     ms = b.step(steps)
+    best = b.tuning
+    other = dict(best)
+    if best.get("launch", 0) == capi.CS_LAUNCH_TILES:
+        other.update(launch=capi.CS_LAUNCH_PERSISTENT, block_threads=1024, workgroups_per_cu=2, cells_per_thread=1)
+    else:
+        other.update(launch=capi.CS_LAUNCH_TILES)
+    b.tuning = other
+    ms_other = b.step(other_steps)
+    b.tuning = best
     ms_pl = b.step_per_lattice(per_lattice_steps)
     cells = b.cells
+    kinds = {capi.CS_LAUNCH_PERSISTENT: "persistent", capi.CS_LAUNCH_TILES: "tiles"}
     out = {"lattices": len(b), "cells": cells, "steps": steps,
-           "GLUPS_persistent": round(cells * steps / ms / 1e6, 3),
+           "GLUPS_batch": round(cells * steps / ms / 1e6, 3), "launch": kinds[best.get("launch", 0)],
+           "GLUPS_" + kinds[other["launch"]]: round(cells * other_steps / ms_other / 1e6, 3),
            "GLUPS_per_lattice_launches": round(cells * per_lattice_steps / ms_pl / 1e6, 3),
            "seconds": round(ms / 1e3, 3), "tuning": rec["tuning"]}
     b.close()

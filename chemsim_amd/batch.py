@@ -96,16 +96,18 @@ class Batch:
         t = capi.cs_batch_tuning()
         call("cs_batch_get_tuning", self.handle, C.byref(t))
         return {"block_threads": t.block_threads, "workgroups_per_cu": t.workgroups_per_cu,
-                "cells_per_thread": t.cells_per_thread, "order": t.order}
+                "cells_per_thread": t.cells_per_thread, "order": t.order, "launch": t.launch}
 
     @tuning.setter
     def tuning(self, d: dict) -> None:
         t = capi.cs_batch_tuning(int(d["block_threads"]), int(d["workgroups_per_cu"]),
-                                 int(d["cells_per_thread"]), int(d["order"]))
+                                 int(d["cells_per_thread"]), int(d["order"]),
+                                 int(d.get("launch", capi.CS_LAUNCH_PERSISTENT)))
         call("cs_batch_set_tuning", self.handle, C.byref(t))
 
     def step(self, n: int) -> float:
-        """n steps of every lattice in one persistent launch; returns milliseconds."""
+        """n steps of every active lattice (persistent or tile-list launches,
+        by the tuning); returns milliseconds."""
         ms = C.c_float()
         call("cs_batch_step", self.handle, int(n), C.byref(ms))
         return ms.value

@@ -21,6 +21,7 @@ CS_BB_REFERENCE, CS_BB_HALFWAY = range(2)
 CS_FLUID, CS_WALL, CS_LID, CS_INLET, CS_OUTLET = range(5)
 CS_ROUND_NEAREST, CS_ROUND_STOCHASTIC = range(2)
 CS_ORDER_LARGEST_FIRST, CS_ORDER_GIVEN, CS_ORDER_SMALLEST_FIRST = range(3)
+CS_LAUNCH_PERSISTENT, CS_LAUNCH_TILES = range(2)
 
 
 class cs_params(C.Structure):
@@ -47,6 +48,7 @@ class cs_batch_tuning(C.Structure):
         ("workgroups_per_cu", C.c_int32),
         ("cells_per_thread", C.c_int32),
         ("order", C.c_int32),
+        ("launch", C.c_int32),
     ]
 
 

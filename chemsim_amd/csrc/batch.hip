@@ -1,12 +1,17 @@
-// Batch of independent lattices stepped by one persistent launch.
+// Batch of independent lattices stepped together.
 //
 // The reference steps one lattice per State::step (lbm.rs:694-714).  A batch
-// holds many lattices of uneven sizes in one device arena; k_batch is a
-// persistent grid whose workgroups take whole lattices from a device-side
-// queue (largest first by default) and run every step on them, with a
-// workgroup barrier between steps.  No inter-workgroup synchronisation is
-// needed, and the per-cell code is the same gather/finish as the single-
-// lattice step, so the results equal per-lattice launches bit for bit.
+// holds many lattices of uneven sizes in one device arena, with two launch
+// kinds chosen by the tuning:
+//  - persistent (k_batch): a grid whose workgroups take whole lattices from a
+//    device-side queue (largest first by default) and run every step of the
+//    call on them, with a workgroup barrier between steps -- no
+//    inter-workgroup synchronisation;
+//  - tiles (k_batch_tiles): one launch per step over the flattened list of
+//    4 x 64 tiles of every lattice, so a large lattice is spread over the
+//    whole GPU.
+// The per-cell code is the single-lattice gather/finish either way, so the
+// results equal per-lattice launches bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +40,22 @@ struct BatchLat {
 };
 
 enum : int { BM_PERIODIC = 1, BM_HALFWAY = 2, BM_FLAGS = 4, BM_SR = 8 };
+
+// Per-lattice step state: which buffer holds the current populations and how
+// many steps the lattice has taken (the stochastic-rounding key).  Lattices
+// taken out of the batch keep theirs.
+struct LatState {
+    int cur;
+    int pad;
+    uint64_t steps;
+};
+
+// One entry of the flattened tile list: lattice index and tile index within
+// it (tiles of 64 columns x 4 rows, row-major).
+struct TileRef {
+    int lat;
+    int tile;
+};
 
 // One step of one lattice by one workgroup: wave `wave` of `nwaves` takes
 // rows wave, wave + nwaves, ...; lane l the columns l + 64 j.
@@ -72,7 +93,7 @@ template <class St, int EDGE, int MODE, bool SR, int CPT>
 __global__ __launch_bounds__(1024) void k_batch(const BatchLat<typename St::T>* __restrict__ lats,
                                                 const int* __restrict__ order, int n, int* __restrict__ queue,
                                                 void* __restrict__ arena, const uint32_t* __restrict__ nodes,
-                                                int steps, int cur0, uint64_t step0)
+                                                const LatState* __restrict__ state, int steps)
 {
     using S = typename St::S;
     __shared__ int s_q;
@@ -87,6 +108,8 @@ __global__ __launch_bounds__(1024) void k_batch(const BatchLat<typename St::T>* 
         const BatchLat<typename St::T> L = lats[li];
         S* base = (S*)arena + L.off;
         const uint32_t* node = nodes + L.node_off;
+        const int cur0 = state[li].cur;
+        const uint64_t step0 = state[li].steps;
         for (int s = 0; s < steps; ++s) {
             const int cur = (cur0 + s) & 1;
             const S* fin = base + (int64_t)cur * 9 * L.plane;
@@ -101,14 +124,15 @@ __global__ __launch_bounds__(1024) void k_batch(const BatchLat<typename St::T>* 
 // Per-lattice velocity snapshot (one block per lattice).
 template <class St>
 __global__ __launch_bounds__(256) void k_batch_snapshot(const BatchLat<typename St::T>* __restrict__ lats, int n,
-                                                        const void* __restrict__ arena, int cur, float* __restrict__ snap,
+                                                        const void* __restrict__ arena,
+                                                        const LatState* __restrict__ state, float* __restrict__ snap,
                                                         const int64_t* __restrict__ snap_off)
 {
     using T = typename St::T;
     const int li = blockIdx.x;
     if (li >= n) return;
     const BatchLat<T> L = lats[li];
-    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)cur * 9 * L.plane;
+    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)state[li].cur * 9 * L.plane;
     const int64_t cells = (int64_t)L.w * L.h;
     float* s = snap + snap_off[li];
     for (int64_t d = threadIdx.x; d < cells; d += 256) {
@@ -126,7 +150,8 @@ __global__ __launch_bounds__(256) void k_batch_snapshot(const BatchLat<typename 
 // Per-lattice residual ||u - u_snap|| / ||u|| over non-solid cells, f64.
 template <class St>
 __global__ __launch_bounds__(256) void k_batch_residual(const BatchLat<typename St::T>* __restrict__ lats, int n,
-                                                        const void* __restrict__ arena, int cur,
+                                                        const void* __restrict__ arena,
+                                                        const LatState* __restrict__ state,
                                                         const uint32_t* __restrict__ nodes,
                                                         const float* __restrict__ snap,
                                                         const int64_t* __restrict__ snap_off, double* __restrict__ out)
@@ -136,7 +161,7 @@ __global__ __launch_bounds__(256) void k_batch_residual(const BatchLat<typename 
     const int li = blockIdx.x;
     if (li >= n) return;
     const BatchLat<T> L = lats[li];
-    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)cur * 9 * L.plane;
+    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)state[li].cur * 9 * L.plane;
     const int64_t cells = (int64_t)L.w * L.h;
     const float* s = snap + snap_off[li];
     double num = 0.0, den = 0.0;
@@ -166,6 +191,38 @@ __global__ __launch_bounds__(256) void k_batch_residual(const BatchLat<typename 
         __syncthreads();
     }
     if (threadIdx.x == 0) out[li] = sh[1][0] > 0 ? sqrt(sh[0][0] / sh[1][0]) : (sh[0][0] > 0 ? INFINITY : 0.0);
+}
+
+// One step of every lattice in a tile list: block b updates tile tiles[b]
+// (4 rows x 64 columns, one cell per thread) of its lattice with the
+// single-lattice gather/finish.  `s` is the step's index within the call.
+template <class St, int EDGE, int MODE, bool SR>
+__global__ __launch_bounds__(256) void k_batch_tiles(const BatchLat<typename St::T>* __restrict__ lats,
+                                                     const TileRef* __restrict__ tiles, void* __restrict__ arena,
+                                                     const uint32_t* __restrict__ nodes,
+                                                     const LatState* __restrict__ state, int s)
+{
+    using S = typename St::S;
+    using T = typename St::T;
+    const TileRef tr = tiles[blockIdx.x];
+    const int li = __builtin_amdgcn_readfirstlane(tr.lat);
+    const int t = __builtin_amdgcn_readfirstlane(tr.tile);
+    const BatchLat<T> L = lats[li];
+    const int tiles_x = (L.w + 63) >> 6;
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int r = ty * 4 + (int)(threadIdx.x >> 6);
+    if (r >= L.h) return;
+    const int c = tx * 64 + (int)(threadIdx.x & 63);
+    const int cur = (state[li].cur + s) & 1;
+    const uint32_t key = sr_key(L.seed, state[li].steps + (uint64_t)s);
+    const S* fin = (const S*)arena + L.off + (int64_t)cur * 9 * L.plane;
+    S* fout = (S*)arena + L.off + (int64_t)(cur ^ 1) * 9 * L.plane;
+    const uint32_t* node = nodes + L.node_off;
+    T B[9];
+    const int cc = c < L.w ? c : L.w - 1;
+    const int type = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.plane, L.k, r, cc, B);
+    if (c < L.w)
+        finish<St, MODE, SR, uint32_t>(fin, fout, L.pitch, L.plane, L.k, r, c, type, B, key, (uint32_t)(r * L.w + c));
 }
 
 // Mode group of a lattice: periodic, mode (fluid/ref/half), stochastic rounding.
@@ -198,10 +255,16 @@ struct cs_batch {
     void* d_lats = nullptr;
     int* d_order = nullptr;
     int* d_queue = nullptr;
+    TileRef* d_tiles = nullptr;
+    int64_t max_tiles = 0;
+    LatState* d_state = nullptr;
     bool lats_dirty = true;
-    int cur = 0;
-    uint64_t steps = 0;
-    cs_batch_tuning tuning{256, 2, 1, CS_ORDER_LARGEST_FIRST};
+    std::vector<LatState> state;      // per lattice: current buffer, steps taken
+    // host copies behind asynchronous uploads (kept alive until the next upload,
+    // which follows the stream's earlier work)
+    std::vector<int> host_order;
+    std::vector<TileRef> host_tiles;
+    cs_batch_tuning tuning{256, 2, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_PERSISTENT};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -299,8 +362,10 @@ static std::vector<std::array<int, 3>> upload_order(cs_batch* b)
         if (groups.empty() || groups.back()[0] != g) groups.push_back({g, k, 0});
         groups.back()[2] += 1;
     }
-    if (!ord.empty())
-        CS_HIP(hipMemcpyAsync(b->d_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, b->stream));
+    b->host_order = ord;
+    if (!b->host_order.empty())
+        CS_HIP(hipMemcpyAsync(b->d_order, b->host_order.data(), sizeof(int) * b->host_order.size(),
+                              hipMemcpyHostToDevice, b->stream));
     return groups;
 }
 
@@ -313,7 +378,74 @@ static void launch_group(cs_batch* b, int group, int first, int count, int steps
     const dim3 block((unsigned)b->tuning.block_threads);
     auto go = [&]<int EDGE, int MODE, bool SR>() {
         hipLaunchKernelGGL((k_batch<St, EDGE, MODE, SR, CPT>), dim3((unsigned)grid), block, 0, b->stream, lats, order,
-                           count, queue, b->arena, b->nodes, steps, b->cur, b->steps);
+                           count, queue, b->arena, b->nodes, b->d_state, steps);
+    };
+    auto go_sr = [&]<int EDGE, int MODE>() {
+        if constexpr (is_16bit<St>()) {
+            if (group & 1) {
+                go.template operator()<EDGE, MODE, true>();
+                return;
+            }
+        }
+        go.template operator()<EDGE, MODE, false>();
+    };
+    auto go_mode = [&]<int EDGE>() {
+        const int m = (group % 6) / 2;
+        if (m == MODE_FLUID) go_sr.template operator()<EDGE, MODE_FLUID>();
+        else if (m == MODE_HALF) go_sr.template operator()<EDGE, MODE_HALF>();
+        else go_sr.template operator()<EDGE, MODE_REF>();
+    };
+    if (group >= 6) go_mode.template operator()<CS_EDGE_PERIODIC>();
+    else go_mode.template operator()<CS_EDGE_ZERO>();
+}
+
+static void upload_state(cs_batch* b)
+{
+    CS_HIP(hipMemcpyAsync(b->d_state, b->state.data(), sizeof(LatState) * (size_t)b->n, hipMemcpyHostToDevice,
+                          b->stream));
+}
+
+static void advance_state(cs_batch* b, int steps)
+{
+    for (int i = 0; i < b->n; ++i) {
+        if (!b->active[(size_t)i]) continue;
+        b->state[(size_t)i].cur = (b->state[(size_t)i].cur + steps) & 1;
+        b->state[(size_t)i].steps += (uint64_t)steps;
+    }
+}
+
+// Tile list of the active lattices, grouped like upload_order; returns
+// (group, first tile, tile count) triples into d_tiles.
+static std::vector<std::array<int, 3>> upload_tiles(cs_batch* b, const std::vector<std::array<int, 3>>& groups)
+{
+    std::vector<TileRef>& tiles = b->host_tiles;
+    tiles.clear();
+    std::vector<std::array<int, 3>> out;
+    for (const auto& g : groups) {
+        const int first = (int)tiles.size();
+        for (int k = g[1]; k < g[1] + g[2]; ++k) {
+            const int li = b->host_order[(size_t)k];
+            const int nt = ((b->w[(size_t)li] + 63) / 64) * ((b->h[(size_t)li] + 3) / 4);
+            for (int t = 0; t < nt; ++t) tiles.push_back(TileRef{li, t});
+        }
+        out.push_back({g[0], first, (int)tiles.size() - first});
+    }
+    CS_REQUIRE((int64_t)tiles.size() <= b->max_tiles, CS_ERR_STATE, "tile list larger than allocated");
+    if (!tiles.empty())
+        CS_HIP(hipMemcpyAsync(b->d_tiles, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice,
+                              b->stream));
+    return out;
+}
+
+template <class St>
+static void launch_tiles_group(cs_batch* b, int group, int first, int count, int s)
+{
+    using T = typename St::T;
+    const auto* lats = (const BatchLat<T>*)b->d_lats;
+    const TileRef* tiles = b->d_tiles + first;
+    auto go = [&]<int EDGE, int MODE, bool SR>() {
+        hipLaunchKernelGGL((k_batch_tiles<St, EDGE, MODE, SR>), dim3((unsigned)count), dim3(256), 0, b->stream, lats,
+                           tiles, b->arena, b->nodes, b->d_state, s);
     };
     auto go_sr = [&]<int EDGE, int MODE>() {
         if constexpr (is_16bit<St>()) {
@@ -337,20 +469,30 @@ static void launch_group(cs_batch* b, int group, int first, int count, int steps
 static void launch_batch(cs_batch* b, int steps)
 {
     upload_descriptors(b);
+    upload_state(b);
     const auto groups = upload_order(b);
     CS_REQUIRE(groups.size() <= 16, CS_ERR_STATE, "too many boundary-mode groups");
-    CS_HIP(hipMemsetAsync(b->d_queue, 0, sizeof(int) * 16, b->stream));
-    for (size_t gi = 0; gi < groups.size(); ++gi) {
-        const auto& g = groups[gi];
-        const int grid = std::max(1, std::min(g[2], b->ctx->prop.multiProcessorCount * b->tuning.workgroups_per_cu));
-        dispatch_batch_storage(b->storage, [&]<class St>() {
-            if (b->tuning.cells_per_thread >= 2) launch_group<St, 2>(b, g[0], g[1], g[2], steps, b->d_queue + gi, grid);
-            else launch_group<St, 1>(b, g[0], g[1], g[2], steps, b->d_queue + gi, grid);
-        });
+    if (b->tuning.launch == CS_LAUNCH_TILES) {
+        // one launch per step and mode group over the flattened tile list
+        const auto tg = upload_tiles(b, groups);
+        for (int s = 0; s < steps; ++s)
+            for (const auto& g : tg)
+                if (g[2] > 0)
+                    dispatch_batch_storage(b->storage,
+                                           [&]<class St>() { launch_tiles_group<St>(b, g[0], g[1], g[2], s); });
+    } else {
+        CS_HIP(hipMemsetAsync(b->d_queue, 0, sizeof(int) * 16, b->stream));
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            const auto& g = groups[gi];
+            const int grid = std::max(1, std::min(g[2], b->ctx->prop.multiProcessorCount * b->tuning.workgroups_per_cu));
+            dispatch_batch_storage(b->storage, [&]<class St>() {
+                if (b->tuning.cells_per_thread >= 2) launch_group<St, 2>(b, g[0], g[1], g[2], steps, b->d_queue + gi, grid);
+                else launch_group<St, 1>(b, g[0], g[1], g[2], steps, b->d_queue + gi, grid);
+            });
+        }
     }
     CS_HIP(hipGetLastError());
-    b->cur = (b->cur + steps) & 1;
-    b->steps += (uint64_t)steps;
+    advance_state(b, steps);
 }
 
 // One k_step launch per lattice per step (single-lattice code path).
@@ -361,9 +503,9 @@ static void launch_per_lattice(cs_batch* b, int steps)
         std::vector<BatchLat<T>> L((size_t)b->n);
         for (int i = 0; i < b->n; ++i) L[(size_t)i] = describe<T>(b, i);
         for (int s = 0; s < steps; ++s) {
-            const int cur = (b->cur + s) & 1;
             for (int i = 0; i < b->n; ++i) {
                 if (!b->active[(size_t)i]) continue;
+                const int cur = (b->state[(size_t)i].cur + s) & 1;
                 const BatchLat<T>& d = L[(size_t)i];
                 StepArgs<T> a;
                 typename St::S* base = (typename St::S*)b->arena + d.off;
@@ -375,7 +517,7 @@ static void launch_per_lattice(cs_batch* b, int steps)
                 a.pitch = d.pitch;
                 a.plane = d.plane;
                 a.tiles_x = (d.w + 63) / 64;
-                a.sr_key = sr_key(d.seed, b->steps + (uint64_t)s);
+                a.sr_key = sr_key(d.seed, b->state[(size_t)i].steps + (uint64_t)s);
                 a.cell_base = 0;
                 a.k = d.k;
                 const int nblocks = a.tiles_x * ((d.h + 3) / 4);
@@ -406,11 +548,14 @@ static void launch_per_lattice(cs_batch* b, int steps)
         }
     });
     CS_HIP(hipGetLastError());
-    b->cur = (b->cur + steps) & 1;
-    b->steps += (uint64_t)steps;
+    advance_state(b, steps);
 }
 
-static void* lattice_buf(cs_batch* b, int i) { return (char*)b->arena + (size_t)b->off[(size_t)i] * b->esize + (size_t)b->cur * 9 * b->plane[(size_t)i] * b->esize; }
+static void* lattice_buf(cs_batch* b, int i)
+{
+    return (char*)b->arena + (size_t)b->off[(size_t)i] * b->esize +
+           (size_t)b->state[(size_t)i].cur * 9 * b->plane[(size_t)i] * b->esize;
+}
 
 }  // namespace cs
 
@@ -445,6 +590,8 @@ extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const i
         }
         b->arena_elems = acc;
         b->node_elems = nacc;
+        for (int i = 0; i < count; ++i) b->max_tiles += (int64_t)((w[i] + 63) / 64) * ((h[i] + 3) / 4);
+        b->state.assign((size_t)count, LatState{0, 0, 0});
         cs_params p{};
         p.tau = 1.0;
         p.delta_x = 1.0;
@@ -465,6 +612,8 @@ extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const i
             CS_HIP(hipMemset(b->nodes, 0, sizeof(uint32_t) * (size_t)nacc));
             CS_HIP(hipMalloc(&b->d_order, sizeof(int) * (size_t)count));
             CS_HIP(hipMalloc(&b->d_queue, sizeof(int) * 16));
+            CS_HIP(hipMalloc(&b->d_tiles, sizeof(TileRef) * (size_t)b->max_tiles));
+            CS_HIP(hipMalloc(&b->d_state, sizeof(LatState) * (size_t)count));
             CS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
             CS_HIP(hipEventCreate(&b->ev0));
             CS_HIP(hipEventCreate(&b->ev1));
@@ -482,7 +631,7 @@ extern "C" int cs_batch_destroy(cs_batch* b)
         if (!b) return;
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         for (void* p : {b->arena, (void*)b->nodes, b->d_lats, (void*)b->d_order, (void*)b->d_queue, (void*)b->snap,
-                        (void*)b->d_res})
+                        (void*)b->d_res, (void*)b->d_tiles, (void*)b->d_state})
             if (p) (void)hipFree(p);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -683,6 +832,7 @@ extern "C" int cs_batch_set_tuning(cs_batch* b, const cs_batch_tuning* t)
                    "workgroups_per_cu must be 1..16");
         CS_REQUIRE(t->cells_per_thread == 1 || t->cells_per_thread == 2, CS_ERR_ARG, "cells_per_thread must be 1 or 2");
         CS_REQUIRE(t->order >= CS_ORDER_LARGEST_FIRST && t->order <= CS_ORDER_SMALLEST_FIRST, CS_ERR_ARG, "bad order");
+        CS_REQUIRE(t->launch == CS_LAUNCH_PERSISTENT || t->launch == CS_LAUNCH_TILES, CS_ERR_ARG, "bad launch kind");
         b->tuning = *t;
     });
 }
@@ -733,26 +883,29 @@ extern "C" int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* 
         const size_t bytes = (size_t)b->arena_elems * b->esize;
         CS_HIP(hipMalloc(&backup, bytes));
         CS_HIP(hipMemcpyAsync(backup, b->arena, bytes, hipMemcpyDeviceToDevice, b->stream));
-        const int cur0 = b->cur;
-        const uint64_t steps0 = b->steps;
+        const std::vector<LatState> state0 = b->state;
         const cs_batch_tuning keep = b->tuning;
         cs_batch_tuning win = keep;
         float win_ms = INFINITY;
+        auto trial = [&](const cs_batch_tuning& t) {
+            b->tuning = t;
+            float ms = 0;
+            timed(b, &ms, [&] { launch_batch(b, trial_steps); });
+            if (ms < win_ms) {
+                win_ms = ms;
+                win = b->tuning;
+            }
+        };
         try {
+            trial(cs_batch_tuning{256, 1, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_TILES});
             for (int blk : {256, 512, 1024})
                 for (int wpc : {1, 2, 4, 8})
                     for (int cpt : {1, 2})
-                        for (int ord : {CS_ORDER_LARGEST_FIRST, CS_ORDER_GIVEN}) {
-                            b->tuning = cs_batch_tuning{blk, wpc, cpt, ord};
-                            float t = 0;
-                            timed(b, &t, [&] { launch_batch(b, trial_steps); });
-                            if (t < win_ms) {
-                                win_ms = t;
-                                win = b->tuning;
-                            }
-                        }
+                        for (int ord : {CS_ORDER_LARGEST_FIRST, CS_ORDER_GIVEN})
+                            trial(cs_batch_tuning{blk, wpc, cpt, ord, CS_LAUNCH_PERSISTENT});
         } catch (...) {
             b->tuning = keep;
+            b->state = state0;
             (void)hipMemcpyAsync(b->arena, backup, bytes, hipMemcpyDeviceToDevice, b->stream);
             (void)hipStreamSynchronize(b->stream);
             (void)hipFree(backup);
@@ -763,8 +916,7 @@ extern "C" int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* 
         (void)hipFree(backup);
         CS_HIP(e1);
         CS_HIP(e2);
-        b->cur = cur0;
-        b->steps = steps0;
+        b->state = state0;
         b->tuning = win;
         if (best) *best = win;
         if (best_ms) *best_ms = win_ms;
@@ -790,10 +942,11 @@ extern "C" int cs_batch_snapshot_velocity(cs_batch* b)
                              sizeof(int64_t) * (size_t)b->n, hipMemcpyHostToDevice));
         }
         upload_descriptors(b);
+        upload_state(b);
         const int64_t* soff = (const int64_t*)((char*)b->d_res + sizeof(double) * (size_t)b->n);
         dispatch_batch_storage(b->storage, [&]<class St>() {
             hipLaunchKernelGGL((k_batch_snapshot<St>), dim3((unsigned)b->n), dim3(256), 0, b->stream,
-                               (const BatchLat<typename St::T>*)b->d_lats, b->n, b->arena, b->cur, b->snap, soff);
+                               (const BatchLat<typename St::T>*)b->d_lats, b->n, b->arena, b->d_state, b->snap, soff);
         });
         CS_HIP(hipGetLastError());
         CS_HIP(hipStreamSynchronize(b->stream));
@@ -806,11 +959,12 @@ extern "C" int cs_batch_residuals(cs_batch* b, double* residuals)
         CS_REQUIRE(b && residuals, CS_ERR_ARG, "null argument");
         CS_REQUIRE(b->snap != nullptr, CS_ERR_STATE, "residuals requested before a velocity snapshot");
         upload_descriptors(b);
+        upload_state(b);
         const int64_t* soff = (const int64_t*)((char*)b->d_res + sizeof(double) * (size_t)b->n);
         dispatch_batch_storage(b->storage, [&]<class St>() {
             hipLaunchKernelGGL((k_batch_residual<St>), dim3((unsigned)b->n), dim3(256), 0, b->stream,
-                               (const BatchLat<typename St::T>*)b->d_lats, b->n, b->arena, b->cur, b->nodes, b->snap,
-                               soff, b->d_res);
+                               (const BatchLat<typename St::T>*)b->d_lats, b->n, b->arena, b->d_state, b->nodes,
+                               b->snap, soff, b->d_res);
         });
         CS_HIP(hipGetLastError());
         CS_HIP(hipMemcpyAsync(residuals, b->d_res, sizeof(double) * (size_t)b->n, hipMemcpyDeviceToHost, b->stream));

@@ -207,11 +207,17 @@ int cs_lattice_run_to_residual(cs_lattice* lat, double tol, int interval, int64_
  * cs_lattice_step launch per lattice per step, bit for bit.  Each lattice has
  * its own cs_params and flags; all share one storage kind. */
 enum cs_batch_order { CS_ORDER_LARGEST_FIRST = 0, CS_ORDER_GIVEN = 1, CS_ORDER_SMALLEST_FIRST = 2 };
+/* CS_LAUNCH_PERSISTENT: workgroups take whole lattices from a queue and run
+ * all steps of a call on them (block_threads, workgroups_per_cu,
+ * cells_per_thread, order apply).  CS_LAUNCH_TILES: one launch per step over
+ * the 4 x 64 tiles of every lattice. */
+enum cs_batch_launch { CS_LAUNCH_PERSISTENT = 0, CS_LAUNCH_TILES = 1 };
 typedef struct cs_batch_tuning {
     int32_t block_threads;     /* 256, 512 or 1024 */
     int32_t workgroups_per_cu; /* persistent grid = CUs x this (capped at the lattice count) */
     int32_t cells_per_thread;  /* 1 or 2 */
     int32_t order;             /* enum cs_batch_order: queue order */
+    int32_t launch;            /* enum cs_batch_launch */
 } cs_batch_tuning;
 
 int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const int32_t* h, int storage, cs_batch** out);
@@ -226,8 +232,11 @@ int cs_batch_download_raw(cs_batch* b, int index, void* data);
 int cs_batch_velocity(cs_batch* b, int index, float* ux, float* uy);
 int cs_batch_set_tuning(cs_batch* b, const cs_batch_tuning* tuning);
 int cs_batch_get_tuning(const cs_batch* b, cs_batch_tuning* tuning);
-/* `steps` steps of every lattice in one persistent launch; synchronises and
- * returns the elapsed milliseconds (HIP events) if ms is not null. */
+/* `steps` steps of every active lattice (persistent launch or one tile-list
+ * launch per step, by the tuning); synchronises and returns the elapsed
+ * milliseconds (HIP events) if ms is not null.  Each lattice keeps its own
+ * step count and buffer parity, so lattices taken out with
+ * cs_batch_set_active keep their state. */
 int cs_batch_step(cs_batch* b, int steps, float* ms);
 /* The same steps as one fused launch per lattice per step (the comparison
  * path of the batch). */

@@ -68,11 +68,12 @@ def test_batch_matches_oracle(ctx, storage):
 def test_batch_equals_per_lattice_launches(ctx, storage):
     tunings = [dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0),
                dict(block_threads=1024, workgroups_per_cu=2, cells_per_thread=2, order=1),
+               dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=capi.CS_LAUNCH_TILES),
                dict(block_threads=512, workgroups_per_cu=8, cells_per_thread=2, order=2)]
     a, _ = build_mixed(ctx, storage)
     p, _ = build_mixed(ctx, storage)
     with a, p:
-        p.step_per_lattice(9)
+        p.step_per_lattice(12)
         for t in tunings:
             a.tuning = t
             a.step(3)
@@ -119,3 +120,31 @@ def test_cavity_batch_small(ctx):
             op = O.params(s.side, s.side, tau=s.tau, boundary=O.BB_HALFWAY, lid=W.lid_velocity(s.u_lid))
             ref = O.run(op, O.equilibrium(op, *W.rest_state(s.side, s.side)), 50, W.cavity_flags(s.side, s.side))
             np.testing.assert_array_equal(b.download(i), ref)
+
+
+def test_inactive_lattices_keep_their_state(ctx):
+    """A lattice taken out of the batch for an odd number of steps keeps its
+    populations and step count; stepped again, it continues where it stopped
+    (stochastic rounding included)."""
+    for launch in (capi.CS_LAUNCH_PERSISTENT, capi.CS_LAUNCH_TILES):
+        a, specs = build_mixed(ctx, "f16")
+        with a:
+            t = a.tuning
+            t["launch"] = launch
+            a.tuning = t
+            a.step(2)
+            a.set_active(3, False)
+            frozen = a.download_raw(3)
+            a.step(5)
+            np.testing.assert_array_equal(a.download_raw(3), frozen)
+            a.set_active(3, True)
+            a.step(4)
+            w, h, op, flags, init = specs[3]
+            d0 = O.to_f16(O.equilibrium_dev(op, *init))
+            ref = O.run_dev16(op, O.F16, d0, 6, flags, rounding=1, seed=77 + 3)
+            np.testing.assert_array_equal(a.download_raw(3), ref)
+            for i in (0, 5):
+                w, h, op, flags, init = specs[i]
+                d0 = O.to_f16(O.equilibrium_dev(op, *init))
+                np.testing.assert_array_equal(a.download_raw(i), O.run_dev16(op, O.F16, d0, 11, flags, rounding=1,
+                                                                             seed=77 + i))
