@@ -209,6 +209,12 @@ def main():
         extras["batch_512_cavities"] = batch_cavities_run(ctx)
         log(f"[bench] batch: {extras['batch_512_cavities']}")
 
+    if not args.no_extras:
+        # warm-start network: one training step of 8 samples of 128 x 512
+        from chemsim_amd.warmstart_run import time_step
+        extras["cnn_train_step"] = time_step(ctx, iters=10)
+        log(f"[bench] cnn: {extras['cnn_train_step']}")
+
     cpu = None
     if not args.no_cpu:
         cg, threads, csteps = cpu_baseline(n, args.tau, args.u0)

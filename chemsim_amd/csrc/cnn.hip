@@ -296,6 +296,7 @@ inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 
 
 struct Conv {
     int cin = 0, cout = 0, kpad = 0, kpad_d = 0;
+    int dil = 1;  // tap spacing
     bool relu = true;
     float *w = nullptr, *b = nullptr, *gw = nullptr, *gb = nullptr;
     float *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;
@@ -358,15 +359,18 @@ T* dalloc(cs_cnn* net, int64_t n)
 }
 
 void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, const uint16_t* B, int ldb, void* C, int ldc,
-          const float* bias, int M, int N, int K, int splits = 1, const Act* gather = nullptr, int sign = 1)
+          const float* bias, int M, int N, int K, int splits = 1, const Act* gather = nullptr, int sign = 1,
+          int dil = 1)
 {
     GemmArgs g{};
     if (la == GATHER) {
-        CS_REQUIRE(gather && gather->C % 8 == 0 && (int64_t)M == gather->P(), CS_ERR_ARG, "bad gather operand");
+        CS_REQUIRE(gather && gather->C % 8 == 0 && (int64_t)M == gather->P() && dil >= 1, CS_ERR_ARG,
+                   "bad gather operand");
         g.gH = gather->H;
         g.gW = gather->W;
         g.gC = gather->C;
         g.gsign = sign;
+        g.gdil = dil;
     }
     g.A = A;
     g.B = B;
@@ -401,21 +405,23 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
 }
 
 // Transposed gather for the weight gradient: col[k][p] (k < kpad, p < P).
-void im2col_t(cs_cnn* net, const Act& x, int kpad)
+void im2col_t(cs_cnn* net, const Act& x, int kpad, int dil)
 {
     const int64_t P = x.P();
     CS_REQUIRE(P * kpad <= net->col_elems && P % 8 == 0, CS_ERR_STATE, "im2col scratch too small");
     hipLaunchKernelGGL(k_im2col_t, dim3((unsigned)((P + 63) / 64), (unsigned)(kpad / 32)), dim3(256), 0, net->stream,
-                       x.d, net->col, x.N, x.H, x.W, x.C, kpad);
+                       x.d, net->col, x.N, x.H, x.W, x.C, kpad, dil);
     CS_HIP(hipGetLastError());
 }
 
 void conv_forward(cs_cnn* net, int li, const Act& x, const Act& y, float* y32 = nullptr)
 {
     Conv& c = net->convs[(size_t)li];
-    if (y32) gemm(net, GATHER, KMAJOR, EPI_F32, x.d, 0, c.wb, c.kpad, y32, 8, c.b, (int)x.P(), c.cout, c.kpad, 1, &x, 1);
-    else gemm(net, GATHER, KMAJOR, c.relu ? EPI_BF16_RELU : EPI_BF16, x.d, 0, c.wb, c.kpad, y.d, c.cout, c.b,
-              (int)x.P(), c.cout, c.kpad, 1, &x, 1);
+    if (y32)
+        gemm(net, GATHER, KMAJOR, EPI_F32, x.d, 0, c.wb, c.kpad, y32, 8, c.b, (int)x.P(), c.cout, c.kpad, 1, &x, 1, c.dil);
+    else
+        gemm(net, GATHER, KMAJOR, c.relu ? EPI_BF16_RELU : EPI_BF16, x.d, 0, c.wb, c.kpad, y.d, c.cout, c.b,
+             (int)x.P(), c.cout, c.kpad, 1, &x, 1, c.dil);
 }
 
 // Backward of conv li given the f32 gradient dy of its output (post-ReLU).
@@ -443,7 +449,7 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
     hipLaunchKernelGGL(k_transpose16, dim3((unsigned)((P + 63) / 64), (unsigned)((c.cout + 63) / 64)), dim3(256), 0,
                        net->stream, net->dz, net->dzT, P, c.cout);
     CS_HIP(hipGetLastError());
-    im2col_t(net, x, c.kpad);
+    im2col_t(net, x, c.kpad, c.dil);
     // enough K ranges for ~1024 workgroups, each at least 1024 pixels long
     int bm, bn;
     tile_dims(pick_tile(c.cout, c.kpad), bm, bn);
@@ -459,7 +465,7 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
     // data gradient: dx[p][ci] = sum_k colT(dz)[p][k] wd[ci][k]
     Act dzact{net->dz, x.N, x.H, x.W, c.cout};
     gemm(net, GATHER, KMAJOR, EPI_F32, net->dz, 0, c.wd, c.kpad_d, dx32, c.cin, nullptr, (int)P, c.cin, c.kpad_d, 1,
-         &dzact, -1);
+         &dzact, -1, c.dil);
 }
 
 void forward(cs_cnn* net, int n)
@@ -688,9 +694,14 @@ extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int b
                                      {2 * c, 4 * c, 1},  {4 * c, 4 * c, 1},  {4 * c, 8 * c, 1}, {8 * c, 8 * c, 1},
                                      {12 * c, 4 * c, 1}, {4 * c, 4 * c, 1},  {6 * c, 2 * c, 1}, {2 * c, 2 * c, 1},
                                      {3 * c, c, 1},      {c, c, 1},          {c, 8, 0}};
+            // tap spacing: the last encoder level and the bottleneck reach
+            // across the length of the channel
+            const int dil[15] = {1, 1, 1, 1, 1, 2, 4, 8, 2, 1, 1, 1, 1, 1, 1};
             std::mt19937 rng(seed);
-            for (auto& sp : spec) {
+            for (int li = 0; li < 15; ++li) {
+                const int* sp = spec[li];
                 Conv cv;
+                cv.dil = dil[li];
                 cv.cin = sp[0];
                 cv.cout = sp[1];
                 cv.relu = sp[2] != 0;

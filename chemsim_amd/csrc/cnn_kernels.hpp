@@ -33,11 +33,10 @@ __device__ __forceinline__ uint16_t bf16_rne(float x)
 __device__ __forceinline__ float bf16_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 // Operand layouts.  KMAJOR: element (r, k) at base[r*ld + k] (k contiguous).
-// RMAJOR: element (r, k) at base[k*ld + r] (the row index contiguous).
-// GATHER: the A operand of a 3x3 convolution read straight from the NHWC
-// input (implicit im2col): row r = pixel, k = tap*C + c, zero outside the
-// image and for k >= 9C.
-enum Layout : int { KMAJOR = 0, RMAJOR = 1, GATHER = 2 };
+// GATHER: the A operand of a 3x3 convolution (taps `dil` cells apart) read
+// straight from the NHWC input (implicit im2col): row r = pixel,
+// k = tap*C + c, zero outside the image and for k >= 9C.
+enum Layout : int { KMAJOR = 0, GATHER = 2 };
 
 enum Epi : int {
     EPI_F32 = 0,        // C (+ bias) stored as f32
@@ -55,6 +54,7 @@ struct GemmArgs {
     int k_per_split;    // K range of one blockIdx.z (multiple of 32)
     int64_t split_stride;  // elements between the partial outputs of consecutive z
     int gH, gW, gC, gsign;  // GATHER: input height, width, channels (multiple of 8), tap sign (+1 / -1)
+    int gdil;               // GATHER: tap spacing (dilation)
 };
 
 constexpr int BK = 32;
@@ -64,6 +64,7 @@ constexpr int LDS_PITCH = BK + 8;  // bf16 elements per LDS row (padding against
 // vectors per thread, zero outside `rows`), registers -> LDS as [row][k].
 template <int L, int R>
 struct TileLoader {
+    static_assert(L == KMAJOR, "K-contiguous operand");
     static constexpr int VECS = 4 * R;                  // R rows x BK/8 vectors
     static constexpr int NV = (VECS + 255) / 256;
     u32x4 v[NV];
@@ -77,25 +78,9 @@ struct TileLoader {
             const int e = t + 256 * i;
             v[i] = u32x4{0u, 0u, 0u, 0u};
             if (VECS % 256 != 0 && e >= VECS) continue;
-            if (L == KMAJOR) {
-                const int r = e >> 2, kg = e & 3;
-                if (r0 + r < rows && k0 + 8 * kg < kend)
-                    v[i] = *(const u32x4*)(src + (int64_t)(r0 + r) * ld + k0 + 8 * kg);
-            } else {
-                const int kk = e / (R / 8), rg = e - kk * (R / 8);
-                const int r = r0 + 8 * rg;
-                const uint16_t* p = src + (int64_t)(k0 + kk) * ld + r;
-                if (k0 + kk >= kend) {
-                    // zero
-                } else if (r + 7 < rows) {
-                    v[i] = *(const u32x4*)p;
-                } else {
-                    uint16_t tmp[8];
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) tmp[j] = (r + j < rows) ? p[j] : (uint16_t)0;
-                    v[i] = *(const u32x4*)tmp;
-                }
-            }
+            const int r = e >> 2, kg = e & 3;
+            if (r0 + r < rows && k0 + 8 * kg < kend)
+                v[i] = *(const u32x4*)(src + (int64_t)(r0 + r) * ld + k0 + 8 * kg);
         }
     }
     __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const
@@ -105,15 +90,7 @@ struct TileLoader {
         for (int i = 0; i < NV; ++i) {
             const int e = t + 256 * i;
             if (VECS % 256 != 0 && e >= VECS) continue;
-            if (L == KMAJOR) {
-                const int r = e >> 2, kg = e & 3;
-                *(u32x4*)(&lds[r][8 * kg]) = v[i];
-            } else {
-                const int kk = e / (R / 8), rg = e - kk * (R / 8);
-                const uint16_t* x = (const uint16_t*)&v[i];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) lds[8 * rg + j][kk] = x[j];
-            }
+            *(u32x4*)(&lds[e >> 2][8 * (e & 3)]) = v[i];
         }
     }
 };
@@ -156,7 +133,7 @@ struct GatherLoader {
             v[i] = u32x4{0u, 0u, 0u, 0u};
             if (k < kend && k < 9 * g.gC) {
                 const int tap = k / g.gC, c = k - tap * g.gC;
-                const int dy = g.gsign * (tap / 3 - 1), dx = g.gsign * (tap % 3 - 1);
+                const int dy = g.gsign * g.gdil * (tap / 3 - 1), dx = g.gsign * g.gdil * (tap % 3 - 1);
                 const int yy = y[i] + dy, xx = x[i] + dx;
                 if (yy >= 0 && yy < g.gH && xx >= 0 && xx < g.gW)
                     v[i] = *(const u32x4*)(in + base[i] + (int64_t)(dy * g.gW + dx) * g.gC + c);
@@ -309,12 +286,12 @@ inline void launch_gemm(int cfg, dim3 grid, hipStream_t s, const GemmArgs& g)
 }
 
 // Transposed im2col for the weight gradient: col[k][p] = element (p, k) of
-// the forward gather (k = tap*C + c < Kpad; zero for k >= 9C and outside the
-// image).  A block builds a 32 (k) x 64 (pixel) tile: 16-byte gathers along
+// the forward gather (k = tap*C + c < Kpad, taps `dil` cells apart; zero for
+// k >= 9C and outside the image).  A block builds a 32 (k) x 64 (pixel) tile: 16-byte gathers along
 // the channels of each pixel, a transpose through LDS, then 16-byte stores
 // along the pixels of each k row.  P and C are multiples of 8.
 __global__ __launch_bounds__(256) void k_im2col_t(const uint16_t* __restrict__ in, uint16_t* __restrict__ col, int N,
-                                                  int H, int W, int C, int Kpad)
+                                                  int H, int W, int C, int Kpad, int dil)
 {
     __shared__ uint16_t tile[32][64 + 8];
     const int64_t P = (int64_t)N * H * W;
@@ -328,7 +305,7 @@ __global__ __launch_bounds__(256) void k_im2col_t(const uint16_t* __restrict__ i
         u32x4 v = {0u, 0u, 0u, 0u};
         if (p < P && k < 9 * C) {
             const int tap = k / C, c = k - tap * C;
-            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const int dy = dil * (tap / 3 - 1), dx = dil * (tap % 3 - 1);
             const int x = (int)(p % W);
             const int64_t ny = p / W;
             const int y = (int)(ny % H);

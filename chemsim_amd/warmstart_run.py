@@ -109,6 +109,19 @@ def summarize(rows, max_steps: int):
     return out
 
 
+def time_step(ctx: Context, base: int = 32, batch: int = 8, iters: int = 10) -> dict:
+    """One training step (forward + loss + backward, no optimizer) of the
+    network on `batch` random 128x512 inputs: milliseconds and the GEMM
+    throughput against the BF16 dense MFMA peak (2.5 PFLOP/s)."""
+    with Net(ctx, 128, 512, base=base, batch=batch) as net:
+        x = np.random.default_rng(0).standard_normal((batch, 128, 512, 8)).astype(np.float32)
+        net.forward(x)
+        ms, flops = net.time_step(iters)
+    tflops = flops / ms / 1e9
+    return {"batch": batch, "ms": round(ms, 3), "gemm_tflops": round(tflops, 1),
+            "frac_of_bf16_peak": round(tflops / 2500.0, 4), "gemm_gflop_per_step": round(flops / 1e9, 1)}
+
+
 def dataset(ctx: Context, chans, args):
     fields, steps, res = WS.steady_states(ctx, chans, tol=args.data_tol, interval=100, chunk=args.chunk,
                                           max_steps=args.data_max_steps, storage=args.storage, log=log)
@@ -132,7 +145,8 @@ def dataset(ctx: Context, chans, args):
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("cmd", choices=["all", "evaluate"])
+    ap.add_argument("cmd", choices=["all", "evaluate", "time"])
+    ap.add_argument("--iters", type=int, default=10, help="forward+backward passes to time (cmd time)")
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=200)
     ap.add_argument("--seed-train", type=int, default=1)
@@ -158,6 +172,10 @@ def main(argv=None) -> int:
     os.makedirs(args.out, exist_ok=True)
     ctx = Context(0)
     log(f"[warmstart] device {ctx.device_name()}")
+    if args.cmd == "time":
+        print(json.dumps(time_step(ctx, args.base, args.batch, args.iters)), flush=True)
+        ctx.close()
+        return 0
     test = WS.make_channels(args.seed_test, args.test)
     if args.eval_count is not None:
         test = test[:args.eval_count]

@@ -30,6 +30,12 @@ def layer_spec(c: int):
             (2 * c, 2 * c, 1), (3 * c, c, 1), (c, c, 1), (c, 8, 0)]
 
 
+# Dilation of each convolution (the 3x3 taps sit d cells apart): the last
+# encoder level and the bottleneck widen the receptive field to the length of
+# the channel.
+DILATION = [1, 1, 1, 1, 1, 2, 4, 8, 2, 1, 1, 1, 1, 1, 1]
+
+
 def unflatten(params: np.ndarray, c: int):
     """Flat parameter vector -> list of (W [cout][9][cin], b [cout])."""
     out, off = [], 0
@@ -48,35 +54,37 @@ def flatten(grads) -> np.ndarray:
     return np.concatenate([np.concatenate([gW.ravel(), gb.ravel()]) for gW, gb in grads])
 
 
-def im2col(x: np.ndarray, sign: int = 1) -> np.ndarray:
-    """x [N][H][W][C] -> [N*H*W][9][C]; tap t = 3(dy+1) + (dx+1) reads x[y+s*dy][x+s*dx]."""
+def im2col(x: np.ndarray, sign: int = 1, dil: int = 1) -> np.ndarray:
+    """x [N][H][W][C] -> [N*H*W][9][C]; tap t = 3(dy+1) + (dx+1) reads
+    x[y + s*d*dy][x + s*d*dx] (zero outside)."""
     N, H, W, C = x.shape
-    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+    d = dil
+    xp = np.pad(x, ((0, 0), (d, d), (d, d), (0, 0)))
     cols = []
     for dy in (-1, 0, 1):
         for dx in (-1, 0, 1):
-            sy, sx = sign * dy, sign * dx
-            cols.append(xp[:, 1 + sy:1 + sy + H, 1 + sx:1 + sx + W, :])
+            sy, sx = sign * dy * d, sign * dx * d
+            cols.append(xp[:, d + sy:d + sy + H, d + sx:d + sx + W, :])
     return np.stack(cols, axis=3).reshape(N * H * W, 9, C)
 
 
-def conv(x, W, b):
+def conv(x, W, b, dil=1):
     N, H, Wd, C = x.shape
-    col = im2col(x).reshape(N * H * Wd, -1)
+    col = im2col(x, 1, dil).reshape(N * H * Wd, -1)
     y = col @ W.reshape(W.shape[0], -1).T + b
     return y.reshape(N, H, Wd, -1)
 
 
-def conv_back(x, W, dz):
-    """Gradients of y = conv(x, W, b) for upstream dz: (dx, dW, db)."""
+def conv_back(x, W, dz, dil=1):
+    """Gradients of y = conv(x, W, b, dil) for upstream dz: (dx, dW, db)."""
     N, H, Wd, C = x.shape
     cout = W.shape[0]
-    col = im2col(x).reshape(N * H * Wd, -1)
+    col = im2col(x, 1, dil).reshape(N * H * Wd, -1)
     d2 = dz.reshape(-1, cout)
     dW = (d2.T @ col).reshape(W.shape)
     db = d2.sum(0)
     # dx[p][ci] = sum_{t,co} dz[p - d(t)][co] W[co][t][ci]
-    colT = im2col(dz, sign=-1).reshape(N * H * Wd, 9 * cout)       # [p][t][co]
+    colT = im2col(dz, -1, dil).reshape(N * H * Wd, 9 * cout)       # [p][t][co]
     Wd_ = W.transpose(2, 1, 0).reshape(C, 9 * cout)                  # [ci][t][co]
     dx = (colT @ Wd_.T).reshape(N, H, Wd, C)
     return dx, dW, db
@@ -111,7 +119,7 @@ class Net:
 
     def _conv(self, i, x):
         W, b = self.layers[i]
-        z = conv(x, W, b)
+        z = conv(x, W, b, DILATION[i])
         if self.relu[i]:
             z = np.maximum(z, 0.0)
             return self.q(z)
@@ -165,7 +173,7 @@ class Net:
             W, _ = self.layers[i]
             z = self.outs[i]
             dz = q(g * (z > 0)) if self.relu[i] else q(g)
-            dx, dW, db = conv_back(a[f"in{i}"], W, dz)
+            dx, dW, db = conv_back(a[f"in{i}"], W, dz, DILATION[i])
             grads[i] = (dW, db)
             return dx
 

@@ -6,7 +6,7 @@ import pytest
 from oracle import cnn_oracle as K
 
 
-def small_case(c=8, n=2, h=8, w=16, seed=1):
+def small_case(c=8, n=2, h=16, w=32, seed=1):
     rng = np.random.default_rng(seed)
     p = K.init_params(c, seed)
     x = rng.standard_normal((n, h, w, 8))
@@ -46,7 +46,8 @@ def test_matches_torch_autograd():
     X = torch.tensor(x.transpose(0, 3, 1, 2).copy(), dtype=torch.float64)
 
     def cv(i, z, relu=True):
-        y = F.conv2d(z, tensors[i][0], tensors[i][1], padding=1)
+        d = K.DILATION[i]
+        y = F.conv2d(z, tensors[i][0], tensors[i][1], padding=d, dilation=d)
         return torch.relu(y) if relu else y
 
     pool = lambda z: F.avg_pool2d(z, 2)
@@ -77,3 +78,27 @@ def test_bf16_points_stay_close_to_f64():
     assert abs(lbf - l64) < 0.05 * l64
     assert np.linalg.norm(gbf - g64) < 0.05 * np.linalg.norm(g64)
     assert K.bf16(np.float64(1.0 + 2 ** -9)) == 1.0 and K.bf16(np.float64(1.0 + 3 * 2 ** -9)) == 1.0 + 2 ** -7
+
+
+@pytest.mark.parametrize("dil", [1, 2, 3])
+def test_dilated_conv_and_backward_match_torch(dil):
+    """One 3x3 convolution with dilation d on a map larger than its reach:
+    forward, weight/bias and input gradients against torch."""
+    torch = pytest.importorskip("torch")
+    F = torch.nn.functional
+    rng = np.random.default_rng(dil)
+    x = rng.standard_normal((2, 11, 14, 8))
+    W = rng.standard_normal((5, 9, 8))
+    b = rng.standard_normal(5)
+    dz = rng.standard_normal((2, 11, 14, 5))
+    y = K.conv(x, W, b, dil)
+    dx, dW, db = K.conv_back(x, W, dz, dil)
+    X = torch.tensor(x.transpose(0, 3, 1, 2).copy(), requires_grad=True)
+    TW = torch.tensor(W.reshape(5, 3, 3, 8).transpose(0, 3, 1, 2).copy(), requires_grad=True)
+    TB = torch.tensor(b, requires_grad=True)
+    Y = F.conv2d(X, TW, TB, padding=dil, dilation=dil)
+    np.testing.assert_allclose(y, Y.detach().numpy().transpose(0, 2, 3, 1), rtol=1e-12, atol=1e-12)
+    (Y * torch.tensor(dz.transpose(0, 3, 1, 2).copy())).sum().backward()
+    np.testing.assert_allclose(dx, X.grad.numpy().transpose(0, 2, 3, 1), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(dW, TW.grad.numpy().transpose(0, 2, 3, 1).reshape(5, 9, 8), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(db, TB.grad.numpy(), rtol=1e-12, atol=1e-12)

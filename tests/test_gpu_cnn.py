@@ -38,9 +38,12 @@ def small(c=8, n=2, h=16, w=32, seed=11):
     return p, x, t, wt
 
 
-def test_forward_matches_oracle(ctx):
-    c, (p, x, t, wt) = 8, small()
-    with Net(ctx, 16, 32, base=c, batch=2) as net:
+@pytest.mark.parametrize("h,w", [(16, 32), (64, 128)])
+def test_forward_matches_oracle(ctx, h, w):
+    """(64, 128): the dilated bottleneck (taps 4 and 8 apart on an 8 x 16
+    map) reads real neighbours, not only the zero border."""
+    c, (p, x, t, wt) = 8, small(h=h, w=w)
+    with Net(ctx, h, w, base=c, batch=2) as net:
         net.set_params(p)
         np.testing.assert_array_equal(net.get_params(), p)
         y = net.forward(x)
@@ -49,9 +52,10 @@ def test_forward_matches_oracle(ctx):
     assert err < 1e-2, err
 
 
-def test_gradients_match_oracle(ctx):
-    c, (p, x, t, wt) = 8, small(seed=12)
-    with Net(ctx, 16, 32, base=c, batch=2) as net:
+@pytest.mark.parametrize("h,w", [(16, 32), (64, 128)])
+def test_gradients_match_oracle(ctx, h, w):
+    c, (p, x, t, wt) = 8, small(h=h, w=w, seed=12)
+    with Net(ctx, h, w, base=c, batch=2) as net:
         net.set_params(p)
         loss = net.loss_grad(x, t, wt)
         g = net.get_grads()
