@@ -34,12 +34,15 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr"):
+def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr", trt=None):
     from chemsim_amd import Lattice, Params, capi
     from chemsim_amd import workloads as W
     rho, ux, uy = W.taylor_green(n, u0, np.float64 if storage == "f64" else np.float32)
     rnd = capi.CS_ROUND_STOCHASTIC if rounding == "sr" else capi.CS_ROUND_NEAREST
-    lat = Lattice(ctx, n, n, storage, Params(tau=tau, edge=capi.CS_EDGE_PERIODIC, rounding=rnd, seed=2024))
+    p = Params(tau=tau, edge=capi.CS_EDGE_PERIODIC, rounding=rnd, seed=2024)
+    if trt is not None:
+        p.collision, p.tau, p.tau_minus = capi.CS_COLL_TRT, trt[0], trt[1]
+    lat = Lattice(ctx, n, n, storage, p)
     lat.init_equilibrium(rho, ux, uy)
     return lat
 
@@ -203,6 +206,15 @@ def main():
                           "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden),
                           "mass_drift_rel": abs(m1 - m0) / m0}
             log(f"[bench] {key}: {extras[key]}")
+        # TRT operator (lbm.rs:374-451) on the same workload: TRT::new(lambda 1/4, nu of tau 0.8)
+        from chemsim_amd.lattice import trt_new
+        tp, tm = trt_new(0.25, (args.tau - 0.5) / 3.0)
+        lt = tgv_lattice(ctx, n, "f32", args.tau, args.u0, trt=(tp, tm))
+        wt, mst = time_steps(lt, args.steps)
+        lt.close()
+        extras["trt_f32"] = {"GLUPS": round(cells * args.steps / wt / 1e9, 3),
+                             "kernel_us": round(mst / args.steps * 1e3, 2), "tau_plus": tp, "tau_minus": tm}
+        log(f"[bench] trt: {extras['trt_f32']}")
         del x64, y64, r64
 
     if not args.no_batch and not args.no_extras:

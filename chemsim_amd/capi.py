@@ -22,6 +22,7 @@ CS_FLUID, CS_WALL, CS_LID, CS_INLET, CS_OUTLET = range(5)
 CS_ROUND_NEAREST, CS_ROUND_STOCHASTIC = range(2)
 CS_ORDER_LARGEST_FIRST, CS_ORDER_GIVEN, CS_ORDER_SMALLEST_FIRST = range(3)
 CS_LAUNCH_PERSISTENT, CS_LAUNCH_TILES = range(2)
+CS_COLL_BGK, CS_COLL_TRT = range(2)
 
 
 class cs_params(C.Structure):
@@ -39,6 +40,8 @@ class cs_params(C.Structure):
         ("rho0", C.c_double),
         ("rounding", C.c_int32),
         ("seed", C.c_uint32),
+        ("collision", C.c_int32),
+        ("tau_minus", C.c_double),
     ]
 
 
@@ -98,6 +101,7 @@ SIGNATURES = {
     "cs_lattice_init_equilibrium_f64": [_P, _P, _P, _P],
     "cs_lattice_init_nonequilibrium": [_P, _P, _P, _P],
     "cs_bgk_evaluate": [_P, C.c_int64, _P, _P, C.c_float, C.c_float, _P],
+    "cs_trt_new": [C.c_float, C.c_float, C.c_float, C.c_float, _P, _P],
     "cs_lattice_step": [_P, _I],
     "cs_lattice_step_timed": [_P, _I, _FP],
     "cs_lattice_stream": [_P],

@@ -659,6 +659,7 @@ extern "C" int cs_batch_set_params(cs_batch* b, int index, const cs_params* p)
                    "tau, delta_x, delta_t, outlet_rho must be > 0");
         CS_REQUIRE(p->rounding == CS_ROUND_NEAREST || p->rounding == CS_ROUND_STOCHASTIC, CS_ERR_ARG,
                    "bad rounding mode");
+        CS_REQUIRE(p->collision == CS_COLL_BGK, CS_ERR_UNSUPPORTED, "batch lattices use the BGK operator");
         CS_REQUIRE(p->rho0 == b->params[(size_t)index].rho0 || !is_dev_storage(b->storage), CS_ERR_UNSUPPORTED,
                    "rho0 of a deviation-storage batch is fixed at creation (1.0)");
         const bool edge_changed = p->edge != b->params[(size_t)index].edge;

@@ -68,6 +68,7 @@ constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 template <class T>
 struct Consts {
     T K, A, B, C, cs2;
+    T wp, wm, hdt;  // TRT: 1/tau_plus, 1/tau_minus, -dt * 0.5 (lbm.rs:431-440)
     T wq[3];     // w of the rest, axis and diagonal directions
     T baseq[3];  // w * rho0 of the same classes
     T lidq[4];   // lid term of directions 1, 2, 5, 6

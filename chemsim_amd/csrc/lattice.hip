@@ -52,6 +52,10 @@ static Consts<T> make_consts(const cs_params& p)
     k.B = (T)1 / ((T)2 * cs4);
     k.C = (T)-1 / ((T)2 * cs2);
     k.K = -dt / tau;
+    volatile T tau_m = (T)(p.collision == CS_COLL_TRT ? p.tau_minus : p.tau);
+    k.wp = (T)1 / tau;
+    k.wm = (T)1 / tau_m;
+    k.hdt = -dt * (T)0.5;
     volatile T n0 = 16, n1 = 4, n5 = 1, d = 36;
     const T w0 = n0 / d, w1 = n1 / d, w5 = n5 / d;
     const T ws[3] = {w0, w1, w5};
@@ -183,8 +187,9 @@ __global__ void k_bounce_back(typename St::S* __restrict__ f, const uint8_t* __r
     for (int i = 0; i < 9; ++i) f[i * P + own] = v[OPP[i]];
 }
 
-// State::collide (lbm.rs:731-739) with BGK::evaluate (lbm.rs:350-364), every cell, in place.
-template <class St>
+// State::collide (lbm.rs:731-739) with BGK::evaluate (lbm.rs:350-364) or
+// TRT::evaluate (lbm.rs:401-444), every cell, in place.
+template <class St, int COLL>
 __global__ void k_collide(typename St::S* __restrict__ f, int w, int h, int pitch, int64_t P,
                           Consts<typename St::T> k)
 {
@@ -195,7 +200,8 @@ __global__ void k_collide(typename St::S* __restrict__ f, int w, int h, int pitc
     const int64_t own = (int64_t)r * pitch + c;
     T B[9];
     load_own<St>(f, P, own, B);
-    bgk_collide<T>(k, B);
+    if (COLL == CS_COLL_TRT) trt_collide<T>(k, B);
+    else bgk_collide<T>(k, B);
 #pragma unroll
     for (int i = 0; i < 9; ++i) St::st(f, i * P + own, B[i]);
 }
@@ -367,7 +373,7 @@ struct DevBuf {
 
 static int g_cpt_override = 0;
 
-template <class St, int EDGE, int MODE, int CPT, bool SR>
+template <class St, int EDGE, int MODE, int CPT, bool SR, int COLL>
 static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
 {
     StepArgs<typename St::T> a;
@@ -385,9 +391,11 @@ static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
     const int tiles_y = (l->h + 3) / 4;
     const int nblocks = a.tiles_x * tiles_y;
     if (9 * l->plane * (int64_t)sizeof(typename St::S) < ((int64_t)1 << 32))
-        hipLaunchKernelGGL((k_step<St, EDGE, MODE, CPT, SR, uint32_t>), dim3(nblocks), dim3(256), 0, l->stream, a);
+        hipLaunchKernelGGL((k_step<St, EDGE, MODE, CPT, SR, uint32_t, COLL>), dim3(nblocks), dim3(256), 0, l->stream,
+                           a);
     else
-        hipLaunchKernelGGL((k_step<St, EDGE, MODE, CPT, SR, int64_t>), dim3(nblocks), dim3(256), 0, l->stream, a);
+        hipLaunchKernelGGL((k_step<St, EDGE, MODE, CPT, SR, int64_t, COLL>), dim3(nblocks), dim3(256), 0, l->stream,
+                           a);
 }
 
 // Lattices without geometry and 4- or 8-byte storage take the vector kernel
@@ -395,7 +403,7 @@ static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
 // vectors and the buffers allow 32-bit offsets.  16-bit storage keeps the
 // scalar kernel with 4 cells per lane, which measured faster (124 vs 144 us
 // per 4096^2 step with stochastic rounding).
-template <class St, int EDGE, bool SR>
+template <class St, int EDGE, bool SR, int COLL>
 static bool launch_step_vec(cs_lattice* l, const void* fin, void* fout)
 {
     if constexpr (sizeof(typename St::S) < 4) {
@@ -417,20 +425,21 @@ static bool launch_step_vec(cs_lattice* l, const void* fin, void* fout)
         a.cell_base = 0;
         a.k = consts_of<St>(l);
         const int nblocks = a.tiles_x * ((l->h + ROWS - 1) / ROWS);
-        hipLaunchKernelGGL((k_step_vec<St, EDGE, SR, VB, ROWS>), dim3(nblocks), dim3(64 * ROWS), 0, l->stream, a);
+        hipLaunchKernelGGL((k_step_vec<St, EDGE, SR, VB, ROWS, COLL>), dim3(nblocks), dim3(64 * ROWS), 0, l->stream,
+                           a);
         return true;
     }
 }
 
-template <class St, int EDGE, int MODE, bool SR>
+template <class St, int EDGE, int MODE, bool SR, int COLL>
 static void launch_step_sr(cs_lattice* l, const void* fin, void* fout)
 {
-    if (MODE == MODE_FLUID && g_cpt_override == 0 && launch_step_vec<St, EDGE, SR>(l, fin, fout)) return;
+    if (MODE == MODE_FLUID && g_cpt_override == 0 && launch_step_vec<St, EDGE, SR, COLL>(l, fin, fout)) return;
     int cpt = g_cpt_override;
     if (cpt == 0) cpt = (l->storage == CS_BF16DEV || l->storage == CS_F16DEV) ? 4 : 1;
-    if (cpt >= 4) launch_step_cpt<St, EDGE, MODE, 4, SR>(l, fin, fout);
-    else if (cpt == 2) launch_step_cpt<St, EDGE, MODE, 2, SR>(l, fin, fout);
-    else launch_step_cpt<St, EDGE, MODE, 1, SR>(l, fin, fout);
+    if (cpt >= 4) launch_step_cpt<St, EDGE, MODE, 4, SR, COLL>(l, fin, fout);
+    else if (cpt == 2) launch_step_cpt<St, EDGE, MODE, 2, SR, COLL>(l, fin, fout);
+    else launch_step_cpt<St, EDGE, MODE, 1, SR, COLL>(l, fin, fout);
 }
 
 template <class St, int EDGE, int MODE>
@@ -438,11 +447,18 @@ static void launch_step_mode(cs_lattice* l, const void* fin, void* fout)
 {
     if constexpr (is_16bit<St>()) {
         if (l->params.rounding == CS_ROUND_STOCHASTIC) {
-            launch_step_sr<St, EDGE, MODE, true>(l, fin, fout);
+            launch_step_sr<St, EDGE, MODE, true, CS_COLL_BGK>(l, fin, fout);
             return;
         }
     }
-    launch_step_sr<St, EDGE, MODE, false>(l, fin, fout);
+    // TRT: reference arithmetic only (set_params rejects it for the deviation storages)
+    if constexpr (std::is_same_v<St, StF32> || std::is_same_v<St, StF64>) {
+        if (l->params.collision == CS_COLL_TRT) {
+            launch_step_sr<St, EDGE, MODE, false, CS_COLL_TRT>(l, fin, fout);
+            return;
+        }
+    }
+    launch_step_sr<St, EDGE, MODE, false, CS_COLL_BGK>(l, fin, fout);
 }
 
 template <class St>
@@ -666,6 +682,10 @@ extern "C" int cs_lattice_set_params(cs_lattice* l, const cs_params* p)
         CS_REQUIRE(p->outlet_rho > 0, CS_ERR_ARG, "outlet_rho must be > 0");
         CS_REQUIRE(p->rounding == CS_ROUND_NEAREST || p->rounding == CS_ROUND_STOCHASTIC, CS_ERR_ARG,
                    "bad rounding mode");
+        CS_REQUIRE(p->collision == CS_COLL_BGK || p->collision == CS_COLL_TRT, CS_ERR_ARG, "bad collision operator");
+        CS_REQUIRE(p->collision == CS_COLL_BGK || p->tau_minus > 0, CS_ERR_ARG, "tau_minus must be > 0");
+        CS_REQUIRE(p->collision == CS_COLL_BGK || l->storage == CS_F32 || l->storage == CS_F64, CS_ERR_UNSUPPORTED,
+                   "the TRT operator runs on F32 and F64 storage");
         const bool edge_changed = p->edge != l->params.edge;
         // Re-encode 16-bit storage when the deviation base changes.
         if ((l->storage == CS_F32DEV || l->storage == CS_BF16DEV || l->storage == CS_F16DEV) &&
@@ -951,8 +971,14 @@ extern "C" int cs_lattice_collide(cs_lattice* l)
         require_unfused(l);
         dispatch_storage(l->storage, [&]<class St>() {
             if constexpr (std::is_same_v<St, StF32> || std::is_same_v<St, StF64>) {
-                hipLaunchKernelGGL((k_collide<St>), cell_grid(l), dim3(256), 0, l->stream,
-                                   (typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane, consts_of<St>(l));
+                if (l->params.collision == CS_COLL_TRT)
+                    hipLaunchKernelGGL((k_collide<St, CS_COLL_TRT>), cell_grid(l), dim3(256), 0, l->stream,
+                                       (typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
+                                       consts_of<St>(l));
+                else
+                    hipLaunchKernelGGL((k_collide<St, CS_COLL_BGK>), cell_grid(l), dim3(256), 0, l->stream,
+                                       (typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
+                                       consts_of<St>(l));
             }
         });
         CS_HIP(hipGetLastError());
@@ -1196,6 +1222,21 @@ extern "C" int cs_bgk_evaluate(cs_ctx* ctx, int64_t n, const float* f, const flo
                            df + 2 * n, n);
         CS_HIP(hipGetLastError());
         CS_HIP(hipMemcpy(out, df + 2 * n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+// TRT::new (lbm.rs:380-390), evaluated in f32 like the reference.
+extern "C" int cs_trt_new(float lambda, float ks_viscosity, float delta_x, float delta_t, float* tau_plus,
+                          float* tau_minus)
+{
+    return guarded([&] {
+        CS_REQUIRE(tau_plus && tau_minus && delta_x > 0.f && delta_t > 0.f, CS_ERR_ARG, "bad argument");
+        volatile float dx = delta_x, dt = delta_t, three = 3.0f;
+        const float cs = dx / (std::sqrt((float)three) * dt);
+        const float tp = dt * ((ks_viscosity / (cs * cs)) + 0.5f);
+        const float tm = dt * ((lambda / ((tp / dt) - 0.5f)) + 0.5f);
+        *tau_plus = tp;
+        *tau_minus = tm;
     });
 }
 

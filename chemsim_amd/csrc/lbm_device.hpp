@@ -210,6 +210,42 @@ __device__ __forceinline__ void bgk_collide(const Consts<T>& k, T (&B)[9])
     }
 }
 
+// TRT::evaluate (lbm.rs:401-444) of one cell: f_eq from the cell's moments,
+// f_sw = swap_populations (the opposite direction; f_0 for i = 0) and
+// f_eq_sw = swap_equilibrium, which keeps f_eq_0 and takes the opposite
+// *populations* for i = 1..8 (lbm.rs:311-323), then
+//   omega_i = ((f+ - f_eq+) wp + (f- - f_eq-) wm) (-dt / 2),  f_i' = f_i + omega_i
+// with f+- = f_i +- f_sw_i and f_eq+- = f_eq_i +- f_eq_sw_i, in that order.
+template <class T>
+__device__ __forceinline__ void trt_collide(const Consts<T>& k, T (&B)[9])
+{
+#pragma clang fp contract(off)
+    const T rho = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
+    const T mx = ((((B[1] - B[3]) + B[5]) - B[6]) - B[7]) + B[8];
+    const T my = ((((B[2] - B[4]) + B[5]) + B[6]) - B[7]) - B[8];
+    const T inv = T(1) / rho;
+    const T ux = inv * mx, uy = inv * my;
+    const T v2 = ux * ux + uy * uy;
+    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+    T feq[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T s = ((T(1) + vc[i] * k.A) + (vc[i] * vc[i]) * k.B) + v2 * k.C;
+        feq[i] = (rho * k.w(i)) * s;
+    }
+    T o[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T fsw = B[OPP[i]];
+        const T feqsw = i == 0 ? feq[0] : B[OPP[i]];
+        const T fp = B[i] + fsw, fm = B[i] - fsw;
+        const T feqp = feq[i] + feqsw, feqm = feq[i] - feqsw;
+        o[i] = B[i] + ((fp - feqp) * k.wp + (fm - feqm) * k.wm) * k.hdt;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) B[i] = o[i];
+}
+
 // Deviation arithmetic: with g_i = f_i - w_i rho0,
 //   drho = sum g_i, rho = rho0 + drho, m = sum g_i c_i, u = m / rho,
 //   g_eq_i = w_i (drho + rho ((vc A + vc^2 B) + v2 C)),  g'_i = g_i + (g_i - g_eq_i) K,
@@ -380,7 +416,7 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
 // Collides and stores cell (r, c).  Solid cells of the half-way mode are
 // carried unchanged: their stored values are copied bit for bit.  With SR the
 // 16-bit narrowing is stochastic, keyed by (sr_key, cell index).
-template <class St, int MODE, bool SR, class Off = int64_t>
+template <class St, int MODE, bool SR, class Off = int64_t, int COLL = CS_COLL_BGK>
 __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, typename St::S* __restrict__ fout,
                                        int pitch, int64_t P, const Consts<typename St::T>& k, int r, int c,
                                        int type, typename St::T (&B)[9], uint32_t sr_key, uint32_t cell)
@@ -392,8 +428,14 @@ __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, t
         for (int i = 0; i < 9; ++i) *at<Off>(fout, (Off)i * Pl + own) = *at<Off>(fin, (Off)i * Pl + own);
         return;
     }
-    if constexpr (St::DEV) bgk_collide_dev(k, B);
-    else bgk_collide(k, B);
+    if constexpr (COLL == CS_COLL_TRT) {
+        static_assert(!St::DEV, "TRT runs in the reference arithmetic");
+        trt_collide(k, B);
+    } else if constexpr (St::DEV) {
+        bgk_collide_dev(k, B);
+    } else {
+        bgk_collide(k, B);
+    }
     if constexpr (SR) {
         uint32_t h = lowbias32(cell ^ sr_key);
 #pragma unroll
@@ -412,7 +454,7 @@ __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, t
 // Fused step.  A block is 4 waves; wave y handles row 4*ty + y, lane l the
 // columns 64*CPT*tx + l + 64*j, j < CPT.  All loads of a thread's CPT cells
 // are issued before any arithmetic.
-template <class St, int EDGE, int MODE, int CPT, bool SR, class Off>
+template <class St, int EDGE, int MODE, int CPT, bool SR, class Off, int COLL = CS_COLL_BGK>
 __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
 {
     using S = typename St::S;
@@ -437,7 +479,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
     for (int j = 0; j < CPT; ++j) {
         const int c = c0 + 64 * j;
         if (c < a.w)
-            finish<St, MODE, SR, Off>(fin, fout, a.pitch, a.plane, a.k, r, c, type[j], B[j], a.sr_key,
+            finish<St, MODE, SR, Off, COLL>(fin, fout, a.pitch, a.plane, a.k, r, c, type[j], B[j], a.sr_key,
                                       a.cell_base + (uint32_t)(r * a.w + c));
     }
 }
@@ -479,7 +521,7 @@ __device__ __forceinline__ S shfl_elem(S v, int src_lane)
 
 // One row per wave (ROWS rows per block), 64*V columns per wave, VB bytes
 // per lane and plane (V = VB / sizeof(S) cells).  w % V == 0.
-template <class St, int EDGE, bool SR, int VB = 16, int ROWS = 4>
+template <class St, int EDGE, bool SR, int VB = 16, int ROWS = 4, int COLL = CS_COLL_BGK>
 __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T> a)
 {
     using S = typename St::S;
@@ -554,7 +596,8 @@ __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T>
     if (!active) return;
 #pragma unroll
     for (int j = 0; j < V; ++j) {
-        if constexpr (St::DEV) bgk_collide_dev(a.k, B[j]);
+        if constexpr (COLL == CS_COLL_TRT) trt_collide(a.k, B[j]);
+        else if constexpr (St::DEV) bgk_collide_dev(a.k, B[j]);
         else bgk_collide(a.k, B[j]);
     }
     const uint32_t own = (uint32_t)r * (uint32_t)a.pitch + (uint32_t)c0;

@@ -207,6 +207,8 @@ inline cs_params params_for(const Discretization& d, Scalar tau)
     p.boundary = CS_BB_REFERENCE;
     p.outlet_rho = 1.0;
     p.rho0 = 1.0;
+    p.collision = CS_COLL_BGK;
+    p.tau_minus = tau;
     return p;
 }
 }  // namespace detail
@@ -262,8 +264,11 @@ public:
     virtual Populations evaluate(const L& lattice, const Populations& equilibrium,
                                  const Discretization& disc) const = 0;
     virtual Scalar kinematic_shear_viscosity(const Discretization& disc) const = 0;
-    // Operators that are BGK expose it, so State runs the fused device step.
+    // Operators that are BGK expose it.
     virtual const BGK* as_bgk() const { return nullptr; }
+    // Operators the device step implements (BGK, TRT) fill in their
+    // parameters and return true; State then runs the fused device step.
+    virtual bool device_params(cs_params&) const { return false; }
 };
 
 // D2Q9 with device-resident populations (lbm.rs:180-200, 285-325).
@@ -413,6 +418,65 @@ struct BGK final : CollisionOperator<D2Q9> {
         return (d.delta_x * d.delta_x / (Scalar(3) * d.delta_t * d.delta_t)) * (tau - d.delta_t / Scalar(2));
     }
     const BGK* as_bgk() const override { return this; }
+    bool device_params(cs_params& p) const override
+    {
+        p.collision = CS_COLL_BGK;
+        p.tau = tau;
+        p.tau_minus = tau;
+        return true;
+    }
+};
+
+// TRT (lbm.rs:374-451), run by the device step with the reference's
+// arithmetic (swap_equilibrium included).
+struct TRT final : CollisionOperator<D2Q9> {
+    Scalar tau_minus, tau_plus;
+    TRT(Scalar tm, Scalar tp) : tau_minus(tm), tau_plus(tp) {}
+    // TRT::new (lbm.rs:380-390)
+    static TRT new_(Scalar lambda, Scalar ks_viscosity, const Discretization& disc)
+    {
+        float tp = 0.f, tm = 0.f;
+        check(cs_trt_new(lambda, ks_viscosity, disc.delta_x, disc.delta_t, &tp, &tm));
+        return TRT(tm, tp);
+    }
+    // lbm.rs:392-399
+    Scalar lambda(const Discretization& disc) const
+    {
+        const Scalar dt = disc.delta_t;
+        Scalar result = 1.0f;
+        result *= ((tau_plus / dt) - 0.5f);
+        result *= ((tau_minus / dt) - 0.5f);
+        return result;
+    }
+    // On the device: the lattice's populations through the TRT collide kernel
+    // (the passed equilibrium is unused, as in lbm.rs:411-412).
+    Populations evaluate(const D2Q9& lattice, const Populations&, const Discretization& disc) const override
+    {
+        const auto [w, h] = lattice.size();
+        detail::LatticeHandle tmp(w, h, CS_F32);
+        cs_params p = detail::params_for(disc, tau_plus);
+        device_params(p);
+        check(cs_lattice_set_params(tmp.p, &p));
+        const auto f = detail::flatten(lattice.populations(), w, h);
+        check(cs_lattice_upload_f32(tmp.p, f.data()));
+        check(cs_lattice_collide(tmp.p));
+        std::vector<float> out(f.size());
+        check(cs_lattice_download_f32(tmp.p, out.data()));
+        return detail::wrap(out, w, h);
+    }
+    // lbm.rs:446-450: cs^2 (tau_plus / dt - 1/2)
+    Scalar kinematic_shear_viscosity(const Discretization& disc) const override
+    {
+        const Scalar cs = disc.isothermal_speed_of_sound();
+        return cs * cs * (tau_plus / disc.delta_t - 0.5f);
+    }
+    bool device_params(cs_params& p) const override
+    {
+        p.collision = CS_COLL_TRT;
+        p.tau = tau_plus;
+        p.tau_minus = tau_minus;
+        return true;
+    }
 };
 
 // ---- State (lbm.rs:670-819) --------------------------------------------------
@@ -443,8 +507,8 @@ public:
     // device pass when the operator is BGK.
     void step()
     {
-        if (const BGK* b = collision->as_bgk()) {
-            prepare(b->tau);
+        if (device_operator()) {
+            prepare();
             lattice->invalidate_host();
             check(cs_lattice_step(lattice->handle(), 1));
         } else {
@@ -457,7 +521,7 @@ public:
     // lbm.rs:716-729: f_i(r, c) <- f_i(r + cx, c - cy), zero fill.
     void stream()
     {
-        prepare(tau_or_one());
+        prepare();
         cs_lattice* h = lattice->handle();
         lattice->invalidate_host();
         check(cs_lattice_stream(h));
@@ -465,7 +529,7 @@ public:
     // lbm.rs:741-751: f_i <- f_opp(i) at solid cells.
     void bounce_back()
     {
-        prepare(tau_or_one());
+        prepare();
         cs_lattice* h = lattice->handle();
         lattice->invalidate_host();
         check(cs_lattice_bounce_back(h));
@@ -473,8 +537,8 @@ public:
     // lbm.rs:731-739: the operator over every cell with the lattice's equilibrium.
     void collide()
     {
-        if (const BGK* b = collision->as_bgk()) {
-            prepare(b->tau);
+        if (device_operator()) {
+            prepare();
             cs_lattice* h = lattice->handle();
             lattice->invalidate_host();
             check(cs_lattice_collide(h));
@@ -500,22 +564,26 @@ public:
     // lbm.rs:815-818: min over cells of f_eq[0] < 0
     bool is_unstable() const
     {
-        prepare(tau_or_one());
+        prepare();
         int u = 0;
         check(cs_lattice_is_unstable(lattice->handle(), &u));
         return u != 0;
     }
 
 private:
-    Scalar tau_or_one() const
+    bool device_operator() const
     {
-        const BGK* b = collision ? collision->as_bgk() : nullptr;
-        return b ? b->tau : Scalar(1);
+        cs_params p{};
+        return collision && collision->device_params(p);
     }
-    // Parameters and geometry reach the device before any device pass.
-    void prepare(Scalar tau) const
+    // Parameters (the operator's when the device implements it, else BGK
+    // with tau = 1 for the streaming and bounce-back passes) and geometry
+    // reach the device before any device pass.
+    void prepare() const
     {
-        lattice->set_params(detail::params_for(discretization, tau));
+        cs_params p = detail::params_for(discretization, Scalar(1));
+        if (collision) collision->device_params(p);
+        lattice->set_params(p);
         if (geometry.version() != uploaded_geometry_) {
             check(cs_lattice_set_flags(lattice->handle(), geometry.flags().data()));
             uploaded_geometry_ = geometry.version();

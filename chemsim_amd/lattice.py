@@ -26,7 +26,8 @@ def _ptr(a: np.ndarray) -> C.c_void_p:
 
 @dataclass
 class Params:
-    """BGK + discretization + boundary choices (src/lbm.rs:75-86, 345-347)."""
+    """Collision operator + discretization + boundary choices (src/lbm.rs:75-86,
+    345-347; TRT: collision = CS_COLL_TRT, tau = tau_plus, lbm.rs:374-377)."""
     tau: float = 1.0
     delta_x: float = 1.0
     delta_t: float = 1.0
@@ -40,11 +41,21 @@ class Params:
     rho0: float = 1.0
     rounding: int = capi.CS_ROUND_NEAREST
     seed: int = 0
+    collision: int = capi.CS_COLL_BGK
+    tau_minus: float = 1.0
 
     def to_c(self) -> capi.cs_params:
         return capi.cs_params(self.tau, self.delta_x, self.delta_t, int(self.edge), int(self.boundary),
                               self.lid_ux, self.lid_uy, self.inlet_ux, self.inlet_uy, self.outlet_rho,
-                              self.rho0, int(self.rounding), int(self.seed) & 0xffffffff)
+                              self.rho0, int(self.rounding), int(self.seed) & 0xffffffff, int(self.collision),
+                              float(self.tau_minus))
+
+
+def trt_new(lam: float, ks_viscosity: float, delta_x: float = 1.0, delta_t: float = 1.0):
+    """TRT::new (src/lbm.rs:380-390): (tau_plus, tau_minus) in f32."""
+    tp, tm = C.c_float(), C.c_float()
+    call("cs_trt_new", float(lam), float(ks_viscosity), float(delta_x), float(delta_t), C.byref(tp), C.byref(tm))
+    return tp.value, tm.value
 
 
 class Context:

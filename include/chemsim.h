@@ -79,8 +79,12 @@ enum cs_boundary { CS_BB_REFERENCE = 0, CS_BB_HALFWAY = 1 };
  * (Geometry = af::Array<bool>, lbm.rs:99) is flag == WALL || flag == LID. */
 enum cs_flag { CS_FLUID = 0, CS_WALL = 1, CS_LID = 2, CS_INLET = 3, CS_OUTLET = 4 };
 
+/* Collision operator of a lattice: BGK { tau } (lbm.rs:345-370) or
+ * TRT { tau_plus = tau, tau_minus } (lbm.rs:374-451; F32 and F64 storage). */
+enum cs_collision { CS_COLL_BGK = 0, CS_COLL_TRT = 1 };
+
 typedef struct cs_params {
-    double tau;        /* BGK { tau } (lbm.rs:345-347) */
+    double tau;        /* BGK { tau } (lbm.rs:345-347); TRT tau_plus */
     double delta_x;    /* Discretization { delta_x, delta_t } (lbm.rs:75-79) */
     double delta_t;
     int32_t edge;      /* enum cs_edge */
@@ -93,6 +97,8 @@ typedef struct cs_params {
     double rho0;       /* base density of deviation storage (extension) */
     int32_t rounding;  /* 16-bit storage: CS_ROUND_NEAREST or CS_ROUND_STOCHASTIC */
     uint32_t seed;     /* seed of the stochastic rounding */
+    int32_t collision; /* enum cs_collision */
+    double tau_minus;  /* TRT { tau_minus } (lbm.rs:374-377) */
 } cs_params;
 
 /* Narrowing of the 16-bit storages.  STOCHASTIC rounds each stored value up
@@ -185,6 +191,9 @@ int cs_lattice_is_unstable(cs_lattice* lat, int* unstable);         /* :814-818 
  * f32.  The unfused CollisionOperator path of a host layer; synchronous. */
 int cs_bgk_evaluate(cs_ctx* ctx, int64_t n, const float* f, const float* feq, float tau, float delta_t,
                     float* out);
+/* TRT::new (lbm.rs:380-390): tau_plus and tau_minus from lambda and the
+ * kinematic shear viscosity, in f32 as the reference evaluates them. */
+int cs_trt_new(float lambda, float ks_viscosity, float delta_x, float delta_t, float* tau_plus, float* tau_minus);
 /* Moments in the working precision, widened to f64 (any null output is skipped). */
 int cs_lattice_moments_f64(cs_lattice* lat, double* rho, double* ux, double* uy);
 

@@ -36,6 +36,10 @@ void orc_consts_f32_make(const orc_params* p, orc_consts_f32* k)
     k->B = 1.0f / (2.0f * cs4);                    /* lbm.rs:65 */
     k->C = -1.0f / (2.0f * cs2);                   /* lbm.rs:66 */
     k->K = -dt / tau;                              /* lbm.rs:357 */
+    volatile float tau_m = (float)p->tau_minus;
+    k->wp = 1.0f / tau;                            /* lbm.rs:431-432, tau = tau_plus */
+    k->wm = 1.0f / tau_m;
+    k->hdt = -dt * 0.5f;                           /* lbm.rs:440 */
     volatile float w0n = 16.0f, w1n = 4.0f, w5n = 1.0f, d = 36.0f; /* lbm.rs:209-219 */
     float w0 = w0n / d, w1 = w1n / d, w5 = w5n / d;
     const float ws[9] = {w0, w1, w1, w1, w1, w5, w5, w5, w5};
@@ -57,6 +61,10 @@ void orc_consts_f64_make(const orc_params* p, orc_consts_f64* k)
     k->B = 1.0 / (2.0 * cs4);
     k->C = -1.0 / (2.0 * cs2);
     k->K = -dt / tau;
+    volatile double tau_m = p->tau_minus;
+    k->wp = 1.0 / tau;
+    k->wm = 1.0 / tau_m;
+    k->hdt = -dt * 0.5;
     volatile double w0n = 16.0, w1n = 4.0, w5n = 1.0, d = 36.0;
     double w0 = w0n / d, w1 = w1n / d, w5 = w5n / d;
     const double ws[9] = {w0, w1, w1, w1, w1, w5, w5, w5, w5};
@@ -65,6 +73,17 @@ void orc_consts_f64_make(const orc_params* p, orc_consts_f64* k)
         k->lid[i] = 6.0 * W_EXACT[i] * (CX[i] * p->lid_ux + CY[i] * p->lid_uy);
         k->base[i] = ws[i] * p->rho0;
     }
+}
+
+void orc_trt_new(float lambda, float ks_viscosity, float delta_x, float delta_t, float* tau_plus,
+                 float* tau_minus)
+{
+    volatile float dx = delta_x, dt = delta_t, three = 3.0f;
+    float cs = dx / (sqrtf(three) * dt);
+    float tp = dt * ((ks_viscosity / (cs * cs)) + 0.5f);   /* lbm.rs:386 */
+    float tm = dt * ((lambda / ((tp / dt) - 0.5f)) + 0.5f); /* lbm.rs:387 */
+    *tau_plus = tp;
+    *tau_minus = tm;
 }
 
 static void orc_consts_make_body_f32(const orc_params* p, orc_consts_f32* k) { orc_consts_f32_make(p, k); }

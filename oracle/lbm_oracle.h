@@ -33,6 +33,7 @@ enum { ORC_EDGE_ZERO = 0, ORC_EDGE_PERIODIC = 1 };
 enum { ORC_BB_REFERENCE = 0, ORC_BB_HALFWAY = 1 };
 enum { ORC_FLUID = 0, ORC_WALL = 1, ORC_LID = 2, ORC_INLET = 3, ORC_OUTLET = 4 };
 enum { ORC_BF16 = 0, ORC_F16 = 1 };
+enum { ORC_COLL_BGK = 0, ORC_COLL_TRT = 1 };
 
 typedef struct orc_params {
     int w, h;          /* columns, rows */
@@ -45,6 +46,8 @@ typedef struct orc_params {
     double inlet_ux, inlet_uy; /* Zou-He inlet velocity (extension) */
     double outlet_rho;         /* Zou-He outlet density (extension) */
     double rho0;               /* base density of 16-bit deviation storage */
+    int collision;             /* ORC_COLL_*: BGK{tau} or TRT{tau_plus = tau, tau_minus} */
+    double tau_minus;          /* TRT (lbm.rs:374-377) */
 } orc_params;
 
 /* Derived per-run constants, in the working precision.
@@ -54,10 +57,12 @@ typedef struct orc_params {
 typedef struct orc_consts_f32 {
     float K, A, B, C, cs2;
     float w[9], lid[9], base[9];
+    float wp, wm, hdt; /* TRT: 1/tau_plus, 1/tau_minus, -dt * 0.5 (lbm.rs:431-440) */
 } orc_consts_f32;
 typedef struct orc_consts_f64 {
     double K, A, B, C, cs2;
     double w[9], lid[9], base[9];
+    double wp, wm, hdt;
 } orc_consts_f64;
 
 void orc_consts_f32_make(const orc_params* p, orc_consts_f32* out);
@@ -65,6 +70,16 @@ void orc_consts_f64_make(const orc_params* p, orc_consts_f64* out);
 /* Deviation-path constants: as orc_consts_f32_make, with A, B, C, cs2 taken
  * from the f64 evaluation rounded once (extension). */
 void orc_consts_devf32_make(const orc_params* p, orc_consts_f32* out);
+/* TRT::new (lbm.rs:380-390) in f32: tau_plus, tau_minus from lambda and the
+ * kinematic shear viscosity. */
+void orc_trt_new(float lambda, float ks_viscosity, float delta_x, float delta_t, float* tau_plus,
+                 float* tau_minus);
+/* TRT::evaluate (lbm.rs:401-444) of one plane set: out = F + omega with
+ * F_eq recomputed from F and the swapped equilibrium of Lattice::swap_equilibrium
+ * (lbm.rs:311-323: f_eq with directions 1..8 replaced by the opposite
+ * populations).  n cells per plane. */
+void orc_trt_f32(const orc_params* p, int n, const float* F, float* out);
+void orc_trt_f64(const orc_params* p, int n, const double* F, double* out);
 
 /* Consistent start (extension): F (an equilibrium, plain or deviation form)
  * += the first-order Chapman-Enskog non-equilibrium part of post-collision

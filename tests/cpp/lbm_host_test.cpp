@@ -9,6 +9,9 @@
 //   bgk_eval.f32  BGK::evaluate of the initial populations and their equilibrium
 //   moments.f32   density, ux, uy, speed, pressure after the fused run
 //   edited.f32    10 fused steps, a geometry edit, 10 more
+//   trt.f32       20 State::step with TRT::new(0.25, 10) (main.rs:188-189)
+//   trt_eval.f32  TRT::evaluate of the initial populations
+//   trt_tau.f32   (tau_plus, tau_minus)
 // `lbm_host_test cpu` runs only checks that need no GPU.
 #include <lbm.hpp>
 
@@ -95,6 +98,11 @@ static void cpu_checks()
     lbm::BGK b(0.6f);
     EXPECT(b.as_bgk() == &b);
     EXPECT(std::fabs(b.kinematic_shear_viscosity(disc) - (0.1f / 3.0f)) < 1e-7f);
+    const auto trt = lbm::TRT::new_(0.25f, 10.0f, disc);
+    EXPECT(std::fabs(trt.lambda(disc) - 0.25f) < 1e-5f);
+    EXPECT(std::fabs(trt.kinematic_shear_viscosity(disc) - 10.0f) < 1e-4f);
+    cs_params tp{};
+    EXPECT(trt.device_params(tp) && tp.collision == CS_COLL_TRT && (float)tp.tau == trt.tau_plus);
     lbm::Geometry g(4, 3);
     const auto v0 = g.version();
     g.set(3, 2, true);
@@ -181,6 +189,14 @@ int main(int argc, char** argv)
             for (size_t x = 20; x < 30; ++x) s.geometry.set(x, 10, true);
             for (int i = 0; i < 10; ++i) s.step();
             write(out + "/edited.f32", flat(s.populations()));
+        }
+        {
+            const auto trt = lbm::TRT::new_(0.25f, 10.0f, lbm::Discretization{1.0f, 1.0f});
+            auto s = make_state(w, h, std::make_unique<lbm::TRT>(trt));
+            write(out + "/trt_eval.f32", flat(trt.evaluate(*s.lattice, s.equilibrium(), s.discretization)));
+            for (int i = 0; i < 20; ++i) s.step();
+            write(out + "/trt.f32", flat(s.populations()));
+            write(out + "/trt_tau.f32", {trt.tau_plus, trt.tau_minus});
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());

@@ -36,7 +36,7 @@ def random_flags(rng, h, w, lids=True):
 def gpu_params(op: O.Params) -> Params:
     return Params(tau=op.tau, delta_x=op.delta_x, delta_t=op.delta_t, edge=op.edge, boundary=op.boundary,
                   lid_ux=op.lid_ux, lid_uy=op.lid_uy, inlet_ux=op.inlet_ux, inlet_uy=op.inlet_uy,
-                  outlet_rho=op.outlet_rho, rho0=op.rho0)
+                  outlet_rho=op.outlet_rho, rho0=op.rho0, collision=op.collision, tau_minus=op.tau_minus)
 
 
 @pytest.mark.parametrize("storage", ["f32", "f64"])
@@ -444,3 +444,57 @@ def test_vector_fluid_step_matches_oracle(ctx, storage, edge, shape):
             capi.call("cs_set_cells_per_thread", 0)
     np.testing.assert_array_equal(outs[0], ref)
     np.testing.assert_array_equal(outs[1], ref)
+
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+@pytest.mark.parametrize("case", ["fluid_vector", "fluid_ragged", "reference_mask", "halfway_lid"])
+def test_trt_matches_oracle(ctx, storage, edge, case):
+    """TRT (lbm.rs:374-451, with its swap_equilibrium) in the fused step, every
+    kernel path: value for value with the oracle."""
+    rng = np.random.default_rng(31)
+    h, w = (12, 264) if case == "fluid_vector" else (11, 37)
+    dt = STORE[storage]
+    f = rand_state(rng, h, w, dt)
+    flags = None
+    kw = {}
+    if case == "reference_mask":
+        flags = random_flags(rng, h, w, lids=False)
+    elif case == "halfway_lid":
+        flags = random_flags(rng, h, w)
+        kw = dict(boundary=O.BB_HALFWAY, lid=(0.02, -0.03))
+    tp, tm = O.trt_new(0.25, 0.08)
+    op = O.params(w, h, edge=edge, trt=(tp, tm), **kw)
+    with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+        lat.upload(f)
+        lat.step(6)
+        np.testing.assert_array_equal(lat.download(dt), O.run(op, f, 6, flags))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_trt_unfused_phases(ctx, storage):
+    rng = np.random.default_rng(32)
+    h, w = 10, 21
+    dt = STORE[storage]
+    f = rand_state(rng, h, w, dt)
+    flags = random_flags(rng, h, w, lids=False)
+    op = O.params(w, h, trt=(0.9, 0.6))
+    with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+        lat.upload(f)
+        lat.stream()
+        lat.bounce_back()
+        q = O.bounce_back(O.stream(f, O.EDGE_ZERO), flags)
+        np.testing.assert_array_equal(lat.download(dt), q)
+        lat.collide()
+        np.testing.assert_array_equal(lat.download(dt), O.trt(op, q))
+
+
+def test_trt_new_and_storage_check(ctx):
+    from chemsim_amd.lattice import trt_new
+    for lam, nu in ((0.25, 10.0), (3.0 / 16, 0.1)):
+        assert trt_new(lam, nu) == O.trt_new(lam, nu)
+    op = O.params(16, 8, trt=(0.9, 0.6))
+    with Lattice(ctx, 16, 8, "f16") as lat:
+        with pytest.raises(capi.ChemsimError):
+            lat.set_params(gpu_params(op))

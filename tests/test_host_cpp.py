@@ -2,7 +2,8 @@
 tests/cpp/lbm_host_test.cpp the way main.rs drives lbm.rs, against the
 oracle: the fused BGK step, the three phases called one by one, a user
 collision operator on the plugin path, BGK::evaluate, the moment fields and a
-geometry edit between steps -- value for value."""
+geometry edit between steps, and the TRT operator (fused and evaluate) --
+value for value."""
 import os
 import subprocess
 
@@ -58,3 +59,9 @@ def test_host_layer_matches_oracle(tmp_path):
     flags2 = flags.copy()
     flags2[10, 20:30] = 1
     np.testing.assert_array_equal(load("edited.f32"), O.run(op, O.run(op, f0, 10, flags), 10, flags2))
+    # TRT::new(0.25, 10) as main.rs:188-189 builds it
+    tp, tm = np.fromfile(tmp_path / "trt_tau.f32", np.float32)
+    assert (float(tp), float(tm)) == O.trt_new(0.25, 10.0)
+    top = O.params(W_, H_, trt=(float(tp), float(tm)))
+    np.testing.assert_array_equal(load("trt_eval.f32"), O.trt(top, f0))
+    np.testing.assert_array_equal(load("trt.f32"), O.run(top, f0, 20, flags))
