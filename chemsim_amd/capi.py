@@ -23,6 +23,7 @@ CS_ROUND_NEAREST, CS_ROUND_STOCHASTIC = range(2)
 CS_ORDER_LARGEST_FIRST, CS_ORDER_GIVEN, CS_ORDER_SMALLEST_FIRST = range(3)
 CS_LAUNCH_PERSISTENT, CS_LAUNCH_TILES = range(2)
 CS_COLL_BGK, CS_COLL_TRT = range(2)
+CS_RENDER_DENSITY, CS_RENDER_SPEED, CS_RENDER_VELOCITY, CS_RENDER_MOMENTUM = range(4)
 
 
 class cs_params(C.Structure):
@@ -102,6 +103,7 @@ SIGNATURES = {
     "cs_lattice_init_nonequilibrium": [_P, _P, _P, _P],
     "cs_bgk_evaluate": [_P, C.c_int64, _P, _P, C.c_float, C.c_float, _P],
     "cs_trt_new": [C.c_float, C.c_float, C.c_float, C.c_float, _P, _P],
+    "cs_lattice_render": [_P, _I, _I, _P, _FP],
     "cs_lattice_step": [_P, _I],
     "cs_lattice_step_timed": [_P, _I, _FP],
     "cs_lattice_stream": [_P],

@@ -141,4 +141,5 @@ struct cs_lattice {
     int* iscratch = nullptr;
     double time = 0.0;
     uint64_t steps = 0;         // fused steps taken (keys the stochastic rounding)
+    void* render_buf = nullptr; // render pass scratch (fields, partials, RGB)
 };

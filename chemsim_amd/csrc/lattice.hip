@@ -652,7 +652,7 @@ extern "C" int cs_lattice_destroy(cs_lattice* l)
         if (!l) return;
         if (l->stream) (void)hipStreamSynchronize(l->stream);
         for (void* p : {l->buf[0], l->buf[1], (void*)l->flags, (void*)l->node, (void*)l->snap, (void*)l->partials,
-                        (void*)l->dscratch, (void*)l->iscratch})
+                        (void*)l->dscratch, (void*)l->iscratch, l->render_buf})
             if (p) (void)hipFree(p);
         if (l->ev0) (void)hipEventDestroy(l->ev0);
         if (l->ev1) (void)hipEventDestroy(l->ev1);

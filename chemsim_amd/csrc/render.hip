@@ -1,0 +1,224 @@
+// Device-side render pass (render.rs:7-178 as main.rs:157-176 drives it):
+// the display mode's field of the lattice's moments, z-scored over the whole
+// lattice, through the logistic function into HSV, then RGB bytes, with the
+// solid cells painted blue.  The reference computes the fields with
+// ArrayFire and then loops over every pixel on the host; here three passes
+// run on the GPU and only the RGB bytes travel.
+//
+//   scalar modes (density, speed; render_scalar_field, render.rs:23-89):
+//     h = 0, s = 1, v = logistic((x - mean) / std)
+//   vector modes (velocity, momentum density; render_vector_field, render.rs:91-178):
+//     m = vx^2 + vy^2, h = (atan2(vy, vx) + pi) / (2 pi), s = 0.8, v = logistic((m - mean) / std)
+//   each channel clamped to [0, 1], HSV -> RGB, byte = clamp(round(256 c), 0, 255);
+//   solid cells (0, 0, 255) (render_geometry, render.rs:7-21).
+// mean and std are the population mean and standard deviation over all cells
+// (af::mean_all, af::stdev_all), accumulated in f64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+
+#include "cs_internal.hpp"
+#include "lattice_kernels.hpp"
+#include "lbm_device.hpp"
+
+namespace cs {
+
+enum : int { RENDER_BLOCKS = 1024 };
+
+template <class St>
+__global__ __launch_bounds__(256) void k_render_field(const typename St::S* __restrict__ f, int w, int h, int pitch,
+                                                      int64_t P, Consts<typename St::T> k, int mode,
+                                                      float* __restrict__ sval, float* __restrict__ phase,
+                                                      double* __restrict__ partials)
+{
+    using T = typename St::T;
+    __shared__ double sh[2][256];
+    const int64_t n = (int64_t)w * h;
+    double s1 = 0.0, s2 = 0.0;
+    for (int64_t d = (int64_t)blockIdx.x * 256 + threadIdx.x; d < n; d += (int64_t)gridDim.x * 256) {
+        const int r = (int)(d / w), c = (int)(d - (int64_t)r * w);
+        T B[9];
+        load_own<St>(f, P, (int64_t)r * pitch + c, B);
+        T rho, mx, my, ux, uy;
+        moments<St>(k, B, rho, mx, my, ux, uy);
+        float s;
+        if (mode == CS_RENDER_DENSITY) {
+            s = (float)rho;
+        } else if (mode == CS_RENDER_SPEED) {
+            const float vx = (float)ux, vy = (float)uy;
+            s = sqrtf(vx * vx + vy * vy);  // lbm.rs:151-154
+        } else {
+            const float vx = mode == CS_RENDER_VELOCITY ? (float)ux : (float)mx;
+            const float vy = mode == CS_RENDER_VELOCITY ? (float)uy : (float)my;
+            s = vx * vx + vy * vy;         // render.rs:112
+            phase[d] = atan2f(vy, vx);     // af::arg(cplx2(vx, vy)), render.rs:113-114
+        }
+        sval[d] = s;
+        s1 += (double)s;
+        s2 += (double)s * (double)s;
+    }
+    sh[0][threadIdx.x] = s1;
+    sh[1][threadIdx.x] = s2;
+    __syncthreads();
+    for (int q = 128; q > 0; q >>= 1) {
+        if ((int)threadIdx.x < q) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + q];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + q];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sh[0][0];
+        partials[2 * blockIdx.x + 1] = sh[1][0];
+    }
+}
+
+// mean and 1/std as the reference uses them: Matrix::new_filled(avg as f32)
+// and scale(1.0 / std as f32) (render.rs:40-45).
+__global__ __launch_bounds__(256) void k_render_stats(const double* __restrict__ partials, int nblocks, int64_t n,
+                                                      float* __restrict__ stats)
+{
+    __shared__ double sh[2][256];
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+        s1 += partials[2 * b];
+        s2 += partials[2 * b + 1];
+    }
+    sh[0][threadIdx.x] = s1;
+    sh[1][threadIdx.x] = s2;
+    __syncthreads();
+    for (int q = 128; q > 0; q >>= 1) {
+        if ((int)threadIdx.x < q) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + q];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + q];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double mean = sh[0][0] / (double)n;
+        const double var = fmax(sh[1][0] / (double)n - mean * mean, 0.0);
+        stats[0] = (float)mean;
+        stats[1] = 1.0f / (float)sqrt(var);
+    }
+}
+
+// HSV -> RGB, hue in [0, 1] (sextants of 60 degrees).
+__device__ __forceinline__ void hsv_to_rgb(float hue, float sat, float val, float& r, float& g, float& b)
+{
+    const float c = val * sat;
+    const float h6 = hue * 6.0f;
+    const float x = c * (1.0f - fabsf(fmodf(h6, 2.0f) - 1.0f));
+    const float m = val - c;
+    float r1, g1, b1;
+    if (h6 < 1.0f) { r1 = c; g1 = x; b1 = 0.0f; }
+    else if (h6 < 2.0f) { r1 = x; g1 = c; b1 = 0.0f; }
+    else if (h6 < 3.0f) { r1 = 0.0f; g1 = c; b1 = x; }
+    else if (h6 < 4.0f) { r1 = 0.0f; g1 = x; b1 = c; }
+    else if (h6 < 5.0f) { r1 = x; g1 = 0.0f; b1 = c; }
+    else { r1 = c; g1 = 0.0f; b1 = x; }
+    r = r1 + m;
+    g = g1 + m;
+    b = b1 + m;
+}
+
+__device__ __forceinline__ uint8_t to_byte(float v)
+{
+    // (256.0 * v).round().min(255.0).max(0.0) as u8 (render.rs:80-84); round half away from zero
+    return (uint8_t)fmaxf(fminf(roundf(256.0f * v), 255.0f), 0.0f);
+}
+
+__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+__global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__ sval, const float* __restrict__ phase,
+                                                       const float* __restrict__ stats,
+                                                       const uint8_t* __restrict__ flags, int w, int h, int pitch,
+                                                       int mode, int overlay, uint8_t* __restrict__ rgb)
+{
+    const int64_t n = (int64_t)w * h;
+    const int64_t d = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (d >= n) return;
+    const float avg = stats[0], inv = stats[1];
+    const float z = (sval[d] - avg) * inv;
+    const float val = clamp01(1.0f / (1.0f + expf(-z)));  // af::sigmoid
+    float hue = 0.0f, sat = 1.0f;
+    if (mode == CS_RENDER_VELOCITY || mode == CS_RENDER_MOMENTUM) {
+        const float pi = 3.14159265358979323846f;
+        hue = clamp01((phase[d] + pi) * (0.318309886183790671538f * 0.5f));  // render.rs:121-125
+        sat = 0.8f;
+    }
+    float r, g, b;
+    hsv_to_rgb(hue, sat, val, r, g, b);
+    uint8_t o[3] = {to_byte(r), to_byte(g), to_byte(b)};
+    if (overlay && flags) {
+        const int rr = (int)(d / w), cc = (int)(d - (int64_t)rr * w);
+        const int t = flags[(int64_t)rr * pitch + cc];
+        if (t == CS_WALL || t == CS_LID) {
+            o[0] = 0;
+            o[1] = 0;
+            o[2] = 255;
+        }
+    }
+    rgb[3 * d] = o[0];
+    rgb[3 * d + 1] = o[1];
+    rgb[3 * d + 2] = o[2];
+}
+
+template <class St>
+static const Consts<typename St::T>& render_consts(const cs_lattice* l)
+{
+    if constexpr (std::is_same_v<typename St::T, double>) return l->k64;
+    else return l->k32;
+}
+
+}  // namespace cs
+
+using namespace cs;
+
+extern "C" int cs_lattice_render(cs_lattice* l, int mode, int overlay, uint8_t* rgb, float* ms)
+{
+    return guarded([&] {
+        CS_REQUIRE(l != nullptr && rgb != nullptr, CS_ERR_ARG, "null argument");
+        CS_REQUIRE(mode >= CS_RENDER_DENSITY && mode <= CS_RENDER_MOMENTUM, CS_ERR_ARG, "bad render mode");
+        CS_HIP(hipSetDevice(l->ctx->device));
+        const int64_t n = (int64_t)l->w * l->h;
+        if (!l->render_buf) {
+            CS_HIP(hipMalloc(&l->render_buf, sizeof(float) * 2 * (size_t)n + sizeof(double) * 2 * RENDER_BLOCKS +
+                                                 16 * sizeof(float) + 3 * (size_t)n));
+        }
+        float* sval = (float*)l->render_buf;
+        float* phase = sval + n;
+        double* partials = (double*)(phase + n);
+        float* stats = (float*)(partials + 2 * RENDER_BLOCKS);
+        uint8_t* drgb = (uint8_t*)(stats + 16);
+        const int blocks = (int)std::min<int64_t>(RENDER_BLOCKS, (n + 255) / 256);
+        CS_HIP(hipEventRecord(l->ev0, l->stream));
+        auto field = [&]<class St>() {
+            hipLaunchKernelGGL((k_render_field<St>), dim3((unsigned)blocks), dim3(256), 0, l->stream,
+                               (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
+                               render_consts<St>(l), mode, sval, phase, partials);
+        };
+        switch (l->storage) {
+        case CS_F32: field.template operator()<StF32>(); break;
+        case CS_F64: field.template operator()<StF64>(); break;
+        case CS_F32DEV: field.template operator()<StF32Dev>(); break;
+        case CS_BF16DEV: field.template operator()<StBF16>(); break;
+        case CS_F16DEV: field.template operator()<StF16>(); break;
+        default: throw Error{CS_ERR_ARG, "bad storage"};
+        }
+        hipLaunchKernelGGL(k_render_stats, dim3(1), dim3(256), 0, l->stream, partials, blocks, n, stats);
+        hipLaunchKernelGGL(k_render_pixels, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, l->stream, sval, phase,
+                           stats, l->has_flags ? (const uint8_t*)l->flags : nullptr, l->w, l->h, l->pitch, mode,
+                           overlay, drgb);
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipEventRecord(l->ev1, l->stream));
+        CS_HIP(hipMemcpyAsync(rgb, drgb, 3 * (size_t)n, hipMemcpyDeviceToHost, l->stream));
+        CS_HIP(hipStreamSynchronize(l->stream));
+        if (ms) {
+            float t = 0;
+            CS_HIP(hipEventElapsedTime(&t, l->ev0, l->ev1));
+            *ms = t;
+        }
+    });
+}

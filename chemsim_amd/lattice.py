@@ -155,6 +155,14 @@ class Lattice:
             arrs = [np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float32) for a in (rho, ux, uy)]
             call("cs_lattice_init_equilibrium", self.handle, *[_ptr(a) for a in arrs])
 
+    def render(self, mode: int = capi.CS_RENDER_DENSITY, overlay: bool = True):
+        """RGB image [h][w][3] of a display mode (src/render.rs:7-178) and the
+        GPU milliseconds of the render passes."""
+        out = np.empty((self.h, self.w, 3), np.uint8)
+        ms = C.c_float()
+        call("cs_lattice_render", self.handle, int(mode), int(bool(overlay)), _ptr(out), C.byref(ms))
+        return out, ms.value
+
     def init_nonequilibrium(self, rho, ux, uy) -> None:
         """f = f_eq(rho, u) + first-order Chapman-Enskog f_neq from grad u."""
         shape = (self.h, self.w)

@@ -186,6 +186,16 @@ int cs_lattice_pressure(cs_lattice* lat, float* p);                 /* :784-787 
 int cs_lattice_equilibrium(cs_lattice* lat, float* feq);            /* :156-160, 804-807 */
 int cs_lattice_non_equilibrium(cs_lattice* lat, float* fneq);       /* :162-173, 809-812 */
 int cs_lattice_is_unstable(cs_lattice* lat, int* unstable);         /* :814-818 */
+
+/* ---- render pass (next row: render.rs:7-178 as main.rs:157-176 drives it) ---- */
+/* Display modes of main.rs:37-52. */
+enum cs_render_mode { CS_RENDER_DENSITY = 0, CS_RENDER_SPEED = 1, CS_RENDER_VELOCITY = 2, CS_RENDER_MOMENTUM = 3 };
+/* render_scalar_field (density, speed) or render_vector_field (velocity,
+ * momentum density) of the current populations into rgb (3 bytes per
+ * pixel, row-major h x w: pixel (x, y) at 3 (y w + x)), then, if overlay,
+ * render_geometry (solid cells (0, 0, 255)).  Synchronous; ms (if not null)
+ * receives the GPU time of the passes. */
+int cs_lattice_render(cs_lattice* lat, int mode, int overlay, uint8_t* rgb, float* ms);
 /* BGK::evaluate (lbm.rs:350-364) on caller-supplied host populations f and
  * f_eq (n floats each, any layout): out = f + (f - f_eq) * K, K = (-dt)/tau in
  * f32.  The unfused CollisionOperator path of a host layer; synchronous. */
