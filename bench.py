@@ -215,6 +215,20 @@ def main():
         extras["trt_f32"] = {"GLUPS": round(cells * args.steps / wt / 1e9, 3),
                              "kernel_us": round(mst / args.steps * 1e3, 2), "tau_plus": tp, "tau_minus": tm}
         log(f"[bench] trt: {extras['trt_f32']}")
+        # render pass (render.rs) of the 4096^2 state, each display mode; bytes moved per
+        # render: 36 B/cell populations + 16 B/cell field scratch + 3 B/pixel
+        from chemsim_amd import capi as _c
+        lr = tgv_lattice(ctx, n, "f32", args.tau, args.u0)
+        rend = {}
+        for name, mode in (("density", _c.CS_RENDER_DENSITY), ("speed", _c.CS_RENDER_SPEED),
+                           ("velocity", _c.CS_RENDER_VELOCITY), ("momentum", _c.CS_RENDER_MOMENTUM)):
+            lr.render(mode)
+            t = min(lr.render(mode)[1] for _ in range(3))
+            rend[name + "_ms"] = round(t, 3)
+        rend["GBs_density"] = round(55 * cells / (rend["density_ms"] * 1e-3) / 1e9, 1)
+        lr.close()
+        extras["render_4096"] = rend
+        log(f"[bench] render: {rend}")
         del x64, y64, r64
 
     if not args.no_batch and not args.no_extras:
