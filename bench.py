@@ -34,7 +34,7 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr", trt=None):
+def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr", trt=None, collision=None, ks_viscosity=0.0):
     from chemsim_amd import Lattice, Params, capi
     from chemsim_amd import workloads as W
     rho, ux, uy = W.taylor_green(n, u0, np.float64 if storage == "f64" else np.float32)
@@ -42,6 +42,8 @@ def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr", trt=None):
     p = Params(tau=tau, edge=capi.CS_EDGE_PERIODIC, rounding=rnd, seed=2024)
     if trt is not None:
         p.collision, p.tau, p.tau_minus = capi.CS_COLL_TRT, trt[0], trt[1]
+    if collision is not None:
+        p.collision, p.ks_viscosity = collision, ks_viscosity
     lat = Lattice(ctx, n, n, storage, p)
     lat.init_equilibrium(rho, ux, uy)
     return lat
@@ -215,6 +217,16 @@ def main():
         extras["trt_f32"] = {"GLUPS": round(cells * args.steps / wt / 1e9, 3),
                              "kernel_us": round(mst / args.steps * 1e3, 2), "tau_plus": tp, "tau_minus": tm}
         log(f"[bench] trt: {extras['trt_f32']}")
+        # Regularized (lbm.rs:596-666) and KBC (lbm.rs:458-590, nu of tau 0.8) on the same workload
+        from chemsim_amd import capi as _cc
+        for key, coll in (("regularized_f32", _cc.CS_COLL_REGULARIZED), ("kbc_f32", _cc.CS_COLL_KBC)):
+            lo = tgv_lattice(ctx, n, "f32", args.tau, args.u0, collision=coll,
+                             ks_viscosity=(args.tau - 0.5) / 3.0)
+            wo, mso = time_steps(lo, args.steps)
+            lo.close()
+            extras[key] = {"GLUPS": round(cells * args.steps / wo / 1e9, 3),
+                           "kernel_us": round(mso / args.steps * 1e3, 2)}
+            log(f"[bench] {key}: {extras[key]}")
         # render pass (render.rs) of the 4096^2 state, each display mode; bytes moved per
         # render: 36 B/cell populations + 16 B/cell field scratch + 3 B/pixel
         from chemsim_amd import capi as _c

@@ -69,6 +69,8 @@ template <class T>
 struct Consts {
     T K, A, B, C, cs2;
     T wp, wm, hdt;  // TRT: 1/tau_plus, 1/tau_minus, -dt * 0.5 (lbm.rs:431-440)
+    T dx, beta;     // KBC: delta_x, 1 / (2 nu / cs^2 + 1) (lbm.rs:478, 547-550)
+    T sfq[3];       // Regularized: w / (2 cs^4) of the rest, axis, diagonal classes (lbm.rs:651)
     T wq[3];     // w of the rest, axis and diagonal directions
     T baseq[3];  // w * rho0 of the same classes
     T lidq[4];   // lid term of directions 1, 2, 5, 6

@@ -56,12 +56,16 @@ static Consts<T> make_consts(const cs_params& p)
     k.wp = (T)1 / tau;
     k.wm = (T)1 / tau_m;
     k.hdt = -dt * (T)0.5;
+    k.dx = dx;
+    volatile T nu = (T)p.ks_viscosity;
+    k.beta = (T)1 / (((T)2 * nu / (cs * cs)) + (T)1);
     volatile T n0 = 16, n1 = 4, n5 = 1, d = 36;
     const T w0 = n0 / d, w1 = n1 / d, w5 = n5 / d;
     const T ws[3] = {w0, w1, w5};
     for (int q = 0; q < 3; ++q) {
         k.wq[q] = ws[q];
         k.baseq[q] = ws[q] * (T)p.rho0;
+        k.sfq[q] = ws[q] / ((T)2 * cs4);
     }
     // Ladd moving wall: f_i = f*_opp(i) + 2 w_i rho_w (c_i . u_w) / cs^2,
     // rho_w = 1, cs^2 = 1/3 (extension).  Directions 1, 2, 5, 6; the opposite
@@ -201,6 +205,8 @@ __global__ void k_collide(typename St::S* __restrict__ f, int w, int h, int pitc
     T B[9];
     load_own<St>(f, P, own, B);
     if (COLL == CS_COLL_TRT) trt_collide<T>(k, B);
+    else if (COLL == CS_COLL_REGULARIZED) reg_collide<T>(k, B);
+    else if (COLL == CS_COLL_KBC) kbc_collide<T>(k, B);
     else bgk_collide<T>(k, B);
 #pragma unroll
     for (int i = 0; i < 9; ++i) St::st(f, i * P + own, B[i]);
@@ -451,11 +457,13 @@ static void launch_step_mode(cs_lattice* l, const void* fin, void* fout)
             return;
         }
     }
-    // TRT: reference arithmetic only (set_params rejects it for the deviation storages)
+    // TRT, Regularized, KBC: reference arithmetic only (set_params rejects them for the deviation storages)
     if constexpr (std::is_same_v<St, StF32> || std::is_same_v<St, StF64>) {
-        if (l->params.collision == CS_COLL_TRT) {
-            launch_step_sr<St, EDGE, MODE, false, CS_COLL_TRT>(l, fin, fout);
-            return;
+        switch (l->params.collision) {
+        case CS_COLL_TRT: launch_step_sr<St, EDGE, MODE, false, CS_COLL_TRT>(l, fin, fout); return;
+        case CS_COLL_REGULARIZED: launch_step_sr<St, EDGE, MODE, false, CS_COLL_REGULARIZED>(l, fin, fout); return;
+        case CS_COLL_KBC: launch_step_sr<St, EDGE, MODE, false, CS_COLL_KBC>(l, fin, fout); return;
+        default: break;
         }
     }
     launch_step_sr<St, EDGE, MODE, false, CS_COLL_BGK>(l, fin, fout);
@@ -682,10 +690,11 @@ extern "C" int cs_lattice_set_params(cs_lattice* l, const cs_params* p)
         CS_REQUIRE(p->outlet_rho > 0, CS_ERR_ARG, "outlet_rho must be > 0");
         CS_REQUIRE(p->rounding == CS_ROUND_NEAREST || p->rounding == CS_ROUND_STOCHASTIC, CS_ERR_ARG,
                    "bad rounding mode");
-        CS_REQUIRE(p->collision == CS_COLL_BGK || p->collision == CS_COLL_TRT, CS_ERR_ARG, "bad collision operator");
-        CS_REQUIRE(p->collision == CS_COLL_BGK || p->tau_minus > 0, CS_ERR_ARG, "tau_minus must be > 0");
+        CS_REQUIRE(p->collision >= CS_COLL_BGK && p->collision <= CS_COLL_KBC, CS_ERR_ARG, "bad collision operator");
+        CS_REQUIRE(p->collision != CS_COLL_TRT || p->tau_minus > 0, CS_ERR_ARG, "tau_minus must be > 0");
+        CS_REQUIRE(p->collision != CS_COLL_KBC || p->ks_viscosity > 0, CS_ERR_ARG, "ks_viscosity must be > 0");
         CS_REQUIRE(p->collision == CS_COLL_BGK || l->storage == CS_F32 || l->storage == CS_F64, CS_ERR_UNSUPPORTED,
-                   "the TRT operator runs on F32 and F64 storage");
+                   "the TRT, Regularized and KBC operators run on F32 and F64 storage");
         const bool edge_changed = p->edge != l->params.edge;
         // Re-encode 16-bit storage when the deviation base changes.
         if ((l->storage == CS_F32DEV || l->storage == CS_BF16DEV || l->storage == CS_F16DEV) &&
@@ -971,10 +980,14 @@ extern "C" int cs_lattice_collide(cs_lattice* l)
         require_unfused(l);
         dispatch_storage(l->storage, [&]<class St>() {
             if constexpr (std::is_same_v<St, StF32> || std::is_same_v<St, StF64>) {
-                if (l->params.collision == CS_COLL_TRT)
-                    hipLaunchKernelGGL((k_collide<St, CS_COLL_TRT>), cell_grid(l), dim3(256), 0, l->stream,
+                auto go = [&]<int COLL>() {
+                    hipLaunchKernelGGL((k_collide<St, COLL>), cell_grid(l), dim3(256), 0, l->stream,
                                        (typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
                                        consts_of<St>(l));
+                };
+                if (l->params.collision == CS_COLL_TRT) go.template operator()<CS_COLL_TRT>();
+                else if (l->params.collision == CS_COLL_REGULARIZED) go.template operator()<CS_COLL_REGULARIZED>();
+                else if (l->params.collision == CS_COLL_KBC) go.template operator()<CS_COLL_KBC>();
                 else
                     hipLaunchKernelGGL((k_collide<St, CS_COLL_BGK>), cell_grid(l), dim3(256), 0, l->stream,
                                        (typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,

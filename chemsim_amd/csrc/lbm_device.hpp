@@ -246,6 +246,108 @@ __device__ __forceinline__ void trt_collide(const Consts<T>& k, T (&B)[9])
     for (int i = 0; i < 9; ++i) B[i] = o[i];
 }
 
+// Moments and equilibrium of one cell as Lattice::equilibrium computes them
+// (lbm.rs:117-160, 43-71).
+template <class T>
+__device__ __forceinline__ void cell_equilibrium(const Consts<T>& k, const T (&B)[9], T& rho, T& ux, T& uy,
+                                                 T (&feq)[9])
+{
+#pragma clang fp contract(off)
+    rho = (((((((B[0] + B[1]) + B[2]) + B[3]) + B[4]) + B[5]) + B[6]) + B[7]) + B[8];
+    const T mx = ((((B[1] - B[3]) + B[5]) - B[6]) - B[7]) + B[8];
+    const T my = ((((B[2] - B[4]) + B[5]) + B[6]) - B[7]) - B[8];
+    const T inv = T(1) / rho;
+    ux = inv * mx;
+    uy = inv * my;
+    const T v2 = ux * ux + uy * uy;
+    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T s = ((T(1) + vc[i] * k.A) + (vc[i] * vc[i]) * k.B) + v2 * k.C;
+        feq[i] = (rho * k.w(i)) * s;
+    }
+}
+
+// Regularized::evaluate (lbm.rs:606-661): f_eq plus the projection of the
+// non-equilibrium stress on the Hermite tensors, in the reference's order
+//   P_ab = sum_i f_neq_i c_ia c_ib,  Q_i,ab = c_ia c_ib - cs^2 delta_ab,
+//   reg_i = (((f_eq_i + P_xx (Q_xx sf)) + P_xy (Q_xy sf)) + P_yx (Q_yx sf)) + P_yy (Q_yy sf),
+// sf = w_i / (2 cs^4).  The wrapped operator is never called (as in the reference).
+template <class T>
+__device__ __forceinline__ void reg_collide(const Consts<T>& k, T (&B)[9])
+{
+#pragma clang fp contract(off)
+    T rho, ux, uy, feq[9];
+    cell_equilibrium(k, B, rho, ux, uy, feq);
+    T pxx = T(0), pxy = T(0), pyx = T(0), pyy = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T fneq = B[i] - feq[i];
+        const T cx = T(CX[i]), cy = T(CY[i]);
+        pxx = pxx + fneq * (cx * cx);
+        pxy = pxy + fneq * (cx * cy);
+        pyx = pyx + fneq * (cy * cx);
+        pyy = pyy + fneq * (cy * cy);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const T cx = T(CX[i]), cy = T(CY[i]);
+        const T sf = k.sfq[Consts<T>::wclass(i)];
+        T reg = feq[i];
+        reg = reg + pxx * ((cx * cx - k.cs2) * sf);
+        reg = reg + pxy * ((cx * cy) * sf);
+        reg = reg + pyx * ((cy * cx) * sf);
+        reg = reg + pyy * ((cy * cy - k.cs2) * sf);
+        B[i] = reg;
+    }
+}
+
+// KBC::evaluate (lbm.rs:468-572) of one cell, every expression in the
+// reference's order (its diagnostic print of an entropy sum is left out):
+//   pi~ = sum_i f_i dx^2 - u v,  n~ = v^2 - u^2,
+//   ds_0 = ((8uv) pi~ + n~ n~) rho 0.5,
+//   ds_1,3 = ((((u dx - n~) + 1) n~ - (v (4dx) + 8uv) pi~) rho) 0.25,
+//   ds_2,4 = ((((v (-dx) - n~) - 1) n~ - (u (4dx) + 8uv) pi~) rho) 0.25,
+//   ds_5..8 = ((((8uv + u (4dx)) + 2dx^2) pi~ + (n~ - (v - u) dx) n~) rho) 0.125,
+//   dh_i = (f_i - f_eq_i) - ds_i,
+//   gamma = -(((sum ds dh / f_eq) / (sum dh dh / f_eq)) (2 - 1/beta) + (-1/beta)),
+//   f_i' = f_i + (ds_i (2 (-beta)) + (dh_i gamma) (-beta)).
+template <class T>
+__device__ __forceinline__ void kbc_collide(const Consts<T>& k, T (&B)[9])
+{
+#pragma clang fp contract(off)
+    T rho, u, v, feq[9];
+    cell_equilibrium(k, B, rho, u, v, feq);
+    const T dx = k.dx;
+    const T uv = u * v, u2 = u * u, v2 = v * v;
+    const T dx2 = dx * dx;
+    T temp = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) temp = temp + B[i] * dx2;
+    const T pit = temp - uv;
+    const T nt = v2 - u2;
+    const T uv8 = uv * T(8);
+    const T ds0 = (((uv8 * pit) + nt * nt) * rho) * T(0.5);
+    const T ds13 = (((((u * dx - nt) + T(1)) * nt) - ((v * (dx * T(4))) + uv8) * pit) * rho) * T(0.25);
+    const T ds24 = (((((v * (-dx) - nt) + T(-1)) * nt) - ((u * (dx * T(4))) + uv8) * pit) * rho) * T(0.25);
+    const T ds58 = (((((uv8 + u * (T(4) * dx)) + (T(2) * dx) * dx) * pit) + ((nt - (v - u) * dx) * nt)) * rho) *
+                   T(0.125);
+    const T ds[9] = {ds0, ds13, ds24, ds13, ds24, ds58, ds58, ds58, ds58};
+    T dh[9];
+    T num = T(0), den = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        dh[i] = (B[i] - feq[i]) - ds[i];
+        num = num + (ds[i] * dh[i]) / feq[i];
+        den = den + (dh[i] * dh[i]) / feq[i];
+    }
+    const T beta = k.beta;
+    const T gamma = (((num / den) * (T(2) - T(1) / beta)) + (-T(1) / beta)) * T(-1);
+    const T m2b = T(2) * (-beta);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) B[i] = B[i] + (ds[i] * m2b + (dh[i] * gamma) * (-beta));
+}
+
 // Deviation arithmetic: with g_i = f_i - w_i rho0,
 //   drho = sum g_i, rho = rho0 + drho, m = sum g_i c_i, u = m / rho,
 //   g_eq_i = w_i (drho + rho ((vc A + vc^2 B) + v2 C)),  g'_i = g_i + (g_i - g_eq_i) K,
@@ -428,9 +530,13 @@ __device__ __forceinline__ void finish(const typename St::S* __restrict__ fin, t
         for (int i = 0; i < 9; ++i) *at<Off>(fout, (Off)i * Pl + own) = *at<Off>(fin, (Off)i * Pl + own);
         return;
     }
+    if constexpr (COLL != CS_COLL_BGK) static_assert(!St::DEV, "TRT, Regularized and KBC run in the reference arithmetic");
     if constexpr (COLL == CS_COLL_TRT) {
-        static_assert(!St::DEV, "TRT runs in the reference arithmetic");
         trt_collide(k, B);
+    } else if constexpr (COLL == CS_COLL_REGULARIZED) {
+        reg_collide(k, B);
+    } else if constexpr (COLL == CS_COLL_KBC) {
+        kbc_collide(k, B);
     } else if constexpr (St::DEV) {
         bgk_collide_dev(k, B);
     } else {
@@ -597,6 +703,8 @@ __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T>
 #pragma unroll
     for (int j = 0; j < V; ++j) {
         if constexpr (COLL == CS_COLL_TRT) trt_collide(a.k, B[j]);
+        else if constexpr (COLL == CS_COLL_REGULARIZED) reg_collide(a.k, B[j]);
+        else if constexpr (COLL == CS_COLL_KBC) kbc_collide(a.k, B[j]);
         else if constexpr (St::DEV) bgk_collide_dev(a.k, B[j]);
         else bgk_collide(a.k, B[j]);
     }

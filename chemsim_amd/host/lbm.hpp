@@ -429,6 +429,26 @@ struct BGK final : CollisionOperator<D2Q9> {
 
 // TRT (lbm.rs:374-451), run by the device step with the reference's
 // arithmetic (swap_equilibrium included).
+namespace detail {
+// One collision of the lattice's populations on the device with the operator's
+// cs_params (the evaluate() of every operator the device implements).
+inline Populations collide_on_device(const CollisionOperator<D2Q9>& op, const D2Q9& lattice,
+                                     const Discretization& disc)
+{
+    const auto [w, h] = lattice.size();
+    LatticeHandle tmp(w, h, CS_F32);
+    cs_params p = params_for(disc, 1.0f);
+    op.device_params(p);
+    check(cs_lattice_set_params(tmp.p, &p));
+    const auto f = flatten(lattice.populations(), w, h);
+    check(cs_lattice_upload_f32(tmp.p, f.data()));
+    check(cs_lattice_collide(tmp.p));
+    std::vector<float> out(f.size());
+    check(cs_lattice_download_f32(tmp.p, out.data()));
+    return wrap(out, w, h);
+}
+}  // namespace detail
+
 struct TRT final : CollisionOperator<D2Q9> {
     Scalar tau_minus, tau_plus;
     TRT(Scalar tm, Scalar tp) : tau_minus(tm), tau_plus(tp) {}
@@ -452,17 +472,7 @@ struct TRT final : CollisionOperator<D2Q9> {
     // (the passed equilibrium is unused, as in lbm.rs:411-412).
     Populations evaluate(const D2Q9& lattice, const Populations&, const Discretization& disc) const override
     {
-        const auto [w, h] = lattice.size();
-        detail::LatticeHandle tmp(w, h, CS_F32);
-        cs_params p = detail::params_for(disc, tau_plus);
-        device_params(p);
-        check(cs_lattice_set_params(tmp.p, &p));
-        const auto f = detail::flatten(lattice.populations(), w, h);
-        check(cs_lattice_upload_f32(tmp.p, f.data()));
-        check(cs_lattice_collide(tmp.p));
-        std::vector<float> out(f.size());
-        check(cs_lattice_download_f32(tmp.p, out.data()));
-        return detail::wrap(out, w, h);
+        return detail::collide_on_device(*this, lattice, disc);
     }
     // lbm.rs:446-450: cs^2 (tau_plus / dt - 1/2)
     Scalar kinematic_shear_viscosity(const Discretization& disc) const override
@@ -475,6 +485,50 @@ struct TRT final : CollisionOperator<D2Q9> {
         p.collision = CS_COLL_TRT;
         p.tau = tau_plus;
         p.tau_minus = tau_minus;
+        return true;
+    }
+};
+
+// KBC (lbm.rs:458-590): entropic multi-relaxation with the kinematic shear
+// viscosity as its only parameter.  evaluate() runs the KBC collide kernel on
+// the lattice's populations (the passed equilibrium is unused, lbm.rs:475-476);
+// the reference's diagnostic print of an entropy sum (lbm.rs:574-582) is not made.
+struct KBC final : CollisionOperator<D2Q9> {
+    Scalar ks_viscosity;
+    explicit KBC(Scalar nu) : ks_viscosity(nu) {}
+    static KBC new_(Scalar nu) { return KBC(nu); }  // KBC::new, lbm.rs:462-466
+    Populations evaluate(const D2Q9& lattice, const Populations&, const Discretization& disc) const override
+    {
+        return detail::collide_on_device(*this, lattice, disc);
+    }
+    Scalar kinematic_shear_viscosity(const Discretization&) const override { return ks_viscosity; }  // lbm.rs:586-588
+    bool device_params(cs_params& p) const override
+    {
+        p.collision = CS_COLL_KBC;
+        p.ks_viscosity = ks_viscosity;
+        return true;
+    }
+};
+
+// Regularized<C> (lbm.rs:592-666): Latt & Chopard's regularized collision.  As in
+// the reference, evaluate() never calls the wrapped operator; it only supplies
+// the viscosity (lbm.rs:662-664).
+template <class C>
+struct Regularized final : CollisionOperator<D2Q9> {
+    C underlying;
+    explicit Regularized(C u) : underlying(std::move(u)) {}
+    static Regularized new_(C u) { return Regularized(std::move(u)); }  // lbm.rs:600-604
+    Populations evaluate(const D2Q9& lattice, const Populations&, const Discretization& disc) const override
+    {
+        return detail::collide_on_device(*this, lattice, disc);
+    }
+    Scalar kinematic_shear_viscosity(const Discretization& disc) const override
+    {
+        return underlying.kinematic_shear_viscosity(disc);
+    }
+    bool device_params(cs_params& p) const override
+    {
+        p.collision = CS_COLL_REGULARIZED;
         return true;
     }
 };

@@ -27,7 +27,9 @@ def _ptr(a: np.ndarray) -> C.c_void_p:
 @dataclass
 class Params:
     """Collision operator + discretization + boundary choices (src/lbm.rs:75-86,
-    345-347; TRT: collision = CS_COLL_TRT, tau = tau_plus, lbm.rs:374-377)."""
+    345-347; TRT: collision = CS_COLL_TRT, tau = tau_plus, lbm.rs:374-377;
+    Regularized: collision = CS_COLL_REGULARIZED, lbm.rs:596-666; KBC:
+    collision = CS_COLL_KBC with ks_viscosity, lbm.rs:458-460)."""
     tau: float = 1.0
     delta_x: float = 1.0
     delta_t: float = 1.0
@@ -43,12 +45,13 @@ class Params:
     seed: int = 0
     collision: int = capi.CS_COLL_BGK
     tau_minus: float = 1.0
+    ks_viscosity: float = 0.0
 
     def to_c(self) -> capi.cs_params:
         return capi.cs_params(self.tau, self.delta_x, self.delta_t, int(self.edge), int(self.boundary),
                               self.lid_ux, self.lid_uy, self.inlet_ux, self.inlet_uy, self.outlet_rho,
                               self.rho0, int(self.rounding), int(self.seed) & 0xffffffff, int(self.collision),
-                              float(self.tau_minus))
+                              float(self.tau_minus), float(self.ks_viscosity))
 
 
 def trt_new(lam: float, ks_viscosity: float, delta_x: float = 1.0, delta_t: float = 1.0):

@@ -79,9 +79,12 @@ enum cs_boundary { CS_BB_REFERENCE = 0, CS_BB_HALFWAY = 1 };
  * (Geometry = af::Array<bool>, lbm.rs:99) is flag == WALL || flag == LID. */
 enum cs_flag { CS_FLUID = 0, CS_WALL = 1, CS_LID = 2, CS_INLET = 3, CS_OUTLET = 4 };
 
-/* Collision operator of a lattice: BGK { tau } (lbm.rs:345-370) or
- * TRT { tau_plus = tau, tau_minus } (lbm.rs:374-451; F32 and F64 storage). */
-enum cs_collision { CS_COLL_BGK = 0, CS_COLL_TRT = 1 };
+/* Collision operator of a lattice: BGK { tau } (lbm.rs:345-370),
+ * TRT { tau_plus = tau, tau_minus } (lbm.rs:374-451), Regularized<_>
+ * (lbm.rs:596-666; its evaluate never calls the wrapped operator) or
+ * KBC { ks_viscosity } (lbm.rs:458-590).  Operators other than BGK run on
+ * F32 and F64 storage. */
+enum cs_collision { CS_COLL_BGK = 0, CS_COLL_TRT = 1, CS_COLL_REGULARIZED = 2, CS_COLL_KBC = 3 };
 
 typedef struct cs_params {
     double tau;        /* BGK { tau } (lbm.rs:345-347); TRT tau_plus */
@@ -99,6 +102,7 @@ typedef struct cs_params {
     uint32_t seed;     /* seed of the stochastic rounding */
     int32_t collision; /* enum cs_collision */
     double tau_minus;  /* TRT { tau_minus } (lbm.rs:374-377) */
+    double ks_viscosity; /* KBC { ks_viscosity } (lbm.rs:458-460) */
 } cs_params;
 
 /* Narrowing of the 16-bit storages.  STOCHASTIC rounds each stored value up

@@ -40,11 +40,16 @@ void orc_consts_f32_make(const orc_params* p, orc_consts_f32* k)
     k->wp = 1.0f / tau;                            /* lbm.rs:431-432, tau = tau_plus */
     k->wm = 1.0f / tau_m;
     k->hdt = -dt * 0.5f;                           /* lbm.rs:440 */
+    k->dx = dx;                                    /* lbm.rs:478 */
+    volatile float nu = (float)p->ks_viscosity;
+    float cs_k = dx / (sqrtf(three) * dt);         /* isothermal_speed_of_sound, lbm.rs:548 */
+    k->beta = 1.0f / ((2.0f * nu / (cs_k * cs_k)) + 1.0f); /* lbm.rs:549 */
     volatile float w0n = 16.0f, w1n = 4.0f, w5n = 1.0f, d = 36.0f; /* lbm.rs:209-219 */
     float w0 = w0n / d, w1 = w1n / d, w5 = w5n / d;
     const float ws[9] = {w0, w1, w1, w1, w1, w5, w5, w5, w5};
     for (int i = 0; i < 9; ++i) {
         k->w[i] = ws[i];
+        k->sf[i] = ws[i] / (2.0f * cs4);           /* lbm.rs:651 */
         k->lid[i] = (float)(6.0 * W_EXACT[i] * (CX[i] * p->lid_ux + CY[i] * p->lid_uy));
         k->base[i] = ws[i] * (float)p->rho0;
     }
@@ -65,11 +70,16 @@ void orc_consts_f64_make(const orc_params* p, orc_consts_f64* k)
     k->wp = 1.0 / tau;
     k->wm = 1.0 / tau_m;
     k->hdt = -dt * 0.5;
+    k->dx = dx;
+    volatile double nu = p->ks_viscosity;
+    double cs_k = dx / (sqrt(three) * dt);
+    k->beta = 1.0 / ((2.0 * nu / (cs_k * cs_k)) + 1.0);
     volatile double w0n = 16.0, w1n = 4.0, w5n = 1.0, d = 36.0;
     double w0 = w0n / d, w1 = w1n / d, w5 = w5n / d;
     const double ws[9] = {w0, w1, w1, w1, w1, w5, w5, w5, w5};
     for (int i = 0; i < 9; ++i) {
         k->w[i] = ws[i];
+        k->sf[i] = ws[i] / (2.0 * cs4);
         k->lid[i] = 6.0 * W_EXACT[i] * (CX[i] * p->lid_ux + CY[i] * p->lid_uy);
         k->base[i] = ws[i] * p->rho0;
     }

@@ -33,7 +33,7 @@ enum { ORC_EDGE_ZERO = 0, ORC_EDGE_PERIODIC = 1 };
 enum { ORC_BB_REFERENCE = 0, ORC_BB_HALFWAY = 1 };
 enum { ORC_FLUID = 0, ORC_WALL = 1, ORC_LID = 2, ORC_INLET = 3, ORC_OUTLET = 4 };
 enum { ORC_BF16 = 0, ORC_F16 = 1 };
-enum { ORC_COLL_BGK = 0, ORC_COLL_TRT = 1 };
+enum { ORC_COLL_BGK = 0, ORC_COLL_TRT = 1, ORC_COLL_REGULARIZED = 2, ORC_COLL_KBC = 3 };
 
 typedef struct orc_params {
     int w, h;          /* columns, rows */
@@ -46,8 +46,10 @@ typedef struct orc_params {
     double inlet_ux, inlet_uy; /* Zou-He inlet velocity (extension) */
     double outlet_rho;         /* Zou-He outlet density (extension) */
     double rho0;               /* base density of 16-bit deviation storage */
-    int collision;             /* ORC_COLL_*: BGK{tau} or TRT{tau_plus = tau, tau_minus} */
+    int collision;             /* ORC_COLL_*: BGK{tau}, TRT{tau_plus = tau, tau_minus},
+                                  Regularized{} or KBC{ks_viscosity} */
     double tau_minus;          /* TRT (lbm.rs:374-377) */
+    double ks_viscosity;       /* KBC (lbm.rs:458-460) */
 } orc_params;
 
 /* Derived per-run constants, in the working precision.
@@ -58,11 +60,15 @@ typedef struct orc_consts_f32 {
     float K, A, B, C, cs2;
     float w[9], lid[9], base[9];
     float wp, wm, hdt; /* TRT: 1/tau_plus, 1/tau_minus, -dt * 0.5 (lbm.rs:431-440) */
+    float dx, beta;    /* KBC: delta_x, 1 / (2 nu / cs^2 + 1) (lbm.rs:478, 547-550) */
+    float sf[9];       /* Regularized: w_i / (2 cs^4) (lbm.rs:651) */
 } orc_consts_f32;
 typedef struct orc_consts_f64 {
     double K, A, B, C, cs2;
     double w[9], lid[9], base[9];
     double wp, wm, hdt;
+    double dx, beta;
+    double sf[9];
 } orc_consts_f64;
 
 void orc_consts_f32_make(const orc_params* p, orc_consts_f32* out);
@@ -80,6 +86,18 @@ void orc_trt_new(float lambda, float ks_viscosity, float delta_x, float delta_t,
  * populations).  n cells per plane. */
 void orc_trt_f32(const orc_params* p, int n, const float* F, float* out);
 void orc_trt_f64(const orc_params* p, int n, const double* F, double* out);
+/* Regularized::evaluate (lbm.rs:606-661): f_eq + the non-equilibrium stress
+ * P_ab = sum_i (f_i - f_eq_i) c_ia c_ib projected on Q_i,ab = c_ia c_ib -
+ * cs^2 delta_ab with weight w_i / (2 cs^4).  The wrapped operator is unused,
+ * as in the reference. */
+void orc_regularized_f32(const orc_params* p, int n, const float* F, float* out);
+void orc_regularized_f64(const orc_params* p, int n, const double* F, double* out);
+/* KBC::evaluate (lbm.rs:468-585): the reference's shear part delta_s, the
+ * remainder delta_h = f - f_eq - delta_s, the stabiliser gamma* and
+ * f + (-2 beta) delta_s - beta gamma* delta_h.  The reference's diagnostic
+ * print (lbm.rs:574-582) has no output and is not restated.  n cells per plane. */
+void orc_kbc_f32(const orc_params* p, int n, const float* F, float* out);
+void orc_kbc_f64(const orc_params* p, int n, const double* F, double* out);
 
 /* Consistent start (extension): F (an equilibrium, plain or deviation form)
  * += the first-order Chapman-Enskog non-equilibrium part of post-collision

@@ -25,19 +25,21 @@ class Params(C.Structure):
                 ("lid_ux", C.c_double), ("lid_uy", C.c_double),
                 ("inlet_ux", C.c_double), ("inlet_uy", C.c_double),
                 ("outlet_rho", C.c_double), ("rho0", C.c_double),
-                ("collision", C.c_int), ("tau_minus", C.c_double)]
+                ("collision", C.c_int), ("tau_minus", C.c_double), ("ks_viscosity", C.c_double)]
 
 
 class Consts32(C.Structure):
     _fields_ = [("K", C.c_float), ("A", C.c_float), ("B", C.c_float), ("C", C.c_float), ("cs2", C.c_float),
                 ("w", C.c_float * 9), ("lid", C.c_float * 9), ("base", C.c_float * 9),
-                ("wp", C.c_float), ("wm", C.c_float), ("hdt", C.c_float)]
+                ("wp", C.c_float), ("wm", C.c_float), ("hdt", C.c_float),
+                ("dx", C.c_float), ("beta", C.c_float), ("sf", C.c_float * 9)]
 
 
 class Consts64(C.Structure):
     _fields_ = [("K", C.c_double), ("A", C.c_double), ("B", C.c_double), ("C", C.c_double), ("cs2", C.c_double),
                 ("w", C.c_double * 9), ("lid", C.c_double * 9), ("base", C.c_double * 9),
-                ("wp", C.c_double), ("wm", C.c_double), ("hdt", C.c_double)]
+                ("wp", C.c_double), ("wm", C.c_double), ("hdt", C.c_double),
+                ("dx", C.c_double), ("beta", C.c_double), ("sf", C.c_double * 9)]
 
 
 def _load():
@@ -81,6 +83,10 @@ for name, args, res in [
     ("orc_add_neq_f32", [_PP, _P, _P, _P, _P, _P], None),
     ("orc_trt_f32", [_PP, C.c_int, _P, _P], None),
     ("orc_trt_f64", [_PP, C.c_int, _P, _P], None),
+    ("orc_regularized_f32", [_PP, C.c_int, _P, _P], None),
+    ("orc_regularized_f64", [_PP, C.c_int, _P, _P], None),
+    ("orc_kbc_f32", [_PP, C.c_int, _P, _P], None),
+    ("orc_kbc_f64", [_PP, C.c_int, _P, _P], None),
     ("orc_trt_new", [C.c_float, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)], None),
     ("orc_add_neq_f64", [_PP, _P, _P, _P, _P, _P], None),
     ("orc_f32_to_bf16_n", [_P, _P, C.c_long], None),
@@ -107,18 +113,24 @@ def _p(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
 
 
-COLL_BGK, COLL_TRT = 0, 1
+COLL_BGK, COLL_TRT, COLL_REGULARIZED, COLL_KBC = 0, 1, 2, 3
 
 
 def params(w, h, tau=1.0, edge=EDGE_ZERO, boundary=BB_REFERENCE, lid=(0.0, 0.0), inlet=(0.0, 0.0),
-           outlet_rho=1.0, rho0=1.0, delta_x=1.0, delta_t=1.0, trt=None) -> Params:
-    """trt = (tau_plus, tau_minus) selects the TRT operator (tau is then tau_plus)."""
-    coll, tau_m = COLL_BGK, float(tau)
+           outlet_rho=1.0, rho0=1.0, delta_x=1.0, delta_t=1.0, trt=None, regularized=False,
+           kbc=None) -> Params:
+    """trt = (tau_plus, tau_minus) selects the TRT operator (tau is then tau_plus);
+    regularized=True the Regularized operator; kbc = ks_viscosity the KBC operator."""
+    coll, tau_m, nu = COLL_BGK, float(tau), 0.0
     if trt is not None:
         coll, tau, tau_m = COLL_TRT, float(trt[0]), float(trt[1])
+    elif regularized:
+        coll = COLL_REGULARIZED
+    elif kbc is not None:
+        coll, nu = COLL_KBC, float(kbc)
     return Params(int(w), int(h), int(edge), int(boundary), float(tau), float(delta_x), float(delta_t),
                   float(lid[0]), float(lid[1]), float(inlet[0]), float(inlet[1]), float(outlet_rho), float(rho0),
-                  int(coll), float(tau_m))
+                  int(coll), float(tau_m), nu)
 
 
 def trt_new(lam, ks_viscosity, delta_x=1.0, delta_t=1.0):
@@ -133,6 +145,22 @@ def trt(p: Params, f):
     f = np.ascontiguousarray(f)
     out = np.empty_like(f)
     getattr(lib, "orc_trt_" + _sfx(f.dtype))(C.byref(p), int(f[0].size), _p(f), _p(out))
+    return out
+
+
+def regularized(p: Params, f):
+    """Regularized::evaluate (lbm.rs:606-661) of populations f [9][h][w]."""
+    f = np.ascontiguousarray(f)
+    out = np.empty_like(f)
+    getattr(lib, "orc_regularized_" + _sfx(f.dtype))(C.byref(p), int(f[0].size), _p(f), _p(out))
+    return out
+
+
+def kbc(p: Params, f):
+    """KBC::evaluate (lbm.rs:468-572) of populations f [9][h][w] (p.ks_viscosity)."""
+    f = np.ascontiguousarray(f)
+    out = np.empty_like(f)
+    getattr(lib, "orc_kbc_" + _sfx(f.dtype))(C.byref(p), int(f[0].size), _p(f), _p(out))
     return out
 
 

@@ -12,6 +12,8 @@
 //   trt.f32       20 State::step with TRT::new(0.25, 10) (main.rs:188-189)
 //   trt_eval.f32  TRT::evaluate of the initial populations
 //   trt_tau.f32   (tau_plus, tau_minus)
+//   reg.f32, reg_eval.f32  the same with Regularized::new(BGK::new(0.6))
+//   kbc.f32, kbc_eval.f32  the same with KBC::new(0.05)
 // `lbm_host_test cpu` runs only checks that need no GPU.
 #include <lbm.hpp>
 
@@ -103,6 +105,12 @@ static void cpu_checks()
     EXPECT(std::fabs(trt.kinematic_shear_viscosity(disc) - 10.0f) < 1e-4f);
     cs_params tp{};
     EXPECT(trt.device_params(tp) && tp.collision == CS_COLL_TRT && (float)tp.tau == trt.tau_plus);
+    const auto reg = lbm::Regularized<lbm::BGK>::new_(lbm::BGK(0.6f));
+    EXPECT(reg.kinematic_shear_viscosity(disc) == b.kinematic_shear_viscosity(disc));
+    EXPECT(reg.device_params(tp) && tp.collision == CS_COLL_REGULARIZED);
+    const auto kbc = lbm::KBC::new_(0.05f);
+    EXPECT(kbc.kinematic_shear_viscosity(disc) == 0.05f);
+    EXPECT(kbc.device_params(tp) && tp.collision == CS_COLL_KBC && (float)tp.ks_viscosity == 0.05f);
     lbm::Geometry g(4, 3);
     const auto v0 = g.version();
     g.set(3, 2, true);
@@ -197,6 +205,20 @@ int main(int argc, char** argv)
             for (int i = 0; i < 20; ++i) s.step();
             write(out + "/trt.f32", flat(s.populations()));
             write(out + "/trt_tau.f32", {trt.tau_plus, trt.tau_minus});
+        }
+        {
+            const auto reg = lbm::Regularized<lbm::BGK>::new_(lbm::BGK(0.6f));
+            auto s = make_state(w, h, std::make_unique<lbm::Regularized<lbm::BGK>>(reg));
+            write(out + "/reg_eval.f32", flat(reg.evaluate(*s.lattice, s.equilibrium(), s.discretization)));
+            for (int i = 0; i < 20; ++i) s.step();
+            write(out + "/reg.f32", flat(s.populations()));
+        }
+        {
+            const auto kbc = lbm::KBC::new_(0.05f);
+            auto s = make_state(w, h, std::make_unique<lbm::KBC>(kbc));
+            write(out + "/kbc_eval.f32", flat(kbc.evaluate(*s.lattice, s.equilibrium(), s.discretization)));
+            for (int i = 0; i < 20; ++i) s.step();
+            write(out + "/kbc.f32", flat(s.populations()));
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());

@@ -36,7 +36,8 @@ def random_flags(rng, h, w, lids=True):
 def gpu_params(op: O.Params) -> Params:
     return Params(tau=op.tau, delta_x=op.delta_x, delta_t=op.delta_t, edge=op.edge, boundary=op.boundary,
                   lid_ux=op.lid_ux, lid_uy=op.lid_uy, inlet_ux=op.inlet_ux, inlet_uy=op.inlet_uy,
-                  outlet_rho=op.outlet_rho, rho0=op.rho0, collision=op.collision, tau_minus=op.tau_minus)
+                  outlet_rho=op.outlet_rho, rho0=op.rho0, collision=op.collision, tau_minus=op.tau_minus,
+                  ks_viscosity=op.ks_viscosity)
 
 
 @pytest.mark.parametrize("storage", ["f32", "f64"])
@@ -498,3 +499,81 @@ def test_trt_new_and_storage_check(ctx):
     with Lattice(ctx, 16, 8, "f16") as lat:
         with pytest.raises(capi.ChemsimError):
             lat.set_params(gpu_params(op))
+
+
+OTHER = {"regularized": dict(regularized=True, tau=0.8), "kbc": dict(kbc=0.05)}
+
+
+@pytest.mark.parametrize("coll", ["regularized", "kbc"])
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+@pytest.mark.parametrize("case", ["fluid_vector", "fluid_ragged", "reference_mask", "halfway_lid"])
+def test_regularized_kbc_match_oracle(ctx, coll, storage, edge, case):
+    """Regularized (lbm.rs:596-666) and KBC (lbm.rs:458-590) in the fused step,
+    every kernel path: value for value with the oracle."""
+    rng = np.random.default_rng(41)
+    h, w = (12, 264) if case == "fluid_vector" else (11, 37)
+    dt = STORE[storage]
+    f = rand_state(rng, h, w, dt)
+    flags = None
+    kw = dict(OTHER[coll])
+    if case == "reference_mask":
+        flags = random_flags(rng, h, w, lids=False)
+    elif case == "halfway_lid":
+        flags = random_flags(rng, h, w)
+        kw.update(boundary=O.BB_HALFWAY, lid=(0.02, -0.03))
+    op = O.params(w, h, edge=edge, **kw)
+    with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+        lat.upload(f)
+        lat.step(6)
+        np.testing.assert_array_equal(lat.download(dt), O.run(op, f, 6, flags))
+
+
+@pytest.mark.parametrize("coll", ["regularized", "kbc"])
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_regularized_kbc_unfused_phases(ctx, coll, storage):
+    rng = np.random.default_rng(42)
+    h, w = 10, 21
+    dt = STORE[storage]
+    f = rand_state(rng, h, w, dt)
+    flags = random_flags(rng, h, w, lids=False)
+    op = O.params(w, h, **OTHER[coll])
+    with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+        lat.upload(f)
+        lat.stream()
+        lat.bounce_back()
+        q = O.bounce_back(O.stream(f, O.EDGE_ZERO), flags)
+        np.testing.assert_array_equal(lat.download(dt), q)
+        lat.collide()
+        want = O.regularized(op, q) if coll == "regularized" else O.kbc(op, q)
+        np.testing.assert_array_equal(lat.download(dt), want)
+
+
+def test_regularized_kbc_parameter_checks(ctx):
+    op = O.params(16, 8, regularized=True)
+    with Lattice(ctx, 16, 8, "bf16") as lat:
+        with pytest.raises(capi.ChemsimError):
+            lat.set_params(gpu_params(op))
+    with Lattice(ctx, 16, 8, "f32") as lat:
+        with pytest.raises(capi.ChemsimError):   # KBC needs a positive viscosity
+            lat.set_params(gpu_params(O.params(16, 8, kbc=0.0)))
+        lat.set_params(gpu_params(O.params(16, 8, kbc=0.01)))
+
+
+@pytest.mark.parametrize("coll", ["regularized", "kbc"])
+def test_regularized_kbc_cavity_full_size(ctx, coll):
+    """A lid-driven cavity 512 x 512 for 200 steps: the FP32 run equals the
+    oracle's value for value."""
+    n, steps = 512, 200
+    flags = W.cavity_flags(n, n)
+    kw = dict(OTHER[coll])
+    op = O.params(n, n, boundary=O.BB_HALFWAY, lid=W.lid_velocity(0.05), **kw)
+    rho, ux, uy = W.rest_state(n, n)
+    f0 = O.equilibrium(op, rho, ux, uy).astype(np.float32)
+    with Lattice(ctx, n, n, "f32", gpu_params(op), flags) as lat:
+        lat.upload(f0)
+        lat.step(steps)
+        got = lat.download(np.float32)
+    ref, _ = O.run_omp(op, f0, steps, flags)
+    np.testing.assert_array_equal(got, ref)
+    assert np.isfinite(got).all()

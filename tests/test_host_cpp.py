@@ -65,3 +65,9 @@ def test_host_layer_matches_oracle(tmp_path):
     top = O.params(W_, H_, trt=(float(tp), float(tm)))
     np.testing.assert_array_equal(load("trt_eval.f32"), O.trt(top, f0))
     np.testing.assert_array_equal(load("trt.f32"), O.run(top, f0, 20, flags))
+    rop = O.params(W_, H_, regularized=True)
+    np.testing.assert_array_equal(load("reg_eval.f32"), O.regularized(rop, f0))
+    np.testing.assert_array_equal(load("reg.f32"), O.run(rop, f0, 20, flags))
+    kop = O.params(W_, H_, kbc=0.05)
+    np.testing.assert_array_equal(load("kbc_eval.f32"), O.kbc(kop, f0))
+    np.testing.assert_array_equal(load("kbc.f32"), O.run(kop, f0, 20, flags))
