@@ -241,6 +241,19 @@ def main():
         lr.close()
         extras["render_4096"] = rend
         log(f"[bench] render: {rend}")
+        # frame recording (display.rs:157-185): 20 frames of 10 steps each, velocity mode, RGBA
+        # frames (64 MiB) to pinned host memory; depth 1 = the reference's order (no overlap)
+        recd = {}
+        for depth in (1, 3):
+            lrec = tgv_lattice(ctx, n, "f32", args.tau, args.u0)
+            lrec.record(2, 1, _c.CS_RENDER_VELOCITY, True, depth, lambda k, a: True)
+            _, st = lrec.record(20, 10, _c.CS_RENDER_VELOCITY, True, depth, lambda k, a: True)
+            lrec.close()
+            recd[f"depth{depth}"] = {"frames_per_s": round(20 / (st["wall_ms"] / 1e3), 1),
+                                     "wall_ms": round(st["wall_ms"], 2), "gpu_ms": round(st["gpu_ms"], 2),
+                                     "host_wait_ms": round(st["wait_ms"], 2)}
+        extras["record_4096"] = recd
+        log(f"[bench] record: {recd}")
         del x64, y64, r64
 
     if not args.no_batch and not args.no_extras:

@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libchemsim_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "chemsim.h")
 
 # enums (include/chemsim.h)
-CS_OK, CS_ERR_ARG, CS_ERR_HIP, CS_ERR_UNSUPPORTED, CS_ERR_STATE, CS_ERR_NOMEM = range(6)
+CS_OK, CS_ERR_ARG, CS_ERR_HIP, CS_ERR_UNSUPPORTED, CS_ERR_STATE, CS_ERR_NOMEM, CS_ERR_CALLBACK = range(7)
+CS_RECORD_MAX_DEPTH = 16
 CS_F32, CS_F64, CS_BF16DEV, CS_F16DEV, CS_F32DEV = range(5)
 CS_EDGE_ZERO, CS_EDGE_PERIODIC = range(2)
 CS_BB_REFERENCE, CS_BB_HALFWAY = range(2)
@@ -45,6 +46,20 @@ class cs_params(C.Structure):
         ("tau_minus", C.c_double),
         ("ks_viscosity", C.c_double),
     ]
+
+
+class cs_record_stats(C.Structure):
+    _fields_ = [
+        ("wall_ms", C.c_double),
+        ("gpu_ms", C.c_double),
+        ("wait_ms", C.c_double),
+        ("callback_ms", C.c_double),
+        ("frames", C.c_int64),
+    ]
+
+
+# int (*cs_frame_fn)(void* user, int64_t frame, const uint8_t* rgba, int w, int h)
+FRAME_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_uint8), C.c_int, C.c_int)
 
 
 class cs_batch_tuning(C.Structure):
@@ -105,6 +120,7 @@ SIGNATURES = {
     "cs_bgk_evaluate": [_P, C.c_int64, _P, _P, C.c_float, C.c_float, _P],
     "cs_trt_new": [C.c_float, C.c_float, C.c_float, C.c_float, _P, _P],
     "cs_lattice_render": [_P, _I, _I, _P, _FP],
+    "cs_lattice_record": [_P, _I, _I, _I, _I, _I, FRAME_FN, _P, C.POINTER(cs_record_stats)],
     "cs_lattice_step": [_P, _I],
     "cs_lattice_step_timed": [_P, _I, _FP],
     "cs_lattice_stream": [_P],

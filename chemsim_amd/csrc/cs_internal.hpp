@@ -114,6 +114,16 @@ size_t storage_bytes(int storage);
 
 }  // namespace cs
 
+struct cs_lattice;
+namespace cs {
+// n fused steps on the lattice's stream (cs_lattice_step without the checks).
+void do_steps(cs_lattice* l, int n);
+// The render passes of the current populations into d_out (device, h*w pixels
+// of `channels` bytes: 3 = RGB, 4 = RGBA with alpha 255), enqueued on the
+// lattice's stream; allocates the lattice's render scratch on first use.
+void render_enqueue(cs_lattice* l, int mode, int overlay, uint8_t* d_out, int channels);
+}  // namespace cs
+
 // ----------------------------------------------------------------- handles --
 struct cs_ctx {
     int device = 0;

@@ -490,7 +490,7 @@ static void launch_step(cs_lattice* l)
     l->steps += 1;
 }
 
-static void do_steps(cs_lattice* l, int n)
+void do_steps(cs_lattice* l, int n)
 {
     for (int s = 0; s < n; ++s) {
         dispatch_storage(l->storage, [&]<class St>() { launch_step<St>(l); });

@@ -131,6 +131,7 @@ __device__ __forceinline__ uint8_t to_byte(float v)
 
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
+template <int CH>
 __global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__ sval, const float* __restrict__ phase,
                                                        const float* __restrict__ stats,
                                                        const uint8_t* __restrict__ flags, int w, int h, int pitch,
@@ -160,9 +161,15 @@ __global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__
             o[2] = 255;
         }
     }
-    rgb[3 * d] = o[0];
-    rgb[3 * d + 1] = o[1];
-    rgb[3 * d + 2] = o[2];
+    if constexpr (CH == 4) {
+        // RGB::to_rgba (display.rs:27-29): alpha 255, one 32-bit store per pixel
+        reinterpret_cast<uint32_t*>(rgb)[d] = (uint32_t)o[0] | ((uint32_t)o[1] << 8) | ((uint32_t)o[2] << 16) |
+                                              (255u << 24);
+    } else {
+        rgb[3 * d] = o[0];
+        rgb[3 * d + 1] = o[1];
+        rgb[3 * d + 2] = o[2];
+    }
 }
 
 template <class St>
@@ -170,6 +177,62 @@ static const Consts<typename St::T>& render_consts(const cs_lattice* l)
 {
     if constexpr (std::is_same_v<typename St::T, double>) return l->k64;
     else return l->k32;
+}
+
+// The lattice's render scratch: two float fields, the partial sums, the
+// statistics and a 3-byte-per-pixel staging area, allocated on first use.
+struct RenderScratch {
+    float* sval;
+    float* phase;
+    double* partials;
+    float* stats;
+    uint8_t* rgb;
+};
+
+static RenderScratch render_scratch(cs_lattice* l)
+{
+    const int64_t n = (int64_t)l->w * l->h;
+    if (!l->render_buf) {
+        CS_HIP(hipMalloc(&l->render_buf, sizeof(float) * 2 * (size_t)n + sizeof(double) * 2 * RENDER_BLOCKS +
+                                             16 * sizeof(float) + 3 * (size_t)n));
+    }
+    RenderScratch r;
+    r.sval = (float*)l->render_buf;
+    r.phase = r.sval + n;
+    r.partials = (double*)(r.phase + n);
+    r.stats = (float*)(r.partials + 2 * RENDER_BLOCKS);
+    r.rgb = (uint8_t*)(r.stats + 16);
+    return r;
+}
+
+void render_enqueue(cs_lattice* l, int mode, int overlay, uint8_t* d_out, int channels)
+{
+    CS_REQUIRE(d_out != nullptr && (channels == 3 || channels == 4), CS_ERR_ARG, "bad render target");
+    const int64_t n = (int64_t)l->w * l->h;
+    const RenderScratch r = render_scratch(l);
+    const int blocks = (int)std::min<int64_t>(RENDER_BLOCKS, (n + 255) / 256);
+    auto field = [&]<class St>() {
+        hipLaunchKernelGGL((k_render_field<St>), dim3((unsigned)blocks), dim3(256), 0, l->stream,
+                           (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
+                           render_consts<St>(l), mode, r.sval, r.phase, r.partials);
+    };
+    switch (l->storage) {
+    case CS_F32: field.template operator()<StF32>(); break;
+    case CS_F64: field.template operator()<StF64>(); break;
+    case CS_F32DEV: field.template operator()<StF32Dev>(); break;
+    case CS_BF16DEV: field.template operator()<StBF16>(); break;
+    case CS_F16DEV: field.template operator()<StF16>(); break;
+    default: throw Error{CS_ERR_ARG, "bad storage"};
+    }
+    hipLaunchKernelGGL(k_render_stats, dim3(1), dim3(256), 0, l->stream, r.partials, blocks, n, r.stats);
+    const uint8_t* fl = l->has_flags ? (const uint8_t*)l->flags : nullptr;
+    if (channels == 4)
+        hipLaunchKernelGGL(k_render_pixels<4>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, l->stream, r.sval,
+                           r.phase, r.stats, fl, l->w, l->h, l->pitch, mode, overlay, d_out);
+    else
+        hipLaunchKernelGGL(k_render_pixels<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, l->stream, r.sval,
+                           r.phase, r.stats, fl, l->w, l->h, l->pitch, mode, overlay, d_out);
+    CS_HIP(hipGetLastError());
 }
 
 }  // namespace cs
@@ -183,35 +246,9 @@ extern "C" int cs_lattice_render(cs_lattice* l, int mode, int overlay, uint8_t* 
         CS_REQUIRE(mode >= CS_RENDER_DENSITY && mode <= CS_RENDER_MOMENTUM, CS_ERR_ARG, "bad render mode");
         CS_HIP(hipSetDevice(l->ctx->device));
         const int64_t n = (int64_t)l->w * l->h;
-        if (!l->render_buf) {
-            CS_HIP(hipMalloc(&l->render_buf, sizeof(float) * 2 * (size_t)n + sizeof(double) * 2 * RENDER_BLOCKS +
-                                                 16 * sizeof(float) + 3 * (size_t)n));
-        }
-        float* sval = (float*)l->render_buf;
-        float* phase = sval + n;
-        double* partials = (double*)(phase + n);
-        float* stats = (float*)(partials + 2 * RENDER_BLOCKS);
-        uint8_t* drgb = (uint8_t*)(stats + 16);
-        const int blocks = (int)std::min<int64_t>(RENDER_BLOCKS, (n + 255) / 256);
+        uint8_t* drgb = render_scratch(l).rgb;
         CS_HIP(hipEventRecord(l->ev0, l->stream));
-        auto field = [&]<class St>() {
-            hipLaunchKernelGGL((k_render_field<St>), dim3((unsigned)blocks), dim3(256), 0, l->stream,
-                               (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
-                               render_consts<St>(l), mode, sval, phase, partials);
-        };
-        switch (l->storage) {
-        case CS_F32: field.template operator()<StF32>(); break;
-        case CS_F64: field.template operator()<StF64>(); break;
-        case CS_F32DEV: field.template operator()<StF32Dev>(); break;
-        case CS_BF16DEV: field.template operator()<StBF16>(); break;
-        case CS_F16DEV: field.template operator()<StF16>(); break;
-        default: throw Error{CS_ERR_ARG, "bad storage"};
-        }
-        hipLaunchKernelGGL(k_render_stats, dim3(1), dim3(256), 0, l->stream, partials, blocks, n, stats);
-        hipLaunchKernelGGL(k_render_pixels, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, l->stream, sval, phase,
-                           stats, l->has_flags ? (const uint8_t*)l->flags : nullptr, l->w, l->h, l->pitch, mode,
-                           overlay, drgb);
-        CS_HIP(hipGetLastError());
+        render_enqueue(l, mode, overlay, drgb, 3);
         CS_HIP(hipEventRecord(l->ev1, l->stream));
         CS_HIP(hipMemcpyAsync(rgb, drgb, 3 * (size_t)n, hipMemcpyDeviceToHost, l->stream));
         CS_HIP(hipStreamSynchronize(l->stream));

@@ -166,6 +166,39 @@ class Lattice:
         call("cs_lattice_render", self.handle, int(mode), int(bool(overlay)), _ptr(out), C.byref(ms))
         return out, ms.value
 
+    def record(self, frames: int, steps_per_frame: int = 1, mode: int = capi.CS_RENDER_DENSITY,
+               overlay: bool = True, depth: int = 3, on_frame=None):
+        """record (src/display.rs:157-185): `frames` times steps_per_frame steps
+        then the render pass into RGBA.  on_frame(k, rgba[h][w][4]) receives
+        each frame in order (the array is valid only during the call; return
+        False to stop); without it the frames are returned as one array
+        [frames][h][w][4].  Returns (frames or None, stats dict)."""
+        out = None if on_frame is not None else np.empty((frames, self.h, self.w, 4), np.uint8)
+        err = []
+
+        def cb(_user, k, ptr, w, h):
+            try:
+                view = np.ctypeslib.as_array(ptr, shape=(h, w, 4))
+                if out is not None:
+                    out[k] = view
+                    return 0
+                return 0 if on_frame(int(k), view) is not False else 1
+            except BaseException as e:  # re-raised after the call
+                err.append(e)
+                return 1
+
+        fn = capi.FRAME_FN(cb)
+        st = capi.cs_record_stats()
+        code = capi.lib.cs_lattice_record(self.handle, int(frames), int(steps_per_frame), int(mode),
+                                          int(bool(overlay)), int(depth), fn, None, C.byref(st))
+        if err:
+            raise err[0]
+        if code != capi.CS_ERR_CALLBACK:
+            capi.check(code, "cs_lattice_record")
+        stats = {k: getattr(st, k) for k, _ in capi.cs_record_stats._fields_}
+        stats["stopped"] = code == capi.CS_ERR_CALLBACK
+        return out, stats
+
     def init_nonequilibrium(self, rho, ux, uy) -> None:
         """f = f_eq(rho, u) + first-order Chapman-Enskog f_neq from grad u."""
         shape = (self.h, self.w)

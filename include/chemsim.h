@@ -48,7 +48,8 @@ enum cs_status {
     CS_ERR_HIP = 2,         /* HIP runtime error */
     CS_ERR_UNSUPPORTED = 3, /* operation not defined for this storage/mode */
     CS_ERR_STATE = 4,       /* call out of order (e.g. residual before snapshot) */
-    CS_ERR_NOMEM = 5
+    CS_ERR_NOMEM = 5,
+    CS_ERR_CALLBACK = 6     /* a caller's callback asked to stop */
 };
 
 /* Storage precision of the populations. */
@@ -200,6 +201,31 @@ enum cs_render_mode { CS_RENDER_DENSITY = 0, CS_RENDER_SPEED = 1, CS_RENDER_VELO
  * render_geometry (solid cells (0, 0, 255)).  Synchronous; ms (if not null)
  * receives the GPU time of the passes. */
 int cs_lattice_render(cs_lattice* lat, int mode, int overlay, uint8_t* rgb, float* ms);
+/* ---- frame recording (next row: display.rs:157-185) --------------------- */
+/* encoder.write_frame of display.rs:181-182: receives frame `frame` (0-based)
+ * as h x w RGBA pixels (4 bytes, alpha 255; pixel (x, y) at 4 (y w + x)),
+ * valid only during the call; return 0 to continue, non-zero to stop. */
+typedef int (*cs_frame_fn)(void* user, int64_t frame, const uint8_t* rgba, int w, int h);
+enum { CS_RECORD_MAX_DEPTH = 16 };
+typedef struct cs_record_stats {
+    double wall_ms;     /* host time from the first step to the last frame delivered */
+    double gpu_ms;      /* lattice stream: first step to last render pass */
+    double wait_ms;     /* host time blocked on frame copies */
+    double callback_ms; /* host time inside the frame function */
+    int64_t frames;     /* frames delivered */
+} cs_record_stats;
+/* record (display.rs:157-185) of this lattice: `frames` times, steps_per_frame
+ * fused steps (the speed-factor loop of main.rs:128-136) then the render pass
+ * of cs_lattice_render into RGBA, handed to fn in frame order.  Up to depth - 1
+ * frames are in flight: their steps and render passes run on the lattice's
+ * stream while earlier frames are copied to pinned host memory on a second
+ * stream and passed to fn (depth 1: no overlap, the reference's order).  Every
+ * frame equals step + cs_lattice_render.  When fn stops the recording, the
+ * call returns CS_ERR_CALLBACK after the frames in flight finish: the lattice
+ * has then advanced by up to depth - 1 frames more than were delivered.
+ * Allocates 2 * depth frames (device + pinned host) for the call; stats may be null. */
+int cs_lattice_record(cs_lattice* lat, int frames, int steps_per_frame, int mode, int overlay, int depth,
+                      cs_frame_fn fn, void* user, cs_record_stats* stats);
 /* BGK::evaluate (lbm.rs:350-364) on caller-supplied host populations f and
  * f_eq (n floats each, any layout): out = f + (f - f_eq) * K, K = (-dt)/tau in
  * f32.  The unfused CollisionOperator path of a host layer; synchronous. */
