@@ -76,6 +76,84 @@ def cpu_baseline(n, tau, u0, budget_s=12.0):
     return n * n * steps / dt / 1e9, used, steps
 
 
+def cpu_unfused_sample(n, tau, u0, budget_s=6.0):
+    """Oracle unfused FP32 step (one pass per reference operation, as ArrayFire
+    runs them), one thread, on a 1024 x 1024 crop-sized TGV; returns GLUPS."""
+    from chemsim_amd import workloads as W
+    from oracle import pyoracle as O
+    m = min(n, 1024)
+    p = O.params(m, m, tau=tau, edge=O.EDGE_PERIODIC)
+    rho, ux, uy = W.taylor_green(m, u0, np.float32)
+    f = O.equilibrium(p, rho, ux, uy)
+    t0 = time.perf_counter()
+    f = O.run_unfused(p, f, 1)
+    one = time.perf_counter() - t0
+    steps = int(max(1, min(100, budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    O.run_unfused(p, f, steps)
+    return m * m * steps / (time.perf_counter() - t0) / 1e9, m, steps
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def tgv_8192_run(ctx, tau, u0, steps=200):
+    """Taylor-Green 8192 x 8192 (67 M cells): FP32, BF16 and FP16 deviation
+    storage (stochastic rounding) against the FP64 run of the same steps."""
+    n = 8192
+    cells = n * n
+    ref = tgv_lattice(ctx, n, "f64", tau, u0)
+    w64, _ = time_steps(ref, steps)
+    r64, x64, y64 = ref.moments_f64()
+    ref.close()
+    den = np.sqrt((x64 ** 2 + y64 ** 2).sum())
+    rden = np.sqrt(((r64 - 1.0) ** 2).sum())
+    out = {"steps": steps, "f64_GLUPS": round(cells * steps / w64 / 1e9, 3)}
+    for st, rnd in (("f32", "rn"), ("bf16", "sr"), ("f16", "sr")):
+        lat = tgv_lattice(ctx, n, st, tau, u0, rnd)
+        lat.step(5)
+        w, ms = time_steps(lat, steps)
+        lat.close()
+        lat = tgv_lattice(ctx, n, st, tau, u0, rnd)
+        lat.step(steps)
+        r, x, y = lat.moments_f64()
+        lat.close()
+        out[st] = {"GLUPS": round(cells * steps / w / 1e9, 3), "kernel_us": round(ms / steps * 1e3, 1),
+                   "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
+                   "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden)}
+        del r, x, y
+    out["bf16_over_f32"] = round(out["bf16"]["GLUPS"] / out["f32"]["GLUPS"], 3)
+    out["f16_over_f32"] = round(out["f16"]["GLUPS"] / out["f32"]["GLUPS"], 3)
+    return out
+
+
+def cavity_128_run(ctx, steps=10000):
+    """Lid-driven cavity 128 x 128, tau 0.6, lid 0.1, half-way walls, from rest,
+    10,000 steps: launch-bound (16 K cells per step)."""
+    from chemsim_amd import Lattice, Params, capi
+    from chemsim_amd import workloads as W
+    n = 128
+    p = Params(tau=0.6, boundary=capi.CS_BB_HALFWAY, lid_ux=0.0, lid_uy=0.1)
+    with Lattice(ctx, n, n, "f32", p, W.cavity_flags(n, n)) as lat:
+        lat.init_equilibrium(*W.rest_state(n, n))
+        lat.step(10)
+        w, ms = time_steps(lat, steps)
+        ux, uy = lat.velocity()
+    return {"steps": steps, "seconds": round(w, 4), "GLUPS": round(n * n * steps / w / 1e9, 3),
+            "us_per_step": round(w / steps * 1e6, 2), "kernel_us_per_step_events": round(ms / steps * 1e3, 2),
+            "max_speed": round(float(np.sqrt(ux ** 2 + uy ** 2).max()), 5)}
+
+
 def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
     """512 independent lid-driven cavities (sides U{64..512}, tau U[0.55,1],
     lid U[0.02,0.1], seed 0), FP32, half-way walls: the autotuned batch launch
@@ -163,6 +241,11 @@ def main():
     bpc = BYTES_PER_CELL[storage]
     achieved = bpc * cells / (kernel_ms * 1e-3) / 1e9
     log(f"[bench] {glups:.2f} GLUPS wall, {kernel_ms * 1e3:.1f} us/step (events), {achieved:.0f} GB/s algorithmic")
+    # four more timed regions of the same length, for the spread (value stays the first)
+    reps = [glups]
+    for _ in range(4):
+        w_r, _ = time_steps(lat, args.steps)
+        reps.append(cells * args.steps / w_r / 1e9)
 
     copy_bytes = 1 << 30
     copy_ms = ctx.bench_copy(copy_bytes, 20)
@@ -173,7 +256,9 @@ def main():
               "frac_of_copy": round(achieved / copy_gbs, 4),
               "frac_at_144B_per_cell": round(144 * cells / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
               if storage == "f32" else None,
-              "mass_final": float(s_final[0])}
+              "mass_final": float(s_final[0]),
+              "repeats_GLUPS": [round(r, 3) for r in reps],
+              "median_GLUPS": round(float(np.median(reps)), 3)}
 
     if not args.no_extras and storage == "f32":
         # FP64 oracle run and 16-bit storage on the same workload, same steps
@@ -255,6 +340,10 @@ def main():
         extras["record_4096"] = recd
         log(f"[bench] record: {recd}")
         del x64, y64, r64
+        extras["tgv_8192"] = tgv_8192_run(ctx, args.tau, args.u0)
+        log(f"[bench] 8192^2: {extras['tgv_8192']}")
+        extras["cavity_128"] = cavity_128_run(ctx)
+        log(f"[bench] cavity 128^2: {extras['cavity_128']}")
 
     if not args.no_batch and not args.no_extras:
         extras["batch_512_cavities"] = batch_cavities_run(ctx)
@@ -271,6 +360,11 @@ def main():
         cg, threads, csteps = cpu_baseline(n, args.tau, args.u0)
         cpu = {"value": round(cg, 5), "unit": "GLUPS", "cores": threads, "kind": "port",
                "sample": f"oracle fused FP32 step, OpenMP over rows, TGV {n}x{n}, {csteps} steps"}
+        ug, um, usteps = cpu_unfused_sample(n, args.tau, args.u0)
+        cpu["unfused_1thread"] = {"GLUPS": round(ug, 5),
+                                  "sample": f"oracle unfused FP32 step (stream, bounce-back, equilibrium, BGK "
+                                            f"passes), 1 thread, TGV {um}x{um}, {usteps} steps"}
+        cpu["host"] = host_cpu()
         log(f"[bench] cpu baseline {cg:.4f} GLUPS on {threads} threads ({csteps} steps)")
 
     traffic = load_profile_traffic(f"k_step_{storage}_periodic_fluid_{n}")

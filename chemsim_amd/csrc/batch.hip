@@ -73,7 +73,7 @@ __device__ __forceinline__ void batch_lattice_step(const BatchLat<typename St::T
             for (int j = 0; j < CPT; ++j) {
                 const int c = c0 + 64 * j;
                 const int cc = c < L.w ? c : L.w - 1;
-                type[j] = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.plane, L.k, r, cc, B[j]);
+                type[j] = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.pitch, L.plane, L.k, r, cc, B[j]);
             }
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_batch_tiles(const BatchLat<typename St:
     const uint32_t* node = nodes + L.node_off;
     T B[9];
     const int cc = c < L.w ? c : L.w - 1;
-    const int type = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.plane, L.k, r, cc, B);
+    const int type = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.pitch, L.plane, L.k, r, cc, B);
     if (c < L.w)
         finish<St, MODE, SR, uint32_t>(fin, fout, L.pitch, L.plane, L.k, r, c, type, B, key, (uint32_t)(r * L.w + c));
 }
@@ -507,7 +507,7 @@ static void launch_per_lattice(cs_batch* b, int steps)
                 if (!b->active[(size_t)i]) continue;
                 const int cur = (b->state[(size_t)i].cur + s) & 1;
                 const BatchLat<T>& d = L[(size_t)i];
-                StepArgs<T> a;
+                StepArgs<T> a{};
                 typename St::S* base = (typename St::S*)b->arena + d.off;
                 a.fin = base + (int64_t)cur * 9 * d.plane;
                 a.fout = base + (int64_t)(cur ^ 1) * 9 * d.plane;
@@ -515,6 +515,7 @@ static void launch_per_lattice(cs_batch* b, int steps)
                 a.w = d.w;
                 a.h = d.h;
                 a.pitch = d.pitch;
+                a.cpitch = d.pitch;  // batch lattices are planar: node rows share the population pitch
                 a.plane = d.plane;
                 a.tiles_x = (d.w + 63) / 64;
                 a.sr_key = sr_key(d.seed, b->state[(size_t)i].steps + (uint64_t)s);

@@ -133,8 +133,14 @@ struct cs_ctx {
 struct cs_lattice {
     cs_ctx* ctx = nullptr;
     int w = 0, h = 0, storage = CS_F32;
-    int pitch = 0;          // row pitch, elements
-    int64_t plane = 0;      // plane stride, elements
+    // Populations are row-interleaved: row r holds planes 0..8 of that row
+    // one after another, so element (i, r, c) is at i * plane + r * pitch + c
+    // with plane = the padded row length and pitch = 9 * plane.  A thread
+    // block then reads and writes a few contiguous row groups instead of
+    // 9 streams 1/9 of the buffer apart.
+    int pitch = 0;          // population row stride, elements
+    int64_t plane = 0;      // population plane stride, elements
+    int cpitch = 0;         // row pitch of the per-cell arrays (flags, node), elements
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;            // buf[cur] holds the current populations
     uint8_t* flags = nullptr;   // h*pitch flags (device) or null

@@ -176,13 +176,13 @@ __global__ void k_stream(const typename St::S* __restrict__ fin, typename St::S*
 // State::bounce_back (lbm.rs:741-751): at solid cells f_i <- f_opp(i), in place.
 template <class St>
 __global__ void k_bounce_back(typename St::S* __restrict__ f, const uint8_t* __restrict__ flags, int w, int h,
-                              int pitch, int64_t P)
+                              int pitch, int cpitch, int64_t P)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (c >= w) return;
     const int64_t own = (int64_t)r * pitch + c;
-    const int t = flags[own];
+    const int t = flags[(int64_t)r * cpitch + c];
     if (!(t == CS_WALL || t == CS_LID)) return;
     typename St::S v[9];
 #pragma unroll
@@ -254,7 +254,8 @@ __global__ void k_reduce_final(const double* __restrict__ partials, int nblocks,
 template <class St>
 __global__ __launch_bounds__(RED_THREADS) void k_sums(const typename St::S* __restrict__ f,
                                                       const uint8_t* __restrict__ flags, int w, int h, int pitch,
-                                                      int64_t P, Consts<typename St::T> k, double* partials)
+                                                      int cpitch, int64_t P, Consts<typename St::T> k,
+                                                      double* partials)
 {
     using T = typename St::T;
     double v[3] = {0.0, 0.0, 0.0};
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_sums(const typename St::S* __re
         const int r = (int)(d / w), c = (int)(d - (int64_t)r * w);
         const int64_t own = (int64_t)r * pitch + c;
         if (flags) {
-            const int t = flags[own];
+            const int t = flags[(int64_t)r * cpitch + c];
             if (t == CS_WALL || t == CS_LID) continue;
         }
         T B[9];
@@ -298,7 +299,7 @@ __global__ void k_snapshot(const typename St::S* __restrict__ f, int w, int h, i
 template <class St>
 __global__ __launch_bounds__(RED_THREADS) void k_residual(const typename St::S* __restrict__ f,
                                                           const uint8_t* __restrict__ flags, int w, int h, int pitch,
-                                                          int64_t P, Consts<typename St::T> k,
+                                                          int cpitch, int64_t P, Consts<typename St::T> k,
                                                           const double* __restrict__ snap, double* partials)
 {
     using T = typename St::T;
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_residual(const typename St::S* 
         const int r = (int)(d / w), c = (int)(d - (int64_t)r * w);
         const int64_t own = (int64_t)r * pitch + c;
         if (flags) {
-            const int t = flags[own];
+            const int t = flags[(int64_t)r * cpitch + c];
             if (t == CS_WALL || t == CS_LID) continue;
         }
         T B[9];
@@ -344,6 +345,9 @@ __global__ __launch_bounds__(256) void k_copy(const f32x4* __restrict__ src, f32
 // --------------------------------------------------------- host helpers ---
 
 static void check_lat(const cs_lattice* lat) { CS_REQUIRE(lat != nullptr, CS_ERR_ARG, "null lattice"); }
+
+// Elements in one population buffer (9 interleaved planes of h rows).
+static int64_t pop_elems(const cs_lattice* l) { return (int64_t)l->h * l->pitch; }
 
 template <class St>
 static const Consts<typename St::T>& consts_of(const cs_lattice* lat);
@@ -382,13 +386,14 @@ static int g_cpt_override = 0;
 template <class St, int EDGE, int MODE, int CPT, bool SR, int COLL>
 static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
 {
-    StepArgs<typename St::T> a;
+    StepArgs<typename St::T> a{};
     a.fin = fin;
     a.fout = fout;
     a.node = l->node;
     a.w = l->w;
     a.h = l->h;
     a.pitch = l->pitch;
+    a.cpitch = l->cpitch;
     a.plane = l->plane;
     a.tiles_x = (l->w + 64 * CPT - 1) / (64 * CPT);
     a.sr_key = sr_key(l->params.seed, l->steps);
@@ -396,7 +401,7 @@ static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
     a.k = consts_of<St>(l);
     const int tiles_y = (l->h + 3) / 4;
     const int nblocks = a.tiles_x * tiles_y;
-    if (9 * l->plane * (int64_t)sizeof(typename St::S) < ((int64_t)1 << 32))
+    if (pop_elems(l) * (int64_t)sizeof(typename St::S) < ((int64_t)1 << 32))
         hipLaunchKernelGGL((k_step<St, EDGE, MODE, CPT, SR, uint32_t, COLL>), dim3(nblocks), dim3(256), 0, l->stream,
                            a);
     else
@@ -405,7 +410,7 @@ static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
 }
 
 // Lattices without geometry and 4- or 8-byte storage take the vector kernel
-// (16 bytes per lane and plane, 16 rows per block) when a row splits into
+// (16 bytes per lane and plane, 4 rows per block) when a row splits into
 // vectors and the buffers allow 32-bit offsets.  16-bit storage keeps the
 // scalar kernel with 4 cells per lane, which measured faster (124 vs 144 us
 // per 4096^2 step with stochastic rounding).
@@ -415,16 +420,17 @@ static bool launch_step_vec(cs_lattice* l, const void* fin, void* fout)
     if constexpr (sizeof(typename St::S) < 4) {
         return false;
     } else {
-        constexpr int VB = 16, ROWS = 16;
+        constexpr int VB = 16, ROWS = 4;
         constexpr int V = VB / (int)sizeof(typename St::S);
-        if (l->w % V != 0 || 9 * l->plane * (int64_t)sizeof(typename St::S) >= ((int64_t)1 << 32)) return false;
-        StepArgs<typename St::T> a;
+        if (l->w % V != 0 || pop_elems(l) * (int64_t)sizeof(typename St::S) >= ((int64_t)1 << 32)) return false;
+        StepArgs<typename St::T> a{};
         a.fin = fin;
         a.fout = fout;
         a.node = nullptr;
         a.w = l->w;
         a.h = l->h;
         a.pitch = l->pitch;
+        a.cpitch = l->cpitch;
         a.plane = l->plane;
         a.tiles_x = (l->w + 64 * V - 1) / (64 * V);
         a.sr_key = sr_key(l->params.seed, l->steps);
@@ -540,7 +546,7 @@ static double residual_now(cs_lattice* l)
     dispatch_storage(l->storage, [&]<class St>() {
         hipLaunchKernelGGL((k_residual<St>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, l->stream,
                            (const typename St::S*)l->buf[l->cur], l->has_flags ? l->flags : nullptr, l->w, l->h,
-                           l->pitch, l->plane, consts_of<St>(l), (const double*)l->snap, l->partials);
+                           l->pitch, l->cpitch, l->plane, consts_of<St>(l), (const double*)l->snap, l->partials);
     });
     double v[2];
     reduce(l, 2, v);
@@ -559,16 +565,24 @@ static void snapshot(cs_lattice* l)
     l->has_snap = true;
 }
 
+// Host SoA (9 dense planes of h x w) <-> the device's row-interleaved planes:
+// one strided 2D copy per plane.
 static void upload_pitched(cs_lattice* l, void* dst, const void* src, size_t elem)
 {
-    CS_HIP(hipMemcpy2DAsync(dst, (size_t)l->pitch * elem, src, (size_t)l->w * elem, (size_t)l->w * elem,
-                            (size_t)9 * l->h, hipMemcpyHostToDevice, l->stream));
+    const size_t hplane = (size_t)l->w * l->h * elem;
+    for (int i = 0; i < 9; ++i)
+        CS_HIP(hipMemcpy2DAsync((char*)dst + (size_t)i * l->plane * elem, (size_t)l->pitch * elem,
+                                (const char*)src + i * hplane, (size_t)l->w * elem, (size_t)l->w * elem,
+                                (size_t)l->h, hipMemcpyHostToDevice, l->stream));
 }
 
 static void download_pitched(cs_lattice* l, void* dst, const void* src, size_t elem)
 {
-    CS_HIP(hipMemcpy2DAsync(dst, (size_t)l->w * elem, src, (size_t)l->pitch * elem, (size_t)l->w * elem,
-                            (size_t)9 * l->h, hipMemcpyDeviceToHost, l->stream));
+    const size_t hplane = (size_t)l->w * l->h * elem;
+    for (int i = 0; i < 9; ++i)
+        CS_HIP(hipMemcpy2DAsync((char*)dst + i * hplane, (size_t)l->w * elem,
+                                (const char*)src + (size_t)i * l->plane * elem, (size_t)l->pitch * elem,
+                                (size_t)l->w * elem, (size_t)l->h, hipMemcpyDeviceToHost, l->stream));
 }
 
 }  // namespace cs
@@ -622,10 +636,12 @@ extern "C" int cs_lattice_create(cs_ctx* ctx, int w, int h, int storage, cs_latt
         l->h = h;
         l->storage = storage;
         const int align = (int)(256 / storage_bytes(storage));  // 256-byte rows
-        l->pitch = (w + align - 1) / align * align;
-        l->plane = (int64_t)l->pitch * h;
+        const int wpad = (w + align - 1) / align * align;
+        l->cpitch = wpad;
+        l->plane = wpad;       // row-interleaved populations (cs_internal.hpp)
+        l->pitch = 9 * wpad;
         try {
-            const size_t bytes = (size_t)9 * l->plane * storage_bytes(storage);
+            const size_t bytes = (size_t)pop_elems(l) * storage_bytes(storage);
             CS_HIP(hipMalloc(&l->buf[0], bytes));
             CS_HIP(hipMalloc(&l->buf[1], bytes));
             CS_HIP(hipMemset(l->buf[0], 0, bytes));
@@ -710,7 +726,7 @@ extern "C" int cs_lattice_set_params(cs_lattice* l, const cs_params* p)
         set_consts(l, *p);
         if (edge_changed && l->has_flags) {
             dim3 g((unsigned)((l->w + 255) / 256), (unsigned)l->h);
-            hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, l->stream, l->flags, l->node, l->w, l->h, l->pitch,
+            hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, l->stream, l->flags, l->node, l->w, l->h, l->cpitch,
                                (int)l->params.edge);
             CS_HIP(hipGetLastError());
         }
@@ -729,14 +745,15 @@ extern "C" int cs_lattice_set_flags(cs_lattice* l, const uint8_t* flags)
         for (int64_t d = 0; d < cells; ++d)
             CS_REQUIRE(flags[d] <= CS_OUTLET, CS_ERR_ARG, "flag value out of range");
         if (!l->flags) {
-            CS_HIP(hipMalloc(&l->flags, (size_t)l->plane));
-            CS_HIP(hipMalloc(&l->node, sizeof(uint32_t) * (size_t)l->plane));
-            CS_HIP(hipMemsetAsync(l->flags, 0, (size_t)l->plane, l->stream));
+            const size_t cells = (size_t)l->cpitch * l->h;
+            CS_HIP(hipMalloc(&l->flags, cells));
+            CS_HIP(hipMalloc(&l->node, sizeof(uint32_t) * cells));
+            CS_HIP(hipMemsetAsync(l->flags, 0, cells, l->stream));
         }
-        CS_HIP(hipMemcpy2DAsync(l->flags, (size_t)l->pitch, flags, (size_t)l->w, (size_t)l->w, (size_t)l->h,
+        CS_HIP(hipMemcpy2DAsync(l->flags, (size_t)l->cpitch, flags, (size_t)l->w, (size_t)l->w, (size_t)l->h,
                                 hipMemcpyHostToDevice, l->stream));
         dim3 g((unsigned)((l->w + 255) / 256), (unsigned)l->h);
-        hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, l->stream, l->flags, l->node, l->w, l->h, l->pitch,
+        hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, l->stream, l->flags, l->node, l->w, l->h, l->cpitch,
                            (int)l->params.edge);
         CS_HIP(hipGetLastError());
         CS_HIP(hipStreamSynchronize(l->stream));
@@ -752,7 +769,7 @@ extern "C" int cs_lattice_upload_f32(cs_lattice* l, const float* soa)
         if (l->storage == CS_F32) {
             upload_pitched(l, l->buf[l->cur], soa, 4);
         } else {
-            DevBuf tmp(sizeof(float) * 9 * (size_t)l->plane);
+            DevBuf tmp(sizeof(float) * (size_t)pop_elems(l));
             upload_pitched(l, tmp.p, soa, 4);
             dispatch_storage(l->storage, [&]<class St>() {
                 hipLaunchKernelGGL((k_from_f32<St>), cell_grid(l), dim3(256), 0, l->stream, (const float*)tmp.p,
@@ -773,7 +790,7 @@ extern "C" int cs_lattice_download_f32(cs_lattice* l, float* soa)
         if (l->storage == CS_F32) {
             download_pitched(l, soa, l->buf[l->cur], 4);
         } else {
-            DevBuf tmp(sizeof(float) * 9 * (size_t)l->plane);
+            DevBuf tmp(sizeof(float) * (size_t)pop_elems(l));
             dispatch_storage(l->storage, [&]<class St>() {
                 hipLaunchKernelGGL((k_to_f32<St>), cell_grid(l), dim3(256), 0, l->stream,
                                    (const typename St::S*)l->buf[l->cur], (float*)tmp.p, l->w, l->h, l->pitch, l->plane,
@@ -883,7 +900,7 @@ extern "C" int cs_lattice_init_nonequilibrium(cs_lattice* l, const float* rho, c
             const TT* p = (const TT*)dev.p;
             hipLaunchKernelGGL((k_init_neq<St>), cell_grid(l), dim3(256), 0, l->stream, p, p + n, p + 2 * n,
                                l->has_flags ? (const uint8_t*)l->flags : nullptr, (typename St::S*)l->buf[l->cur],
-                               l->w, l->h, l->pitch, l->plane, consts_of<St>(l),
+                               l->w, l->h, l->pitch, l->cpitch, l->plane, consts_of<St>(l),
                                NeqArgs<TT>{(TT)l->params.tau, (TT)l->params.lid_ux, (TT)l->params.lid_uy,
                                            l->params.edge == CS_EDGE_PERIODIC ? 1 : 0});
             CS_HIP(hipGetLastError());
@@ -966,7 +983,7 @@ extern "C" int cs_lattice_bounce_back(cs_lattice* l)
             if constexpr (std::is_same_v<St, StF32> || std::is_same_v<St, StF64>) {
                 hipLaunchKernelGGL((k_bounce_back<St>), cell_grid(l), dim3(256), 0, l->stream,
                                    (typename St::S*)l->buf[l->cur], (const uint8_t*)l->flags, l->w, l->h, l->pitch,
-                                   l->plane);
+                                   l->cpitch, l->plane);
             }
         });
         CS_HIP(hipGetLastError());
@@ -1159,7 +1176,7 @@ extern "C" int cs_lattice_sums_f64(cs_lattice* l, double out[3])
         dispatch_storage(l->storage, [&]<class St>() {
             hipLaunchKernelGGL((k_sums<St>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, l->stream,
                                (const typename St::S*)l->buf[l->cur], l->has_flags ? l->flags : nullptr, l->w, l->h,
-                               l->pitch, l->plane, consts_of<St>(l), l->partials);
+                               l->pitch, l->cpitch, l->plane, consts_of<St>(l), l->partials);
         });
         CS_HIP(hipGetLastError());
         reduce(l, 3, out);

@@ -176,7 +176,7 @@ struct NeqArgs {
 template <class St>
 __global__ void k_init_neq(const typename St::T* __restrict__ rho, const typename St::T* __restrict__ ux,
                            const typename St::T* __restrict__ uy, const uint8_t* __restrict__ flags,
-                           typename St::S* __restrict__ f, int w, int h, int pitch, int64_t P,
+                           typename St::S* __restrict__ f, int w, int h, int pitch, int cpitch, int64_t P,
                            Consts<typename St::T> k, NeqArgs<typename St::T> a)
 {
     using T = typename St::T;
@@ -193,7 +193,7 @@ __global__ void k_init_neq(const typename St::T* __restrict__ rho, const typenam
             return (int)NB_NONE;
         }
         idx = (int64_t)rr * w + cc;
-        t = flags ? (int)flags[(int64_t)rr * pitch + cc] : (int)CS_FLUID;
+        t = flags ? (int)flags[(int64_t)rr * cpitch + cc] : (int)CS_FLUID;
         return (t == CS_WALL || t == CS_LID) ? (int)NB_WALL : (int)NB_FLUID;
     };
     T F[9];

@@ -170,6 +170,7 @@ struct StepArgs {
     void* fout;
     const uint32_t* node;
     int w, h, pitch;
+    int cpitch;          // row pitch of node
     int tiles_x;
     int64_t plane;
     uint32_t sr_key;     // stochastic rounding key of this step
@@ -437,8 +438,8 @@ __device__ __forceinline__ void zou_he_outlet(T (&B)[9], T rho)
 // offset type (uint32_t when the 9-plane buffer is < 4 GiB, else int64_t).
 template <class St, int EDGE, int MODE, class Off = int64_t>
 __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, const uint32_t* __restrict__ node,
-                                      int w, int h, int pitch, int64_t P, const Consts<typename St::T>& k, int r,
-                                      int c, typename St::T (&B)[9])
+                                      int w, int h, int pitch, int cpitch, int64_t P,
+                                      const Consts<typename St::T>& k, int r, int c, typename St::T (&B)[9])
 {
     using T = typename St::T;
     int rows[3], cols[3];
@@ -468,7 +469,7 @@ __device__ __forceinline__ int gather(const typename St::S* __restrict__ fin, co
     uint32_t info = 0;
     int type = CS_FLUID;
     if (MODE != MODE_FLUID) {
-        info = node[own];
+        info = node[(Off)r * (Off)cpitch + (Off)c];
         type = (int)(info >> NODE_TYPE_SHIFT);
     }
     const bool solid = (type == CS_WALL || type == CS_LID);
@@ -579,7 +580,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
     for (int j = 0; j < CPT; ++j) {
         const int c = c0 + 64 * j;
         const int cc = c < a.w ? c : a.w - 1;
-        type[j] = gather<St, EDGE, MODE, Off>(fin, a.node, a.w, a.h, a.pitch, a.plane, a.k, r, cc, B[j]);
+        type[j] = gather<St, EDGE, MODE, Off>(fin, a.node, a.w, a.h, a.pitch, a.cpitch, a.plane, a.k, r, cc, B[j]);
     }
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {

@@ -134,7 +134,7 @@ __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f),
 template <int CH>
 __global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__ sval, const float* __restrict__ phase,
                                                        const float* __restrict__ stats,
-                                                       const uint8_t* __restrict__ flags, int w, int h, int pitch,
+                                                       const uint8_t* __restrict__ flags, int w, int h, int cpitch,
                                                        int mode, int overlay, uint8_t* __restrict__ rgb)
 {
     const int64_t n = (int64_t)w * h;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__
     uint8_t o[3] = {to_byte(r), to_byte(g), to_byte(b)};
     if (overlay && flags) {
         const int rr = (int)(d / w), cc = (int)(d - (int64_t)rr * w);
-        const int t = flags[(int64_t)rr * pitch + cc];
+        const int t = flags[(int64_t)rr * cpitch + cc];
         if (t == CS_WALL || t == CS_LID) {
             o[0] = 0;
             o[1] = 0;
@@ -228,10 +228,10 @@ void render_enqueue(cs_lattice* l, int mode, int overlay, uint8_t* d_out, int ch
     const uint8_t* fl = l->has_flags ? (const uint8_t*)l->flags : nullptr;
     if (channels == 4)
         hipLaunchKernelGGL(k_render_pixels<4>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, l->stream, r.sval,
-                           r.phase, r.stats, fl, l->w, l->h, l->pitch, mode, overlay, d_out);
+                           r.phase, r.stats, fl, l->w, l->h, l->cpitch, mode, overlay, d_out);
     else
         hipLaunchKernelGGL(k_render_pixels<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, l->stream, r.sval,
-                           r.phase, r.stats, fl, l->w, l->h, l->pitch, mode, overlay, d_out);
+                           r.phase, r.stats, fl, l->w, l->h, l->cpitch, mode, overlay, d_out);
     CS_HIP(hipGetLastError());
 }
 

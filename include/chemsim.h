@@ -176,8 +176,10 @@ int cs_lattice_collide(cs_lattice* lat);
 /* State::time (lbm.rs:671, advanced at :713). */
 int cs_lattice_time(const cs_lattice* lat, double* t);
 int cs_lattice_sync(cs_lattice* lat);
-/* Device address of the current populations: 9 planes of h rows with a row
- * pitch of *pitch elements, plane stride *plane elements. */
+/* Device address of the current populations: element (i, r, c) at
+ * i * (*plane) + r * (*pitch) + c.  The planes are row-interleaved: row r of
+ * plane i follows row r of plane i - 1 (*plane = padded row length, *pitch =
+ * 9 * *plane). */
 int cs_lattice_device_ptr(cs_lattice* lat, void** f, int* pitch, int64_t* plane);
 /* HIP stream the lattice's kernels run on (hipStream_t). */
 int cs_lattice_stream_handle(cs_lattice* lat, void** stream);
