@@ -32,8 +32,9 @@ namespace cs {
 template <class T>
 struct BatchLat {
     int w, h, pitch, mode;  // mode bits: 1 periodic, 2 half-way, 4 has flags, 8 stochastic rounding
-    int64_t plane;          // plane stride (elements)
-    int64_t off;            // element offset of buffer 0 in the arena (buffer 1 follows, 9 planes later)
+    int cpitch;             // row pitch of the node words (elements)
+    int64_t plane;          // plane stride (elements); populations row-interleaved, pitch = 9 * plane
+    int64_t off;            // element offset of buffer 0 in the arena (buffer 1 follows, h * pitch later)
     int64_t node_off;       // element offset in the node arena
     uint32_t seed;
     Consts<T> k;
@@ -73,7 +74,7 @@ __device__ __forceinline__ void batch_lattice_step(const BatchLat<typename St::T
             for (int j = 0; j < CPT; ++j) {
                 const int c = c0 + 64 * j;
                 const int cc = c < L.w ? c : L.w - 1;
-                type[j] = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.pitch, L.plane, L.k, r, cc, B[j]);
+                type[j] = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.cpitch, L.plane, L.k, r, cc, B[j]);
             }
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
@@ -112,8 +113,8 @@ __global__ __launch_bounds__(1024) void k_batch(const BatchLat<typename St::T>* 
         const uint64_t step0 = state[li].steps;
         for (int s = 0; s < steps; ++s) {
             const int cur = (cur0 + s) & 1;
-            const S* fin = base + (int64_t)cur * 9 * L.plane;
-            S* fout = base + (int64_t)(cur ^ 1) * 9 * L.plane;
+            const S* fin = base + (int64_t)cur * L.h * L.pitch;
+            S* fout = base + (int64_t)(cur ^ 1) * L.h * L.pitch;
             const uint32_t key = sr_key(L.seed, step0 + (uint64_t)s);
             batch_lattice_step<St, EDGE, MODE, SR, CPT>(L, fin, fout, node, key, wave, nwaves, lane);
             __syncthreads();  // step s fully stored before step s + 1 reads it
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(256) void k_batch_snapshot(const BatchLat<typename 
     const int li = blockIdx.x;
     if (li >= n) return;
     const BatchLat<T> L = lats[li];
-    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)state[li].cur * 9 * L.plane;
+    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)state[li].cur * L.h * L.pitch;
     const int64_t cells = (int64_t)L.w * L.h;
     float* s = snap + snap_off[li];
     for (int64_t d = threadIdx.x; d < cells; d += 256) {
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(256) void k_batch_residual(const BatchLat<typename 
     const int li = blockIdx.x;
     if (li >= n) return;
     const BatchLat<T> L = lats[li];
-    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)state[li].cur * 9 * L.plane;
+    const typename St::S* f = (const typename St::S*)arena + L.off + (int64_t)state[li].cur * L.h * L.pitch;
     const int64_t cells = (int64_t)L.w * L.h;
     const float* s = snap + snap_off[li];
     double num = 0.0, den = 0.0;
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(256) void k_batch_residual(const BatchLat<typename 
         const int r = (int)(d / L.w), c = (int)(d - (int64_t)r * L.w);
         const int64_t own = (int64_t)r * L.pitch + c;
         if (L.mode & BM_FLAGS) {
-            const int t = (int)(nodes[L.node_off + own] >> NODE_TYPE_SHIFT);
+            const int t = (int)(nodes[L.node_off + (int64_t)r * L.cpitch + c] >> NODE_TYPE_SHIFT);
             if (t == CS_WALL || t == CS_LID) continue;
         }
         T B[9];
@@ -215,12 +216,12 @@ __global__ __launch_bounds__(256) void k_batch_tiles(const BatchLat<typename St:
     const int c = tx * 64 + (int)(threadIdx.x & 63);
     const int cur = (state[li].cur + s) & 1;
     const uint32_t key = sr_key(L.seed, state[li].steps + (uint64_t)s);
-    const S* fin = (const S*)arena + L.off + (int64_t)cur * 9 * L.plane;
-    S* fout = (S*)arena + L.off + (int64_t)(cur ^ 1) * 9 * L.plane;
+    const S* fin = (const S*)arena + L.off + (int64_t)cur * L.h * L.pitch;
+    S* fout = (S*)arena + L.off + (int64_t)(cur ^ 1) * L.h * L.pitch;
     const uint32_t* node = nodes + L.node_off;
     T B[9];
     const int cc = c < L.w ? c : L.w - 1;
-    const int type = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.pitch, L.plane, L.k, r, cc, B);
+    const int type = gather<St, EDGE, MODE, uint32_t>(fin, node, L.w, L.h, L.pitch, L.cpitch, L.plane, L.k, r, cc, B);
     if (c < L.w)
         finish<St, MODE, SR, uint32_t>(fin, fout, L.pitch, L.plane, L.k, r, c, type, B, key, (uint32_t)(r * L.w + c));
 }
@@ -240,7 +241,7 @@ struct cs_batch {
     cs_ctx* ctx = nullptr;
     int n = 0, storage = CS_F32;
     size_t esize = 4;
-    std::vector<int> w, h, pitch;
+    std::vector<int> w, h, pitch, cpitch;
     std::vector<int64_t> plane, off, node_off;
     std::vector<cs_params> params;
     std::vector<uint8_t> has_flags;
@@ -304,6 +305,7 @@ static BatchLat<T> describe(const cs_batch* b, int i)
     L.w = b->w[i];
     L.h = b->h[i];
     L.pitch = b->pitch[i];
+    L.cpitch = b->cpitch[i];
     const cs_params& p = b->params[i];
     L.mode = (p.edge == CS_EDGE_PERIODIC ? BM_PERIODIC : 0) | (p.boundary == CS_BB_HALFWAY ? BM_HALFWAY : 0) |
              (b->has_flags[i] ? BM_FLAGS : 0) | (p.rounding == CS_ROUND_STOCHASTIC ? BM_SR : 0);
@@ -509,13 +511,13 @@ static void launch_per_lattice(cs_batch* b, int steps)
                 const BatchLat<T>& d = L[(size_t)i];
                 StepArgs<T> a{};
                 typename St::S* base = (typename St::S*)b->arena + d.off;
-                a.fin = base + (int64_t)cur * 9 * d.plane;
-                a.fout = base + (int64_t)(cur ^ 1) * 9 * d.plane;
+                a.fin = base + (int64_t)cur * d.h * d.pitch;
+                a.fout = base + (int64_t)(cur ^ 1) * d.h * d.pitch;
                 a.node = b->nodes + d.node_off;
                 a.w = d.w;
                 a.h = d.h;
                 a.pitch = d.pitch;
-                a.cpitch = d.pitch;  // batch lattices are planar: node rows share the population pitch
+                a.cpitch = d.cpitch;
                 a.plane = d.plane;
                 a.tiles_x = (d.w + 63) / 64;
                 a.sr_key = sr_key(d.seed, b->state[(size_t)i].steps + (uint64_t)s);
@@ -555,7 +557,20 @@ static void launch_per_lattice(cs_batch* b, int steps)
 static void* lattice_buf(cs_batch* b, int i)
 {
     return (char*)b->arena + (size_t)b->off[(size_t)i] * b->esize +
-           (size_t)b->state[(size_t)i].cur * 9 * b->plane[(size_t)i] * b->esize;
+           (size_t)b->state[(size_t)i].cur * b->h[(size_t)i] * b->pitch[(size_t)i] * b->esize;
+}
+
+// Host SoA (9 dense planes of h x w) <-> lattice i's row-interleaved planes,
+// one strided 2D copy per plane on the batch stream.
+static void copy_planes(cs_batch* b, size_t i, void* dst, const void* src, size_t elem, hipMemcpyKind kind)
+{
+    const size_t hplane = (size_t)b->w[i] * b->h[i] * elem, dplane = (size_t)b->plane[i] * elem;
+    const bool up = kind == hipMemcpyHostToDevice;
+    for (int q = 0; q < 9; ++q)
+        CS_HIP(hipMemcpy2DAsync((char*)dst + q * (up ? dplane : hplane), up ? (size_t)b->pitch[i] * elem : b->w[i] * elem,
+                                (const char*)src + q * (up ? hplane : dplane),
+                                up ? b->w[i] * elem : (size_t)b->pitch[i] * elem, (size_t)b->w[i] * elem,
+                                (size_t)b->h[i], kind, b->stream));
 }
 
 }  // namespace cs
@@ -582,8 +597,9 @@ extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const i
             const int p = (w[i] + align - 1) / align * align;
             b->w.push_back(w[i]);
             b->h.push_back(h[i]);
-            b->pitch.push_back(p);
-            b->plane.push_back((int64_t)p * h[i]);
+            b->pitch.push_back(9 * p);  // row-interleaved populations (as the single lattice)
+            b->cpitch.push_back(p);
+            b->plane.push_back(p);
             b->off.push_back(acc);
             b->node_off.push_back(nacc);
             acc += 2 * 9 * (int64_t)p * h[i];
@@ -684,12 +700,12 @@ extern "C" int cs_batch_set_flags(cs_batch* b, int index, const uint8_t* flags)
         const int64_t cells = (int64_t)b->w[i] * b->h[i];
         for (int64_t d = 0; d < cells; ++d) CS_REQUIRE(flags[d] <= CS_OUTLET, CS_ERR_ARG, "flag value out of range");
         uint8_t* tmp = nullptr;
-        CS_HIP(hipMalloc(&tmp, (size_t)b->plane[i]));
-        CS_HIP(hipMemcpy2DAsync(tmp, (size_t)b->pitch[i], flags, (size_t)b->w[i], (size_t)b->w[i], (size_t)b->h[i],
+        CS_HIP(hipMalloc(&tmp, (size_t)b->cpitch[i] * b->h[i]));
+        CS_HIP(hipMemcpy2DAsync(tmp, (size_t)b->cpitch[i], flags, (size_t)b->w[i], (size_t)b->w[i], (size_t)b->h[i],
                                 hipMemcpyHostToDevice, b->stream));
         dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
         hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, b->stream, tmp, b->nodes + b->node_off[i], b->w[i], b->h[i],
-                           b->pitch[i], (int)b->params[i].edge);
+                           b->cpitch[i], (int)b->params[i].edge);
         hipError_t e = hipGetLastError();
         hipError_t e2 = hipStreamSynchronize(b->stream);
         (void)hipFree(tmp);
@@ -739,9 +755,8 @@ extern "C" int cs_batch_upload_f32(cs_batch* b, int index, const float* soa)
         CS_REQUIRE(soa, CS_ERR_ARG, "null data");
         const size_t i = (size_t)index;
         float* tmp = nullptr;
-        CS_HIP(hipMalloc(&tmp, sizeof(float) * 9 * (size_t)b->plane[i]));
-        CS_HIP(hipMemcpy2DAsync(tmp, (size_t)b->pitch[i] * 4, soa, (size_t)b->w[i] * 4, (size_t)b->w[i] * 4,
-                                (size_t)9 * b->h[i], hipMemcpyHostToDevice, b->stream));
+        CS_HIP(hipMalloc(&tmp, sizeof(float) * (size_t)b->h[i] * b->pitch[i]));
+        copy_planes(b, i, tmp, soa, 4, hipMemcpyHostToDevice);
         dispatch_batch_storage(b->storage, [&]<class St>() {
             dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
             hipLaunchKernelGGL((k_from_f32<St>), g, dim3(256), 0, b->stream, (const float*)tmp,
@@ -763,7 +778,7 @@ extern "C" int cs_batch_download_f32(cs_batch* b, int index, float* soa)
         CS_REQUIRE(soa, CS_ERR_ARG, "null data");
         const size_t i = (size_t)index;
         float* tmp = nullptr;
-        CS_HIP(hipMalloc(&tmp, sizeof(float) * 9 * (size_t)b->plane[i]));
+        CS_HIP(hipMalloc(&tmp, sizeof(float) * (size_t)b->h[i] * b->pitch[i]));
         dispatch_batch_storage(b->storage, [&]<class St>() {
             dim3 g((unsigned)((b->w[i] + 255) / 256), (unsigned)b->h[i]);
             hipLaunchKernelGGL((k_to_f32<St>), g, dim3(256), 0, b->stream,
@@ -771,8 +786,12 @@ extern "C" int cs_batch_download_f32(cs_batch* b, int index, float* soa)
                                b->plane[i], batch_consts<typename St::T>(b, index));
         });
         hipError_t e = hipGetLastError();
-        hipError_t e1 = hipMemcpy2DAsync(soa, (size_t)b->w[i] * 4, tmp, (size_t)b->pitch[i] * 4, (size_t)b->w[i] * 4,
-                                         (size_t)9 * b->h[i], hipMemcpyDeviceToHost, b->stream);
+        hipError_t e1 = hipSuccess;
+        try {
+            copy_planes(b, i, soa, tmp, 4, hipMemcpyDeviceToHost);
+        } catch (const Error&) {
+            e1 = hipErrorUnknown;
+        }
         hipError_t e2 = hipStreamSynchronize(b->stream);
         (void)hipFree(tmp);
         CS_HIP(e);
@@ -787,8 +806,7 @@ extern "C" int cs_batch_download_raw(cs_batch* b, int index, void* data)
         check_batch(b, index);
         CS_REQUIRE(data, CS_ERR_ARG, "null data");
         const size_t i = (size_t)index;
-        CS_HIP(hipMemcpy2DAsync(data, (size_t)b->w[i] * b->esize, lattice_buf(b, index), (size_t)b->pitch[i] * b->esize,
-                                (size_t)b->w[i] * b->esize, (size_t)9 * b->h[i], hipMemcpyDeviceToHost, b->stream));
+        copy_planes(b, i, data, lattice_buf(b, index), b->esize, hipMemcpyDeviceToHost);
         CS_HIP(hipStreamSynchronize(b->stream));
     });
 }
