@@ -191,17 +191,18 @@ struct RenderScratch {
 
 static RenderScratch render_scratch(cs_lattice* l)
 {
-    const int64_t n = (int64_t)l->w * l->h;
-    if (!l->render_buf) {
-        CS_HIP(hipMalloc(&l->render_buf, sizeof(float) * 2 * (size_t)n + sizeof(double) * 2 * RENDER_BLOCKS +
-                                             16 * sizeof(float) + 3 * (size_t)n));
-    }
+    const size_t n = (size_t)l->w * l->h;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };  // 256-byte aligned sections
+    const size_t o_phase = up(sizeof(float) * n), o_part = o_phase + up(sizeof(float) * n);
+    const size_t o_stats = o_part + up(sizeof(double) * 2 * RENDER_BLOCKS), o_rgb = o_stats + 256;
+    if (!l->render_buf) CS_HIP(hipMalloc(&l->render_buf, o_rgb + up(3 * n)));
+    char* base = (char*)l->render_buf;
     RenderScratch r;
-    r.sval = (float*)l->render_buf;
-    r.phase = r.sval + n;
-    r.partials = (double*)(r.phase + n);
-    r.stats = (float*)(r.partials + 2 * RENDER_BLOCKS);
-    r.rgb = (uint8_t*)(r.stats + 16);
+    r.sval = (float*)base;
+    r.phase = (float*)(base + o_phase);
+    r.partials = (double*)(base + o_part);
+    r.stats = (float*)(base + o_stats);
+    r.rgb = (uint8_t*)(base + o_rgb);
     return r;
 }
 
