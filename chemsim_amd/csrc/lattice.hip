@@ -439,10 +439,10 @@ static void launch_step_sr(cs_lattice* l, const void* fin, void* fout)
 {
     if (MODE == MODE_FLUID && g_cpt_override == 0 && launch_step_vec<St, EDGE, SR, COLL>(l, fin, fout)) return;
     int cpt = g_cpt_override;
-    // 16-bit storage with geometry: 4 cells per lane (measured in the planar
-    // layout); everything else one (16-bit fluid lattices in the row-interleaved
-    // layout: 108 us at 4096^2 against 112 us with 4 cells per lane).
-    if (cpt == 0) cpt = ((l->storage == CS_BF16DEV || l->storage == CS_F16DEV) && MODE != MODE_FLUID) ? 4 : 1;
+    // 16-bit storage: 4 cells per lane (with stochastic rounding on the
+    // Taylor-Green benchmark 143.6 against 140.4 GLUPS for one cell per lane at
+    // 4096^2, 147.8 against 141.3 at 8192^2).
+    if (cpt == 0) cpt = (l->storage == CS_BF16DEV || l->storage == CS_F16DEV) ? 4 : 1;
     if (cpt >= 4) launch_step_cpt<St, EDGE, MODE, 4, SR, COLL>(l, fin, fout);
     else if (cpt == 2) launch_step_cpt<St, EDGE, MODE, 2, SR, COLL>(l, fin, fout);
     else launch_step_cpt<St, EDGE, MODE, 1, SR, COLL>(l, fin, fout);
