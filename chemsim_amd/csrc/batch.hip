@@ -195,10 +195,10 @@ __global__ __launch_bounds__(256) void k_batch_residual(const BatchLat<typename 
 }
 
 // One step of every lattice in a tile list: block b updates tile tiles[b]
-// (4 rows x 64 columns, one cell per thread) of its lattice with the
+// (TR rows x 64 columns, one cell per thread) of its lattice with the
 // single-lattice gather/finish.  `s` is the step's index within the call.
-template <class St, int EDGE, int MODE, bool SR>
-__global__ __launch_bounds__(256) void k_batch_tiles(const BatchLat<typename St::T>* __restrict__ lats,
+template <class St, int EDGE, int MODE, bool SR, int TR>
+__global__ __launch_bounds__(64 * TR) void k_batch_tiles(const BatchLat<typename St::T>* __restrict__ lats,
                                                      const TileRef* __restrict__ tiles, void* __restrict__ arena,
                                                      const uint32_t* __restrict__ nodes,
                                                      const LatState* __restrict__ state, int s)
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_batch_tiles(const BatchLat<typename St:
     const BatchLat<T> L = lats[li];
     const int tiles_x = (L.w + 63) >> 6;
     const int ty = t / tiles_x, tx = t - ty * tiles_x;
-    const int r = ty * 4 + (int)(threadIdx.x >> 6);
+    const int r = ty * TR + (int)(threadIdx.x >> 6);
     if (r >= L.h) return;
     const int c = tx * 64 + (int)(threadIdx.x & 63);
     const int cur = (state[li].cur + s) & 1;
@@ -416,6 +416,13 @@ static void advance_state(cs_batch* b, int steps)
     }
 }
 
+// Rows per tile of the tiles launch: block_threads / 64 (4, 8 or 16).
+static int tile_rows(const cs_batch* b)
+{
+    const int bt = b->tuning.block_threads;
+    return bt >= 1024 ? 16 : (bt >= 512 ? 8 : 4);
+}
+
 // Tile list of the active lattices, grouped like upload_order; returns
 // (group, first tile, tile count) triples into d_tiles.
 static std::vector<std::array<int, 3>> upload_tiles(cs_batch* b, const std::vector<std::array<int, 3>>& groups)
@@ -427,7 +434,8 @@ static std::vector<std::array<int, 3>> upload_tiles(cs_batch* b, const std::vect
         const int first = (int)tiles.size();
         for (int k = g[1]; k < g[1] + g[2]; ++k) {
             const int li = b->host_order[(size_t)k];
-            const int nt = ((b->w[(size_t)li] + 63) / 64) * ((b->h[(size_t)li] + 3) / 4);
+            const int tr = tile_rows(b);
+            const int nt = ((b->w[(size_t)li] + 63) / 64) * ((b->h[(size_t)li] + tr - 1) / tr);
             for (int t = 0; t < nt; ++t) tiles.push_back(TileRef{li, t});
         }
         out.push_back({g[0], first, (int)tiles.size() - first});
@@ -445,9 +453,17 @@ static void launch_tiles_group(cs_batch* b, int group, int first, int count, int
     using T = typename St::T;
     const auto* lats = (const BatchLat<T>*)b->d_lats;
     const TileRef* tiles = b->d_tiles + first;
+    const int tr = tile_rows(b);
     auto go = [&]<int EDGE, int MODE, bool SR>() {
-        hipLaunchKernelGGL((k_batch_tiles<St, EDGE, MODE, SR>), dim3((unsigned)count), dim3(256), 0, b->stream, lats,
-                           tiles, b->arena, b->nodes, b->d_state, s);
+        if (tr == 16)
+            hipLaunchKernelGGL((k_batch_tiles<St, EDGE, MODE, SR, 16>), dim3((unsigned)count), dim3(1024), 0, b->stream,
+                               lats, tiles, b->arena, b->nodes, b->d_state, s);
+        else if (tr == 8)
+            hipLaunchKernelGGL((k_batch_tiles<St, EDGE, MODE, SR, 8>), dim3((unsigned)count), dim3(512), 0, b->stream,
+                               lats, tiles, b->arena, b->nodes, b->d_state, s);
+        else
+            hipLaunchKernelGGL((k_batch_tiles<St, EDGE, MODE, SR, 4>), dim3((unsigned)count), dim3(256), 0, b->stream,
+                               lats, tiles, b->arena, b->nodes, b->d_state, s);
     };
     auto go_sr = [&]<int EDGE, int MODE>() {
         if constexpr (is_16bit<St>()) {
@@ -917,7 +933,8 @@ extern "C" int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* 
             }
         };
         try {
-            trial(cs_batch_tuning{256, 1, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_TILES});
+            for (int blk : {256, 512, 1024})
+                trial(cs_batch_tuning{blk, 1, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_TILES});
             for (int blk : {256, 512, 1024})
                 for (int wpc : {1, 2, 4, 8})
                     for (int cpt : {1, 2})

@@ -261,7 +261,7 @@ enum cs_batch_order { CS_ORDER_LARGEST_FIRST = 0, CS_ORDER_GIVEN = 1, CS_ORDER_S
 /* CS_LAUNCH_PERSISTENT: workgroups take whole lattices from a queue and run
  * all steps of a call on them (block_threads, workgroups_per_cu,
  * cells_per_thread, order apply).  CS_LAUNCH_TILES: one launch per step over
- * the 4 x 64 tiles of every lattice. */
+ * the (block_threads / 64) x 64 tiles of every lattice. */
 enum cs_batch_launch { CS_LAUNCH_PERSISTENT = 0, CS_LAUNCH_TILES = 1 };
 typedef struct cs_batch_tuning {
     int32_t block_threads;     /* 256, 512 or 1024 */

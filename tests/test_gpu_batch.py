@@ -69,11 +69,13 @@ def test_batch_equals_per_lattice_launches(ctx, storage):
     tunings = [dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0),
                dict(block_threads=1024, workgroups_per_cu=2, cells_per_thread=2, order=1),
                dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=capi.CS_LAUNCH_TILES),
+               dict(block_threads=512, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=capi.CS_LAUNCH_TILES),
+               dict(block_threads=1024, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=capi.CS_LAUNCH_TILES),
                dict(block_threads=512, workgroups_per_cu=8, cells_per_thread=2, order=2)]
     a, _ = build_mixed(ctx, storage)
     p, _ = build_mixed(ctx, storage)
     with a, p:
-        p.step_per_lattice(12)
+        p.step_per_lattice(3 * len(tunings))
         for t in tunings:
             a.tuning = t
             a.step(3)
