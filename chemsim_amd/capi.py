@@ -177,6 +177,7 @@ SIGNATURES = {
     "cs_gemm_bf16": [_P, _I, _I, _I, _P, _P, _P, _FP],
     "cs_bench_copy": [_P, _I64, _I, _FP],
     "cs_set_cells_per_thread": [_I],
+    "cs_set_step_graphs": [_I],
 }
 
 OPTIONAL = set()

@@ -160,4 +160,9 @@ struct cs_lattice {
     double time = 0.0;
     uint64_t steps = 0;         // fused steps taken (keys the stochastic rounding)
     void* render_buf = nullptr; // render pass scratch (fields, partials, RGB)
+    // Launch-bound lattices replay captured runs of fused steps: one graph per
+    // starting buffer parity, valid for the parameters, geometry and launch
+    // shape they were captured with (cleared when any of them changes).
+    hipGraphExec_t step_graph[2] = {nullptr, nullptr};
+    int graph_cpt = -1;         // cells-per-thread override at capture
 };

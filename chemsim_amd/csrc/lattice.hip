@@ -490,12 +490,83 @@ static void launch_step(cs_lattice* l)
     l->steps += 1;
 }
 
+// ---- graph replay of fused steps (launch-bound lattices) ----------------
+// A lattice of at most GRAPH_MAX_CELLS cells spends most of a step in launch
+// overhead; runs of GRAPH_STEPS steps are captured once into a HIP graph and
+// replayed.  GRAPH_STEPS is even, so a run ends on the buffer parity it
+// started from.  Stochastic rounding keys each step's random bits by the step
+// count, which a replay would repeat, so 16-bit storage with stochastic
+// rounding always launches step by step.
+constexpr int GRAPH_STEPS = 64;
+constexpr int64_t GRAPH_MAX_CELLS = (int64_t)1 << 20;
+static bool g_step_graphs = true;
+
+void clear_step_graphs(cs_lattice* l)
+{
+    for (hipGraphExec_t& g : l->step_graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+static bool graph_eligible(const cs_lattice* l)
+{
+    const bool sr16 = (l->storage == CS_BF16DEV || l->storage == CS_F16DEV) &&
+                      l->params.rounding == CS_ROUND_STOCHASTIC;
+    return g_step_graphs && !sr16 && (int64_t)l->w * l->h <= GRAPH_MAX_CELLS;
+}
+
+static void step_once(cs_lattice* l)
+{
+    dispatch_storage(l->storage, [&]<class St>() { launch_step<St>(l); });
+    l->time += l->params.delta_t;  // lbm.rs:713
+}
+
+// GRAPH_STEPS steps from the current parity, captured (first time) and replayed.
+static void replay_steps(cs_lattice* l)
+{
+    if (l->graph_cpt != g_cpt_override) clear_step_graphs(l);
+    const int parity = l->cur;
+    if (!l->step_graph[parity]) {
+        const int cur0 = l->cur;
+        const uint64_t steps0 = l->steps;
+        const double time0 = l->time;
+        CS_HIP(hipStreamSynchronize(l->stream));  // capture starts on an idle stream
+        CS_HIP(hipStreamBeginCapture(l->stream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t graph = nullptr;
+        try {
+            for (int s = 0; s < GRAPH_STEPS; ++s) step_once(l);
+        } catch (...) {
+            (void)hipStreamEndCapture(l->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            l->cur = cur0;
+            l->steps = steps0;
+            l->time = time0;
+            throw;
+        }
+        const hipError_t e = hipStreamEndCapture(l->stream, &graph);
+        l->cur = cur0;  // nothing ran: the capture only recorded the launches
+        l->steps = steps0;
+        l->time = time0;
+        CS_HIP(e);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CS_HIP(e2);
+        l->step_graph[parity] = exec;
+        l->graph_cpt = g_cpt_override;
+    }
+    CS_HIP(hipGraphLaunch(l->step_graph[parity], l->stream));
+    for (int s = 0; s < GRAPH_STEPS; ++s) l->time += l->params.delta_t;  // as step by step
+    l->steps += GRAPH_STEPS;
+}
+
 void do_steps(cs_lattice* l, int n)
 {
-    for (int s = 0; s < n; ++s) {
-        dispatch_storage(l->storage, [&]<class St>() { launch_step<St>(l); });
-        l->time += l->params.delta_t;  // lbm.rs:713
-    }
+    if (n >= GRAPH_STEPS && graph_eligible(l))
+        for (; n >= GRAPH_STEPS; n -= GRAPH_STEPS) replay_steps(l);
+    for (int s = 0; s < n; ++s) step_once(l);
     CS_HIP(hipGetLastError());
 }
 
@@ -672,6 +743,7 @@ extern "C" int cs_lattice_destroy(cs_lattice* l)
         for (void* p : {l->buf[0], l->buf[1], (void*)l->flags, (void*)l->node, (void*)l->snap, (void*)l->partials,
                         (void*)l->dscratch, (void*)l->iscratch, l->render_buf})
             if (p) (void)hipFree(p);
+        clear_step_graphs(l);
         if (l->ev0) (void)hipEventDestroy(l->ev0);
         if (l->ev1) (void)hipEventDestroy(l->ev1);
         if (l->stream) (void)hipStreamDestroy(l->stream);
@@ -718,6 +790,7 @@ extern "C" int cs_lattice_set_params(cs_lattice* l, const cs_params* p)
         }
         l->params = *p;
         set_consts(l, *p);
+        clear_step_graphs(l);
         if (edge_changed && l->has_flags) {
             dim3 g((unsigned)((l->w + 255) / 256), (unsigned)l->h);
             hipLaunchKernelGGL(k_node_info, g, dim3(256), 0, l->stream, l->flags, l->node, l->w, l->h, l->cpitch,
@@ -731,6 +804,7 @@ extern "C" int cs_lattice_set_flags(cs_lattice* l, const uint8_t* flags)
 {
     return guarded([&] {
         check_lat(l);
+        clear_step_graphs(l);
         if (!flags) {
             l->has_flags = false;
             return;
@@ -1296,6 +1370,12 @@ extern "C" int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_pe
 // Launch-shape override: cells per thread of the scalar step kernel (1, 2 or
 // 4; this also turns off the vector kernel of lattices without geometry);
 // 0 restores the default.
+// Graph replay of fused steps on launch-bound lattices (on by default).
+extern "C" int cs_set_step_graphs(int enable)
+{
+    return guarded([&] { g_step_graphs = enable != 0; });
+}
+
 extern "C" int cs_set_cells_per_thread(int cpt)
 {
     return guarded([&] {

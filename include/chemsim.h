@@ -345,6 +345,12 @@ int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy);
  * geometry; 0 = the built-in default).  Process-wide; for tests and tuning.
  * No reference counterpart (ArrayFire picks its own launch shapes). */
 int cs_set_cells_per_thread(int cells_per_thread);
+/* Replay of captured HIP graphs of 64 fused steps for lattices of at most
+ * 2^20 cells, where launch overhead dominates a step (1 = on, the default;
+ * 0 = launch step by step).  Results are identical either way; 16-bit storage
+ * with stochastic rounding always launches step by step.  Process-wide; for
+ * tests and tuning.  No reference counterpart. */
+int cs_set_step_graphs(int enable);
 
 #ifdef __cplusplus
 }
