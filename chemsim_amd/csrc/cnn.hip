@@ -161,7 +161,7 @@ __global__ void k_relu_back(const float* __restrict__ dy, const uint16_t* __rest
 
 // Bias gradient, first stage: block b sums the rows [b*rows, (b+1)*rows) of
 // dz [P][C] per channel into part[b][C] (C divides 256; fixed order).  The
-// second stage is k_reduce_splits over the blocks.
+// second stage is k_colsum over the blocks.
 __global__ void k_bias_partial(const uint16_t* __restrict__ dz, float* __restrict__ part, int64_t P, int C,
                                int64_t rows)
 {
@@ -216,14 +216,25 @@ __global__ void k_transpose16(const uint16_t* __restrict__ src, uint16_t* __rest
     }
 }
 
-// Sum of split-K partials in fixed order.
-__global__ void k_reduce_splits(const float* __restrict__ part, float* __restrict__ out, int64_t n, int splits)
+// Sum of split-K partials in a fixed order: a block takes 64 outputs, 16
+// thread groups each sum every 16th split (coalesced rows of 64 floats), and
+// one thread per output adds the 16 group sums in group order.
+__global__ __launch_bounds__(1024) void k_reduce_splits(const float* __restrict__ part, float* __restrict__ out,
+                                                        int64_t n, int splits)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ float sh[16][64];
+    const int o = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + o;
     float s = 0.f;
-    for (int z = 0; z < splits; ++z) s += part[(int64_t)z * n + i];
-    out[i] = s;
+    if (i < n)
+        for (int z = g; z < splits; z += 16) s += part[(int64_t)z * n + i];
+    sh[g][o] = s;
+    __syncthreads();
+    if (g == 0 && i < n) {
+        float v = 0.f;
+        for (int k = 0; k < 16; ++k) v += sh[k][o];
+        out[i] = v;
+    }
 }
 
 // Weighted MSE over the first `used` channels of the f32 head output y
@@ -458,8 +469,8 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
     while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
     gemm(net, KMAJOR, KMAJOR, EPI_F32, net->dzT, (int)P, net->col, (int)P, net->part, c.kpad, nullptr, c.cout, c.kpad,
          (int)P, splits);
-    hipLaunchKernelGGL(k_reduce_splits, dim3(blocks_for(c.nw())), dim3(256), 0, net->stream, net->part, c.gw, c.nw(),
-                       splits);
+    hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream, net->part,
+                       c.gw, c.nw(), splits);
     CS_HIP(hipGetLastError());
     if (!dx32) return;
     // data gradient: dx[p][ci] = sum_k colT(dz)[p][k] wd[ci][k]
