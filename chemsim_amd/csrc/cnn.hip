@@ -160,23 +160,34 @@ __global__ void k_relu_back(const float* __restrict__ dy, const uint16_t* __rest
 }
 
 // Bias gradient, first stage: block b sums the rows [b*rows, (b+1)*rows) of
-// dz [P][C] per channel into part[b][C] (C divides 256; fixed order).  The
-// second stage is k_colsum over the blocks.
+// dz [P][C] per channel into part[b][C] (8 <= C, C divides 256; fixed order).
+// A thread reads 8 channels (16 bytes) of a pixel per iteration.  The second
+// stage is k_colsum over the blocks.
 __global__ void k_bias_partial(const uint16_t* __restrict__ dz, float* __restrict__ part, int64_t P, int C,
                                int64_t rows)
 {
-    __shared__ float sh[256];
-    const int t = threadIdx.x, step = 256 / C;
-    const int c = t % C, r0 = t / C;
+    __shared__ float sh[256][9];
+    const int t = threadIdx.x, nv = C / 8, step = 256 / nv;
+    const int v = t % nv, r0 = t / nv;
     const int64_t pb = (int64_t)blockIdx.x * rows, pe = min(P, pb + rows);
-    float s = 0.f;
-    for (int64_t p = pb + r0; p < pe; p += step) s += bf16_f32(dz[p * C + c]);
-    sh[t] = s;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int64_t p = pb + r0; p < pe; p += step) {
+        const uint4 q = *(const uint4*)(dz + p * C + 8 * v);
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s[2 * j] += __uint_as_float(w[j] << 16);
+            s[2 * j + 1] += __uint_as_float(w[j] & 0xffff0000u);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sh[t][j] = s[j];
     __syncthreads();
     if (t < C) {
-        float v = 0.f;
-        for (int j = 0; j < step; ++j) v += sh[j * C + t];
-        part[(int64_t)blockIdx.x * C + t] = v;
+        const int vv = t / 8, jj = t % 8;
+        float acc = 0.f;
+        for (int k = 0; k < step; ++k) acc += sh[k * nv + vv][jj];
+        part[(int64_t)blockIdx.x * C + t] = acc;
     }
 }
 
@@ -445,7 +456,8 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
                        P * c.cout, c.relu ? 1 : 0);
     CS_HIP(hipGetLastError());
     CS_HIP(hipGetLastError());
-    CS_REQUIRE(c.cout <= 256 && 256 % c.cout == 0 && P % 8 == 0, CS_ERR_STATE, "unsupported layer shape");
+    CS_REQUIRE(c.cout >= 8 && c.cout <= 256 && 256 % c.cout == 0 && P % 8 == 0, CS_ERR_STATE,
+               "unsupported layer shape");
     {
         const int64_t blocks = std::min<int64_t>(1024, (P + 255) / 256);
         const int64_t rows = (P + blocks - 1) / blocks;
