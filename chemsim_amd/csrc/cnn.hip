@@ -385,6 +385,15 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
           int dil = 1)
 {
     GemmArgs g{};
+    if (lb == SHIFT) {
+        CS_REQUIRE(la != GATHER && gather && gather->C % 8 == 0 && (int64_t)K == gather->P() && dil >= 1, CS_ERR_ARG,
+                   "bad shifted operand");
+        g.gH = gather->H;
+        g.gW = gather->W;
+        g.gC = gather->C;
+        g.gsign = 1;
+        g.gdil = dil;
+    }
     if (la == GATHER) {
         CS_REQUIRE(gather && gather->C % 8 == 0 && (int64_t)M == gather->P() && dil >= 1, CS_ERR_ARG,
                    "bad gather operand");
@@ -415,6 +424,8 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
 #define CS_GEMM(LA, LB, EPI) launch_gemm<LA, LB, EPI>(cfg, grid, net->stream, g)
     if (la == KMAJOR && lb == KMAJOR && epi == EPI_F32) {
         CS_GEMM(KMAJOR, KMAJOR, EPI_F32);
+    } else if (la == KMAJOR && lb == SHIFT && epi == EPI_F32) {
+        CS_GEMM(KMAJOR, SHIFT, EPI_F32);
     } else if (la == GATHER && lb == KMAJOR) {
         if (epi == EPI_F32) CS_GEMM(GATHER, KMAJOR, EPI_F32);
         else if (epi == EPI_BF16) CS_GEMM(GATHER, KMAJOR, EPI_BF16);
@@ -426,13 +437,15 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
     CS_HIP(hipGetLastError());
 }
 
-// Transposed gather for the weight gradient: col[k][p] (k < kpad, p < P).
-void im2col_t(cs_cnn* net, const Act& x, int kpad, int dil)
+// The three column-shifted transposed copies of x that the weight gradient's
+// SHIFT operand reads: col[s][c][p] (3 x C x P).
+void shift3_t(cs_cnn* net, const Act& x, int dil)
 {
     const int64_t P = x.P();
-    CS_REQUIRE(P * kpad <= net->col_elems && P % 8 == 0, CS_ERR_STATE, "im2col scratch too small");
-    hipLaunchKernelGGL(k_im2col_t, dim3((unsigned)((P + 63) / 64), (unsigned)(kpad / 32)), dim3(256), 0, net->stream,
-                       x.d, net->col, x.N, x.H, x.W, x.C, kpad, dil);
+    CS_REQUIRE(3 * P * x.C <= net->col_elems && P % 8 == 0 && x.C % 8 == 0 && dil >= 1 && dil <= 8, CS_ERR_STATE,
+               "shift scratch too small or unsupported shape");
+    hipLaunchKernelGGL(k_shift3_t, dim3((unsigned)((P + 63) / 64), (unsigned)((x.C + 63) / 64)), dim3(256), 0,
+                       net->stream, x.d, net->col, P, x.W, x.C, dil);
     CS_HIP(hipGetLastError());
 }
 
@@ -468,19 +481,20 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
         CS_HIP(hipGetLastError());
     }
     // weight gradient: dW[co][k] = sum_p dzT[co][p] colT[k][p], split over p;
-    // both operands transposed so that p (the reduction) is contiguous
+    // dz transposed so that p (the reduction) is contiguous, colT read from
+    // the three column-shifted transposed copies of x
     hipLaunchKernelGGL(k_transpose16, dim3((unsigned)((P + 63) / 64), (unsigned)((c.cout + 63) / 64)), dim3(256), 0,
                        net->stream, net->dz, net->dzT, P, c.cout);
     CS_HIP(hipGetLastError());
-    im2col_t(net, x, c.kpad, c.dil);
+    shift3_t(net, x, c.dil);
     // enough K ranges for ~1024 workgroups, each at least 1024 pixels long
     int bm, bn;
     tile_dims(pick_tile(c.cout, c.kpad), bm, bn);
     const int64_t tiles = (int64_t)((c.cout + bm - 1) / bm) * ((c.kpad + bn - 1) / bn);
     int splits = (int)std::max<int64_t>(1, std::min<int64_t>({256, P / 1024, (1024 + tiles - 1) / tiles}));
     while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
-    gemm(net, KMAJOR, KMAJOR, EPI_F32, net->dzT, (int)P, net->col, (int)P, net->part, c.kpad, nullptr, c.cout, c.kpad,
-         (int)P, splits);
+    gemm(net, KMAJOR, SHIFT, EPI_F32, net->dzT, (int)P, net->col, (int)P, net->part, c.kpad, nullptr, c.cout, c.kpad,
+         (int)P, splits, &x, 1, c.dil);
     hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream, net->part,
                        c.gw, c.nw(), splits);
     CS_HIP(hipGetLastError());
@@ -772,11 +786,11 @@ extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int b
                                  net->cat2,        net->conv_out[10], net->cat1,       net->conv_out[12], net->conv_out[13]};
             for (int i = 0; i < 15; ++i) net->conv_in.push_back(ins[i]);
             net->head_out = dalloc<float>(net, (int64_t)n * H * W * 8);
-            // scratch: im2col of the largest (pixels x kpad) over forward and data-gradient gathers
+            // scratch: the weight gradient's three shifted copies of the largest input (3 x cin x pixels)
             int64_t col = 0, part = 0;
             for (int i = 0; i < 15; ++i) {
                 const int64_t P = net->conv_in[(size_t)i].P();
-                col = std::max(col, P * std::max(net->convs[(size_t)i].kpad, net->convs[(size_t)i].kpad_d));
+                col = std::max(col, 3 * P * net->convs[(size_t)i].cin);
                 part = std::max(part, (int64_t)64 * net->convs[(size_t)i].nw());
                 part = std::max(part, P * net->convs[(size_t)i].cin);
             }

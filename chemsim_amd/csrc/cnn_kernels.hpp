@@ -6,7 +6,8 @@
 // forward pass, the data gradient and the weight gradient:
 //   forward  out[p][co] = sum_k col(in)[p][k] * W[co][k]    gather inside the GEMM
 //   dgrad    din[p][ci] = sum_k colT(dout)[p][k] * Wd[ci][k] gather (flipped taps) inside the GEMM
-//   wgrad    dW[co][k]  = sum_p doutT[co][p] * colT[k][p]   both transposed in memory, split over p
+//   wgrad    dW[co][k]  = sum_p doutT[co][p] * colT[k][p]   doutT transposed in memory, colT read
+//                                                          from three column-shifted copies, split over p
 // GEMM tiles (BM x BN x 32, chosen by shape) are double-buffered in LDS;
 // 4 waves of 64 lanes each own a sub-tile of v_mfma_f32_16x16x32_bf16
 // fragments with FP32 accumulation.
@@ -36,7 +37,11 @@ __device__ __forceinline__ float bf16_f32(uint16_t b) { return __uint_as_float((
 // GATHER: the A operand of a 3x3 convolution (taps `dil` cells apart) read
 // straight from the NHWC input (implicit im2col): row r = pixel,
 // k = tap*C + c, zero outside the image and for k >= 9C.
-enum Layout : int { KMAJOR = 0, GATHER = 2 };
+// SHIFT: the B operand of the weight gradient, colT[k][p] with k = tap*C + c,
+// read from three column-shifted transposed copies of the input,
+// X3[s][c][p] = in[p + (s - 1) dil][c] (zero across the row ends), at the
+// row offset dy*W of the tap; zero outside the image and for k >= 9C.
+enum Layout : int { KMAJOR = 0, GATHER = 2, SHIFT = 3 };
 
 enum Epi : int {
     EPI_F32 = 0,        // C (+ bias) stored as f32
@@ -152,6 +157,69 @@ struct GatherLoader {
     }
 };
 
+// The SHIFT operand (rows n of colT, K = pixels): a vector of 8 pixels lies in
+// one image row when W is a multiple of 8 (one validity test, one 16-byte
+// load); otherwise the pixels are tested and loaded one by one.
+template <int R>
+struct ShiftLoader {
+    static constexpr int VECS = 4 * R;
+    static constexpr int NV = (VECS + 255) / 256;
+    u32x4 v[NV];
+
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ x3, const GemmArgs& g, int n0, int k0, int kend)
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            v[i] = u32x4{0u, 0u, 0u, 0u};
+            if (VECS % 256 != 0 && e >= VECS) continue;
+            const int n = n0 + (e >> 2), p0 = k0 + 8 * (e & 3);
+            if (n >= g.N || n >= 9 * g.gC || p0 >= kend) continue;
+            const int tap = n / g.gC, c = n - tap * g.gC;
+            const int dy = g.gdil * (tap / 3 - 1);
+            const uint16_t* row = x3 + ((int64_t)(tap % 3) * g.gC + c) * g.K;
+            if (g.gW % 8 == 0) {
+                const int yy = (p0 / g.gW) % g.gH + dy;
+                if (yy >= 0 && yy < g.gH) v[i] = *(const u32x4*)(row + p0 + dy * g.gW);
+            } else {
+                uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int p = p0 + j;
+                    const int yy = (p / g.gW) % g.gH + dy;
+                    const uint32_t b = (yy >= 0 && yy < g.gH) ? (uint32_t)row[p + dy * g.gW] : 0u;
+                    w[j >> 1] |= b << (16 * (j & 1));
+                }
+                v[i] = u32x4{w[0], w[1], w[2], w[3]};
+            }
+        }
+    }
+    __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            if (VECS % 256 != 0 && e >= VECS) continue;
+            *(u32x4*)(&lds[e >> 2][8 * (e & 3)]) = v[i];
+        }
+    }
+};
+
+template <int L, int R>
+struct LoaderB {
+    TileLoader<L, R> l;
+    __device__ __forceinline__ void load(const GemmArgs& g, int n0, int k0, int kend) { l.load(g.B, g.ldb, g.N, n0, k0, kend); }
+    __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const { l.store(lds); }
+};
+template <int R>
+struct LoaderB<SHIFT, R> {
+    ShiftLoader<R> l;
+    __device__ __forceinline__ void load(const GemmArgs& g, int n0, int k0, int kend) { l.load(g.B, g, n0, k0, kend); }
+    __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const { l.store(lds); }
+};
+
 template <int L, int R>
 struct LoaderA {
     TileLoader<L, R> l;
@@ -192,11 +260,11 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
         for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     LoaderA<LA, BM> la;
-    TileLoader<LB, BN> lb;
+    LoaderB<LB, BN> lb;
     la.init(g, m0);
     if (kb < ke) {
         la.load(g, m0, kb, ke);
-        lb.load(g.B, g.ldb, g.N, n0, kb, ke);
+        lb.load(g, n0, kb, ke);
         la.store(As[0]);
         lb.store(Bs[0]);
     }
@@ -206,7 +274,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
         const bool more = k0 + BK < ke;
         if (more) {
             la.load(g, m0, k0 + BK, ke);
-            lb.load(g.B, g.ldb, g.N, n0, k0 + BK, ke);
+            lb.load(g, n0, k0 + BK, ke);
         }
         bf16x8 a[TM], b[TN];
 #pragma unroll
@@ -285,43 +353,44 @@ inline void launch_gemm(int cfg, dim3 grid, hipStream_t s, const GemmArgs& g)
     }
 }
 
-// Transposed im2col for the weight gradient: col[k][p] = element (p, k) of
-// the forward gather (k = tap*C + c < Kpad, taps `dil` cells apart; zero for
-// k >= 9C and outside the image).  A block builds a 32 (k) x 64 (pixel) tile: 16-byte gathers along
-// the channels of each pixel, a transpose through LDS, then 16-byte stores
-// along the pixels of each k row.  P and C are multiples of 8.
-__global__ __launch_bounds__(256) void k_im2col_t(const uint16_t* __restrict__ in, uint16_t* __restrict__ col, int N,
-                                                  int H, int W, int C, int Kpad, int dil)
+// Column-shifted transposed copies of an NHWC input for the SHIFT operand:
+// x3[s][c][p] = in[p + (s - 1) dil][c] when the shifted pixel lies in p's
+// image row, else 0 (s = 0, 1, 2; dil <= 8).  A block loads 64 + 16 pixels x
+// 64 channels into LDS with 16-byte loads and writes the three shifted
+// 64-pixel rows of each channel with 16-byte stores.  P and C are multiples
+// of 8.
+__global__ __launch_bounds__(256) void k_shift3_t(const uint16_t* __restrict__ in, uint16_t* __restrict__ x3,
+                                                  int64_t P, int W, int C, int dil)
 {
-    __shared__ uint16_t tile[32][64 + 8];
-    const int64_t P = (int64_t)N * H * W;
+    __shared__ uint16_t tile[80][65];  // odd pitch: the 8 pixel groups of a channel hit 8 banks
     const int64_t p0 = (int64_t)blockIdx.x * 64;
-    const int k0 = blockIdx.y * 32;
+    const int c0 = blockIdx.y * 64;
     const int t = threadIdx.x;
-    {
-        const int j = t >> 2, kv = t & 3;  // pixel p0 + j, channels k0 + 8kv .. +7
-        const int64_t p = p0 + j;
-        const int k = k0 + 8 * kv;
+    for (int e = t; e < 80 * 8; e += 256) {
+        const int j = e >> 3, cv = e & 7;  // pixel p0 - 8 + j, channels c0 + 8cv .. +7
+        const int64_t q = p0 - 8 + j;
+        const int c = c0 + 8 * cv;
         u32x4 v = {0u, 0u, 0u, 0u};
-        if (p < P && k < 9 * C) {
-            const int tap = k / C, c = k - tap * C;
-            const int dy = dil * (tap / 3 - 1), dx = dil * (tap % 3 - 1);
-            const int x = (int)(p % W);
-            const int64_t ny = p / W;
-            const int y = (int)(ny % H);
-            const int yy = y + dy, xx = x + dx;
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = *(const u32x4*)(in + (p + dy * W + dx) * C + c);
-        }
-        const uint16_t* e = (const uint16_t*)&v;
+        if (q >= 0 && q < P && c < C) v = *(const u32x4*)(in + q * C + c);
+        const uint16_t* h = (const uint16_t*)&v;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) tile[8 * kv + i][j] = e[i];
+        for (int i = 0; i < 8; ++i) tile[j][8 * cv + i] = h[i];
     }
     __syncthreads();
-    {
-        const int row = t >> 3, q = t & 7;  // k0 + row, pixels p0 + 8q .. +7
-        const int64_t p = p0 + 8 * q;
-        if (k0 + row < Kpad && p < P)
-            *(u32x4*)(col + (int64_t)(k0 + row) * P + p) = *(const u32x4*)(&tile[row][8 * q]);
+    for (int e = t; e < 3 * 64 * 8; e += 256) {
+        const int s = e / 512, rem = e - s * 512;
+        const int c = rem >> 3, pv = rem & 7;  // channel c0 + c, pixels p0 + 8pv .. +7
+        const int64_t p = p0 + 8 * pv;
+        if (c0 + c >= C || p >= P) continue;
+        const int dx = (s - 1) * dil;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int xx = (int)((p + i) % W) + dx;
+            const uint32_t b = (xx >= 0 && xx < W) ? (uint32_t)tile[8 + 8 * pv + i + dx][c] : 0u;
+            w[i >> 1] |= b << (16 * (i & 1));
+        }
+        *(u32x4*)(x3 + ((int64_t)s * C + c0 + c) * P + p) = u32x4{w[0], w[1], w[2], w[3]};
     }
 }
 
