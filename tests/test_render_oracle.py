@@ -70,3 +70,25 @@ def test_geometry_overlay_and_speed():
     assert np.array_equal(img[~solid], plain[~solid]) and not np.array_equal(img, plain)
     # speed: sqrt(vx^2 + vy^2) of the velocity, red channel only
     assert np.all(plain[..., 1:] == 0)
+
+
+def test_hsv_matches_colorsys():
+    """The sextant HSV -> RGB formula against Python's colorsys (an independent
+    implementation of the published conversion) on random colours, including
+    the sextant boundaries."""
+    import colorsys
+    rng = np.random.default_rng(3)
+    h = np.concatenate([rng.random(4000), np.arange(7) / 6.0 * 0.999999]).astype(np.float32)
+    s = np.concatenate([rng.random(4000), np.full(7, 0.8)]).astype(np.float32)
+    v = np.concatenate([rng.random(4000), np.full(7, 0.6)]).astype(np.float32)
+    r, g, b = R.hsv_to_rgb(h, s, v)
+    want = np.array([colorsys.hsv_to_rgb(float(a), float(c), float(d)) for a, c, d in zip(h, s, v)])
+    np.testing.assert_allclose(np.stack([r, g, b], 1), want, rtol=0, atol=2e-6)
+
+
+def test_stats_are_population_mean_and_std():
+    rng = np.random.default_rng(4)
+    x = (1 + 0.01 * rng.standard_normal(10000)).astype(np.float32)
+    mean, inv = R.stats(x)
+    assert abs(float(mean) - float(np.mean(x.astype(np.float64)))) < 1e-7
+    assert abs(1.0 / float(inv) / float(np.std(x.astype(np.float64), ddof=0)) - 1) < 1e-5
