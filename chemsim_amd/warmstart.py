@@ -7,7 +7,7 @@ distance to the nearest solid.  The lattice is initialised at the predicted
 equilibrium (compute_equilibrium, src/lbm.rs:43-71, as src/main.rs:257-267
 initialises a lattice), optionally plus the Chapman-Enskog non-equilibrium
 part of the predicted velocity gradient, and the plain BGK step runs to the
-residual tolerance, so the LBM stays the judge of the prediction.  Training
+residual tolerance, so the LBM itself decides when the flow is steady.  Training
 data are steady states computed by the GPU LBM itself (a batch of lattices
 in deviation arithmetic).
 """

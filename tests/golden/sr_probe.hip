@@ -1,6 +1,7 @@
 // Generator of tests/golden/sr_vectors.npz: runs gfx950's stochastic-rounding
 // conversions (v_cvt_sr_bf16_f32, v_cvt_sr_f16_f32) on 2^20 (x, r) pairs and
-// writes x, r and both outputs to gpurun_out/sr_probe.bin; the fixture keeps
+// writes x, r and both outputs to the file named on the command line (default
+// sr_probe.bin); the fixture keeps
 // 2048 pairs of each of the 8 input classes (NaN inputs dropped).
 // Build: hipcc --offload-arch=gfx950 -O2 -o sr_probe sr_probe.hip
 // One thread per pair, grid = ceil(N/256) blocks, guarded by i < N.
@@ -23,7 +24,7 @@ __global__ void k(const float* x, const unsigned* r, unsigned short* ob, unsigne
 }
 static uint32_t s = 12345;
 static uint32_t xr() { s ^= s << 13; s ^= s >> 17; s ^= s << 5; return s; }
-int main() {
+int main(int argc, char** argv) {
   const int n = 1 << 20;
   std::vector<float> x(n); std::vector<unsigned> r(n);
   for (int i = 0; i < n; ++i) {
@@ -44,7 +45,8 @@ int main() {
   std::vector<unsigned short> ob(n), oh(n);
   if (hipMemcpy(ob.data(), db, n * 2, hipMemcpyDeviceToHost) != hipSuccess) { printf("fail\n"); return 1; }
   hipMemcpy(oh.data(), dh, n * 2, hipMemcpyDeviceToHost);
-  FILE* f = fopen("gpurun_out/sr_probe.bin", "wb");
+  FILE* f = fopen(argc > 1 ? argv[1] : "sr_probe.bin", "wb");
+  if (!f) { printf("cannot open output\n"); return 1; }
   fwrite(x.data(), 4, n, f); fwrite(r.data(), 4, n, f); fwrite(ob.data(), 2, n, f); fwrite(oh.data(), 2, n, f); fclose(f);
   printf("sr probe ok n=%d\n", n);
   return 0;
