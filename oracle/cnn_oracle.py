@@ -1,8 +1,8 @@
 """CPU oracle of the warm-start network (float64 numpy).
 
-TEST INFRASTRUCTURE ONLY: the checker of chemsim_amd/csrc/cnn.hip; imported by
+TEST INFRASTRUCTURE ONLY: the parity reference of chemsim_amd/csrc/cnn.hip; imported by
 tests/ only.  The network has no counterpart in the reference (it is a
-north-star extension); this restatement is pinned by central finite
+warm-start extension); this restatement is pinned by central finite
 differences and by torch CPU autograd (tests/test_cnn_oracle.py).
 
 With bf16=True the forward and backward passes round to bfloat16 (nearest

@@ -1,6 +1,6 @@
 /*
  * lbm_oracle.c -- CPU restatement of taktoa/chemsim's D2Q9 BGK hot path
- * (src/lbm.rs), used as the parity checker.  TEST INFRASTRUCTURE ONLY: the
+ * (src/lbm.rs), used as the parity reference.  TEST INFRASTRUCTURE ONLY: the
  * product library never links it.  Build: oracle/Makefile (-ffp-contract=off).
  */
 #include "lbm_oracle.h"

@@ -2,7 +2,7 @@
  * lbm_oracle.h -- CPU restatement of the D2Q9 lattice-Boltzmann hot path of
  * taktoa/chemsim (src/lbm.rs).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity reference: it is loaded by
  * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
  * never by the product library (chemsim_amd/csrc), which has no CPU path.
  *

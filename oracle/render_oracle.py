@@ -1,7 +1,7 @@
 """CPU restatement of the render pass (src/render.rs:7-178 as src/main.rs:157-176
 drives it), numpy float32.
 
-TEST INFRASTRUCTURE ONLY: the checker of chemsim_amd/csrc/render.hip; imported
+TEST INFRASTRUCTURE ONLY: the parity reference of chemsim_amd/csrc/render.hip; imported
 by tests/ only.
 
 Parity note: af::mean_all / af::stdev_all are taken as the population mean and

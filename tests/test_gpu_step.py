@@ -376,7 +376,7 @@ def test_taylor_green_4096_fp32_vs_fp64(ctx):
 
 @pytest.mark.parametrize("storage", ["f32", "f64"])
 def test_taylor_green_8192_matches_oracle(ctx, storage):
-    """The largest north-star lattice, 8192^2, two steps value for value with
+    """The largest benchmark lattice, 8192^2, two steps value for value with
     the oracle: FP32 runs the vector kernel with 32-bit offsets (2.4 GB per
     buffer), FP64 the scalar kernel with 64-bit offsets (4.8 GB per buffer)."""
     n, steps = 8192, 2
