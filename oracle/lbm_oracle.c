@@ -481,8 +481,14 @@ void orc_step_dev16_ex(const orc_params* p, int kind, int rounding, uint32_t see
                        const uint8_t* flags, const uint16_t* P, uint16_t* out)
 {
     const long n = (long)p->w * p->h;
-    float* g = (float*)malloc(sizeof(float) * 9 * n);
-    float* o = (float*)malloc(sizeof(float) * 9 * n);
+    if (n <= 0) return;
+    float* g = (float*)calloc((size_t)(9 * n), sizeof(float));
+    float* o = (float*)calloc((size_t)(9 * n), sizeof(float));
+    if (!g || !o) {
+        free(g);
+        free(o);
+        return;
+    }
     for (long j = 0; j < 9 * n; ++j) g[j] = dev_load(kind, P[j]);
     orc_step_devf32(p, flags, g, o);
     if (rounding == 0) {
