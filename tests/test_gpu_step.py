@@ -601,3 +601,23 @@ def test_regularized_kbc_cavity_full_size(ctx, coll):
     ref, _ = O.run_omp(op, f0, steps, flags)
     np.testing.assert_array_equal(got, ref)
     assert np.isfinite(got).all()
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (7, 1), (2, 3), (5, 4), (3, 65)])
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_degenerate_shapes_match_oracle(ctx, shape, edge, storage):
+    """Tiny and one-cell-wide lattices (every neighbour outside or wrapped onto
+    the cell itself), with and without a mask: value for value with the oracle."""
+    h, w = shape
+    rng = np.random.default_rng(h * 100 + w)
+    dt = STORE[storage]
+    for with_flags in (False, True):
+        f = rand_state(rng, h, w, dt)
+        flags = (rng.random((h, w)) < 0.4).astype(np.uint8) if with_flags else None
+        op = O.params(w, h, tau=0.7, edge=edge)
+        with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+            lat.upload(f)
+            lat.step(3)
+            np.testing.assert_array_equal(lat.download(dt), O.run(op, f, 3, flags),
+                                          err_msg=f"{shape} flags={with_flags}")
