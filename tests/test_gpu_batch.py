@@ -150,3 +150,30 @@ def test_inactive_lattices_keep_their_state(ctx):
                 d0 = O.to_f16(O.equilibrium_dev(op, *init))
                 np.testing.assert_array_equal(a.download_raw(i), O.run_dev16(op, O.F16, d0, 11, flags, rounding=1,
                                                                              seed=77 + i))
+
+
+@pytest.mark.parametrize("launch,threads", [(capi.CS_LAUNCH_TILES, 256), (capi.CS_LAUNCH_TILES, 1024),
+                                            (capi.CS_LAUNCH_PERSISTENT, 256)])
+def test_batch_tiny_lattices_match_oracle(ctx, launch, threads):
+    """1x1, one-cell-wide and small ragged lattices beside a normal one, in
+    both launch kinds: every lattice equals the oracle's run of it alone."""
+    rng = np.random.default_rng(9)
+    shapes = [(1, 1), (1, 9), (9, 1), (3, 5), (70, 2), (64, 64)]
+    b = Batch(ctx, shapes, "f32")
+    specs = []
+    with b:
+        b.tuning = dict(block_threads=threads, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=launch)
+        for i, (w, h) in enumerate(shapes):
+            edge = O.EDGE_PERIODIC if i % 2 else O.EDGE_ZERO
+            flags = (rng.random((h, w)) < 0.3).astype(np.uint8) if i in (3, 5) else None
+            op = O.params(w, h, tau=0.6 + 0.05 * i, edge=edge)
+            b.set_params(i, Params(tau=op.tau, edge=edge))
+            b.set_flags(i, flags)
+            rho = (1 + 0.01 * rng.standard_normal((h, w))).astype(np.float32)
+            ux = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+            uy = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+            b.init_equilibrium(i, rho, ux, uy)
+            specs.append((op, flags, O.equilibrium(op, rho, ux, uy)))
+        b.step(5)
+        for i, (op, flags, f0) in enumerate(specs):
+            np.testing.assert_array_equal(b.download(i), O.run(op, f0, 5, flags), err_msg=f"lattice {i} {shapes[i]}")
