@@ -61,6 +61,7 @@ extern "C" int cs_lattice_record(cs_lattice* l, int frames, int steps_per_frame,
         CS_REQUIRE(depth >= 1 && depth <= CS_RECORD_MAX_DEPTH, CS_ERR_ARG, "ring depth out of range");
         CS_HIP(hipSetDevice(l->ctx->device));
         const size_t fbytes = (size_t)4 * l->w * l->h;
+        const size_t dstride = (fbytes + 255) / 256 * 256;  // device slots 256-byte aligned for the pixel pass
         RecordRing ring;
         ring.l = l;
         CS_HIP(hipStreamCreateWithFlags(&ring.copy, hipStreamNonBlocking));
@@ -70,7 +71,7 @@ extern "C" int cs_lattice_record(cs_lattice* l, int frames, int steps_per_frame,
             CS_HIP(hipEventCreateWithFlags(&ring.rendered[s], hipEventDisableTiming));
             CS_HIP(hipEventCreateWithFlags(&ring.copied[s], hipEventDisableTiming));
         }
-        CS_HIP(hipMalloc(&ring.dslots, fbytes * depth));
+        CS_HIP(hipMalloc(&ring.dslots, dstride * depth));
         CS_HIP(hipHostMalloc((void**)&ring.hslots, fbytes * depth, hipHostMallocDefault));
         const auto t_start = std::chrono::steady_clock::now();  // set-up above is not timed
 
@@ -94,7 +95,7 @@ extern "C" int cs_lattice_record(cs_lattice* l, int frames, int steps_per_frame,
             const int s = (int)(k % depth);
             // slot s last held frame k - depth, delivered (copy finished) in an earlier iteration
             do_steps(l, steps_per_frame);
-            uint8_t* dslot = ring.dslots + fbytes * s;
+            uint8_t* dslot = ring.dslots + dstride * s;
             render_enqueue(l, mode, overlay, dslot, 4);
             CS_HIP(hipEventRecord(ring.rendered[s], l->stream));
             CS_HIP(hipStreamWaitEvent(ring.copy, ring.rendered[s], 0));

@@ -37,8 +37,16 @@ __global__ __launch_bounds__(256) void k_render_field(const typename St::S* __re
     __shared__ double sh[2][256];
     const int64_t n = (int64_t)w * h;
     double s1 = 0.0, s2 = 0.0;
-    for (int64_t d = (int64_t)blockIdx.x * 256 + threadIdx.x; d < n; d += (int64_t)gridDim.x * 256) {
-        const int r = (int)(d / w), c = (int)(d - (int64_t)r * w);
+    // cell d = r * w + c; (r, c) advance by the grid stride without a division per cell
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    const int dr = (int)(stride / w), dc = (int)(stride - (int64_t)dr * w);
+    int64_t d = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int r = (int)(d / w), c = (int)(d - (int64_t)r * w);
+    for (; d < n; d += stride, r += dr, c += dc) {
+        if (c >= w) {
+            c -= w;
+            ++r;
+        }
         T B[9];
         load_own<St>(f, P, (int64_t)r * pitch + c, B);
         T rho, mx, my, ux, uy;
@@ -109,7 +117,9 @@ __device__ __forceinline__ void hsv_to_rgb(float hue, float sat, float val, floa
 {
     const float c = val * sat;
     const float h6 = hue * 6.0f;
-    const float x = c * (1.0f - fabsf(fmodf(h6, 2.0f) - 1.0f));
+    // fmod(h6, 2) for h6 in [0, 6]: h6 - 2 floor(h6 / 2) is exact there (the
+    // halving and doubling are exact and the subtraction meets Sterbenz's bound)
+    const float x = c * (1.0f - fabsf((h6 - 2.0f * floorf(h6 * 0.5f)) - 1.0f));
     const float m = val - c;
     float r1, g1, b1;
     if (h6 < 1.0f) { r1 = c; g1 = x; b1 = 0.0f; }
@@ -131,6 +141,27 @@ __device__ __forceinline__ uint8_t to_byte(float v)
 
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
+// One pixel's colour (render.rs:7-178): z-score, logistic, HSV -> RGB, the
+// geometry overlay; packed as R | G << 8 | B << 16 | 255 << 24.
+__device__ __forceinline__ uint32_t pixel_rgba(float s, float ph, float avg, float inv, int mode, bool wall)
+{
+    if (wall) return 0u | (0u << 8) | (255u << 16) | (255u << 24);   // render_geometry: blue
+    const float z = (s - avg) * inv;
+    const float val = clamp01(1.0f / (1.0f + expf(-z)));  // af::sigmoid
+    float hue = 0.0f, sat = 1.0f;
+    if (mode == CS_RENDER_VELOCITY || mode == CS_RENDER_MOMENTUM) {
+        const float pi = 3.14159265358979323846f;
+        hue = clamp01((ph + pi) * (0.318309886183790671538f * 0.5f));  // render.rs:121-125
+        sat = 0.8f;
+    }
+    float r, g, b;
+    hsv_to_rgb(hue, sat, val, r, g, b);
+    return (uint32_t)to_byte(r) | ((uint32_t)to_byte(g) << 8) | ((uint32_t)to_byte(b) << 16) | (255u << 24);
+}
+
+// Pixel pass: 4 pixels per thread.  CH = 4 writes RGBA (RGB::to_rgba,
+// display.rs:27-29) as one 16-byte store, CH = 3 packed RGB as three 4-byte
+// stores; `rgb`, `sval` and `phase` are 16-byte aligned.
 template <int CH>
 __global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__ sval, const float* __restrict__ phase,
                                                        const float* __restrict__ stats,
@@ -138,37 +169,64 @@ __global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__
                                                        int mode, int overlay, uint8_t* __restrict__ rgb)
 {
     const int64_t n = (int64_t)w * h;
-    const int64_t d = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (d >= n) return;
-    const float avg = stats[0], inv = stats[1];
-    const float z = (sval[d] - avg) * inv;
-    const float val = clamp01(1.0f / (1.0f + expf(-z)));  // af::sigmoid
-    float hue = 0.0f, sat = 1.0f;
-    if (mode == CS_RENDER_VELOCITY || mode == CS_RENDER_MOMENTUM) {
-        const float pi = 3.14159265358979323846f;
-        hue = clamp01((phase[d] + pi) * (0.318309886183790671538f * 0.5f));  // render.rs:121-125
-        sat = 0.8f;
-    }
-    float r, g, b;
-    hsv_to_rgb(hue, sat, val, r, g, b);
-    uint8_t o[3] = {to_byte(r), to_byte(g), to_byte(b)};
-    if (overlay && flags) {
-        const int rr = (int)(d / w), cc = (int)(d - (int64_t)rr * w);
-        const int t = flags[(int64_t)rr * cpitch + cc];
-        if (t == CS_WALL || t == CS_LID) {
-            o[0] = 0;
-            o[1] = 0;
-            o[2] = 255;
+    const int64_t d0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (d0 >= n) return;
+    const bool full = d0 + 4 <= n;
+    const bool vec = mode == CS_RENDER_VELOCITY || mode == CS_RENDER_MOMENTUM;
+    float s[4], ph[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (full) {
+        const float4 v = *reinterpret_cast<const float4*>(sval + d0);
+        s[0] = v.x; s[1] = v.y; s[2] = v.z; s[3] = v.w;
+        if (vec) {
+            const float4 q = *reinterpret_cast<const float4*>(phase + d0);
+            ph[0] = q.x; ph[1] = q.y; ph[2] = q.z; ph[3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s[j] = d0 + j < n ? sval[d0 + j] : 0.0f;
+            if (vec) ph[j] = d0 + j < n ? phase[d0 + j] : 0.0f;
         }
     }
+    bool wall[4] = {false, false, false, false};
+    if (overlay && flags) {
+        int rr = (int)(d0 / w), cc = (int)(d0 - (int64_t)rr * w);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (d0 + j < n) {
+                const int t = flags[(int64_t)rr * cpitch + cc];
+                wall[j] = t == CS_WALL || t == CS_LID;
+            }
+            if (++cc == w) {
+                cc = 0;
+                ++rr;
+            }
+        }
+    }
+    const float avg = stats[0], inv = stats[1];
+    uint32_t px[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) px[j] = pixel_rgba(s[j], ph[j], avg, inv, mode, wall[j]);
     if constexpr (CH == 4) {
-        // RGB::to_rgba (display.rs:27-29): alpha 255, one 32-bit store per pixel
-        reinterpret_cast<uint32_t*>(rgb)[d] = (uint32_t)o[0] | ((uint32_t)o[1] << 8) | ((uint32_t)o[2] << 16) |
-                                              (255u << 24);
+        if (full) {
+            *reinterpret_cast<uint4*>(rgb + 4 * d0) = make_uint4(px[0], px[1], px[2], px[3]);
+        } else {
+            for (int j = 0; j < 4 && d0 + j < n; ++j) reinterpret_cast<uint32_t*>(rgb)[d0 + j] = px[j];
+        }
     } else {
-        rgb[3 * d] = o[0];
-        rgb[3 * d + 1] = o[1];
-        rgb[3 * d + 2] = o[2];
+        if (full) {
+            // bytes R0 G0 B0 R1 | G1 B1 R2 G2 | B2 R3 G3 B3
+            uint32_t* o = reinterpret_cast<uint32_t*>(rgb + 3 * d0);
+            o[0] = (px[0] & 0xffffffu) | (px[1] << 24);
+            o[1] = ((px[1] >> 8) & 0xffffu) | ((px[2] & 0xffffu) << 16);
+            o[2] = ((px[2] >> 16) & 0xffu) | ((px[3] & 0xffffffu) << 8);
+        } else {
+            for (int j = 0; j < 4 && d0 + j < n; ++j) {
+                rgb[3 * (d0 + j)] = (uint8_t)px[j];
+                rgb[3 * (d0 + j) + 1] = (uint8_t)(px[j] >> 8);
+                rgb[3 * (d0 + j) + 2] = (uint8_t)(px[j] >> 16);
+            }
+        }
     }
 }
 
@@ -209,6 +267,7 @@ static RenderScratch render_scratch(cs_lattice* l)
 void render_enqueue(cs_lattice* l, int mode, int overlay, uint8_t* d_out, int channels)
 {
     CS_REQUIRE(d_out != nullptr && (channels == 3 || channels == 4), CS_ERR_ARG, "bad render target");
+    CS_REQUIRE(((uintptr_t)d_out & 15) == 0, CS_ERR_ARG, "render target must be 16-byte aligned");
     const int64_t n = (int64_t)l->w * l->h;
     const RenderScratch r = render_scratch(l);
     const int blocks = (int)std::min<int64_t>(RENDER_BLOCKS, (n + 255) / 256);
@@ -227,12 +286,13 @@ void render_enqueue(cs_lattice* l, int mode, int overlay, uint8_t* d_out, int ch
     }
     hipLaunchKernelGGL(k_render_stats, dim3(1), dim3(256), 0, l->stream, r.partials, blocks, n, r.stats);
     const uint8_t* fl = l->has_flags ? (const uint8_t*)l->flags : nullptr;
+    const unsigned pblocks = (unsigned)((n + 1023) / 1024);  // 4 pixels per thread
     if (channels == 4)
-        hipLaunchKernelGGL(k_render_pixels<4>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, l->stream, r.sval,
-                           r.phase, r.stats, fl, l->w, l->h, l->cpitch, mode, overlay, d_out);
+        hipLaunchKernelGGL(k_render_pixels<4>, dim3(pblocks), dim3(256), 0, l->stream, r.sval, r.phase, r.stats, fl,
+                           l->w, l->h, l->cpitch, mode, overlay, d_out);
     else
-        hipLaunchKernelGGL(k_render_pixels<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, l->stream, r.sval,
-                           r.phase, r.stats, fl, l->w, l->h, l->cpitch, mode, overlay, d_out);
+        hipLaunchKernelGGL(k_render_pixels<3>, dim3(pblocks), dim3(256), 0, l->stream, r.sval, r.phase, r.stats, fl,
+                           l->w, l->h, l->cpitch, mode, overlay, d_out);
     CS_HIP(hipGetLastError());
 }
 
