@@ -52,7 +52,7 @@ struct LatState {
 };
 
 // One entry of the flattened tile list: lattice index and tile index within
-// it (tiles of 64 columns x 4 rows, row-major).
+// it (tiles of 64 columns x TR rows, row-major).
 struct TileRef {
     int lat;
     int tile;
@@ -194,9 +194,14 @@ __global__ __launch_bounds__(256) void k_batch_residual(const BatchLat<typename 
     if (threadIdx.x == 0) out[li] = sh[1][0] > 0 ? sqrt(sh[0][0] / sh[1][0]) : (sh[0][0] > 0 ? INFINITY : 0.0);
 }
 
-// One step of every lattice in a tile list: block b updates tile tiles[b]
+// One step of every lattice in a tile list: each block updates one tile
 // (TR rows x 64 columns, one cell per thread) of its lattice with the
 // single-lattice gather/finish.  `s` is the step's index within the call.
+// Each wave's 64 columns start on a 64-column boundary, so its row segment of
+// a plane is whole aligned cache lines.  Blocks go to the 8 XCDs round-robin;
+// xcd_tile gives each XCD a contiguous run of the list, so a tile's halo rows
+// were mostly just read by the tiles above and below it on the same XCD and
+// come from that XCD's L2.
 template <class St, int EDGE, int MODE, bool SR, int TR>
 __global__ __launch_bounds__(64 * TR) void k_batch_tiles(const BatchLat<typename St::T>* __restrict__ lats,
                                                      const TileRef* __restrict__ tiles, void* __restrict__ arena,
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(64 * TR) void k_batch_tiles(const BatchLat<typename
 {
     using S = typename St::S;
     using T = typename St::T;
-    const TileRef tr = tiles[blockIdx.x];
+    const TileRef tr = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const int li = __builtin_amdgcn_readfirstlane(tr.lat);
     const int t = __builtin_amdgcn_readfirstlane(tr.tile);
     const BatchLat<T> L = lats[li];
