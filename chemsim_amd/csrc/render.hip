@@ -25,6 +25,7 @@
 
 namespace cs {
 
+// Field-pass grid: 1024 blocks of 256 threads (measured faster than 2048 at 4096^2).
 enum : int { RENDER_BLOCKS = 1024 };
 
 template <class St>
