@@ -109,3 +109,25 @@ def test_record_large_frames_overlap(ctx):
             walls[depth] = st["wall_ms"]
     assert sums[1] == sums[3]
     assert walls[3] <= walls[1] * 1.2, walls
+
+
+@pytest.mark.parametrize("w,h", [(5, 3), (1, 7), (33, 1)])
+def test_record_ragged_sizes(ctx, w, h):
+    """Frames of 4·w·h bytes that are not a multiple of 16 still land in
+    aligned device slots; each equals steps + render on a twin lattice."""
+    p = Params(tau=0.8, edge=capi.CS_EDGE_PERIODIC)
+    rng = np.random.default_rng(3)
+    rho = (1 + 0.01 * rng.standard_normal((h, w))).astype(np.float32)
+    ux = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+    uy = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+    print(f"[record] {w}x{h} depth=3", flush=True)
+    with Lattice(ctx, w, h, "f32", p) as a, Lattice(ctx, w, h, "f32", p) as b:
+        a.init_equilibrium(rho, ux, uy)
+        b.init_equilibrium(rho, ux, uy)
+        rec, stats = a.record(5, 2, capi.CS_RENDER_VELOCITY, True, 3)
+        assert rec.shape == (5, h, w, 4) and stats["frames"] == 5
+        for k in range(5):
+            b.step(2)
+            img, _ = b.render(capi.CS_RENDER_VELOCITY, True)
+            np.testing.assert_array_equal(rec[k, :, :, :3], img)
+            assert (rec[k, :, :, 3] == 255).all()

@@ -51,3 +51,29 @@ def test_render_other_storages(ctx, storage):
         ib, _ = b.render(capi.CS_RENDER_VELOCITY)
         d = np.abs(ia.astype(int) - ib.astype(int))
         assert d.max() <= (1 if storage in ("f64", "f32dev") else 3), d.max()
+
+
+@pytest.mark.parametrize("mode", [capi.CS_RENDER_DENSITY, capi.CS_RENDER_SPEED, capi.CS_RENDER_VELOCITY,
+                                  capi.CS_RENDER_MOMENTUM])
+def test_render_ragged_sizes(ctx, mode):
+    """Lattices whose cell count is not a multiple of 4 (the pixel pass's
+    tail), one cell wide or high (overlay rows wrapping inside a thread's
+    pixels) and 1x1 (zero spread: black, as the restatement gives)."""
+    rng = np.random.default_rng(17)
+    for w, h in [(1, 1), (1, 6), (6, 1), (3, 5), (5, 3), (7, 9), (2, 3), (13, 11)]:
+        flags = rng.choice(np.array([O.FLUID, O.WALL, O.LID], np.uint8), size=(h, w), p=[0.7, 0.2, 0.1])
+        p = Params(tau=0.7, boundary=capi.CS_BB_HALFWAY, lid_uy=0.05)
+        rho = (1 + 0.01 * rng.standard_normal((h, w))).astype(np.float32)
+        ux = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+        uy = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
+        print(f"[render] {w}x{h} mode={mode}", flush=True)
+        with Lattice(ctx, w, h, "f32", p, flags) as lat:
+            lat.init_equilibrium(rho, ux, uy)
+            lat.step(3)
+            f = lat.download()
+            for overlay in (True, False):
+                img, _ = lat.render(mode, overlay)
+                ref = R.render(f, mode, flags, overlay)
+                d = np.abs(img.astype(int) - ref.astype(int))
+                assert d.max() <= 1, (w, h, mode, overlay, d.max())
+                assert (d > 0).sum() <= 3, (w, h, mode, overlay, (d > 0).sum())
