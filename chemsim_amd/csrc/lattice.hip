@@ -400,12 +400,12 @@ static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
                            a);
 }
 
-// Lattices without geometry and 4- or 8-byte storage take the vector kernel
-// (16 bytes per lane and plane, 4 rows per block) when a row splits into
-// vectors and the buffers allow 32-bit offsets.  16-bit storage keeps the
-// scalar kernel with 4 cells per lane, which measured faster (124 vs 144 us
-// per 4096^2 step with stochastic rounding).
-template <class St, int EDGE, bool SR, int COLL>
+// Lattices with 4- or 8-byte storage, with or without geometry, take the
+// vector kernel (16 bytes per lane and plane, 4 rows per block) when a row
+// splits into vectors and the buffers allow 32-bit offsets.  16-bit storage
+// keeps the scalar kernel with 4 cells per lane, which measured faster (124
+// vs 144 us per 4096^2 step with stochastic rounding).
+template <class St, int EDGE, int MODE, bool SR, int COLL>
 static bool launch_step_vec(cs_lattice* l, const void* fin, void* fout)
 {
     if constexpr (sizeof(typename St::S) < 4) {
@@ -417,7 +417,7 @@ static bool launch_step_vec(cs_lattice* l, const void* fin, void* fout)
         StepArgs<typename St::T> a{};
         a.fin = fin;
         a.fout = fout;
-        a.node = nullptr;
+        a.node = MODE == MODE_FLUID ? nullptr : l->node;
         a.w = l->w;
         a.h = l->h;
         a.pitch = l->pitch;
@@ -428,17 +428,42 @@ static bool launch_step_vec(cs_lattice* l, const void* fin, void* fout)
         a.cell_base = 0;
         a.k = consts_of<St>(l);
         const int nblocks = a.tiles_x * ((l->h + ROWS - 1) / ROWS);
-        hipLaunchKernelGGL((k_step_vec<St, EDGE, SR, VB, ROWS, COLL>), dim3(nblocks), dim3(64 * ROWS), 0, l->stream,
-                           a);
+        hipLaunchKernelGGL((k_step_vec<St, EDGE, SR, VB, ROWS, COLL, MODE>), dim3(nblocks), dim3(64 * ROWS), 0,
+                           l->stream, a);
         return true;
     }
+}
+
+// Which of the two bit-identical step kernels a lattice takes, by size
+// (FP32, MI355X, microseconds per step, vector / scalar kernel, periodic
+// Taylor-Green and half-way cavity):
+//   128^2 4.3 / 2.5, 6.7 / 3.1;  512^2 5.2 / 3.9, 6.3 / 5.3
+//   1024^2 12.6 / 17.5, 13.5 / 17.4;  2048^2 56.0 / 54.0, 59.5 / 53.9
+//   4096^2 203.7 / 204.5, 210.2 / 226.5;  8192^2 784 / 809, 835 / 885;
+// FP64: 256^2 4.5 / 3.3, 5.4 / 4.4;  512^2 7.5 / 8.0, 8.1 / 9.5;
+//   4096^2 391 / 397, 399 / 421.
+// Small lattices are latency-bound and gain from the scalar kernel's two
+// (FP64) or four (FP32) times as many lanes.  Where one population buffer is
+// 80-300 MB it partly stays in the 256 MB last-level cache between steps,
+// which the scalar kernel's cacheable loads use and the vector kernel's
+// non-temporal loads do not.
+static bool vector_size_class(const cs_lattice* l)
+{
+    const int64_t cells = (int64_t)l->w * l->h;
+    const int64_t esize = (int64_t)storage_bytes(l->storage);
+    const int64_t bytes = pop_elems(l) * esize;
+    if (cells < ((int64_t)1 << (esize == 8 ? 18 : 19))) return false;
+    return bytes < ((int64_t)80 << 20) || bytes > ((int64_t)300 << 20);
 }
 
 template <class St, int EDGE, int MODE, bool SR, int COLL>
 static void launch_step_sr(cs_lattice* l, const void* fin, void* fout)
 {
-    if (MODE == MODE_FLUID && g_cpt_override == 0 && launch_step_vec<St, EDGE, SR, COLL>(l, fin, fout)) return;
-    int cpt = g_cpt_override;
+    const int ov = g_cpt_override;
+    if ((ov == CS_STEP_VECTOR || (ov == 0 && vector_size_class(l))) &&
+        launch_step_vec<St, EDGE, MODE, SR, COLL>(l, fin, fout))
+        return;
+    int cpt = ov > 0 ? ov : 0;
     // 16-bit storage: 4 cells per lane (with stochastic rounding on the
     // Taylor-Green benchmark 143.6 against 140.4 GLUPS for one cell per lane at
     // 4096^2, 147.8 against 141.3 at 8192^2).
@@ -1368,8 +1393,8 @@ extern "C" int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_pe
 }
 
 // Launch-shape override: cells per thread of the scalar step kernel (1, 2 or
-// 4; this also turns off the vector kernel of lattices without geometry);
-// 0 restores the default.
+// 4), CS_STEP_VECTOR for the vector kernel wherever rows split into vectors;
+// 0 restores the choice by lattice size.
 // Graph replay of fused steps on launch-bound lattices (on by default).
 extern "C" int cs_set_step_graphs(int enable)
 {
@@ -1379,7 +1404,8 @@ extern "C" int cs_set_step_graphs(int enable)
 extern "C" int cs_set_cells_per_thread(int cpt)
 {
     return guarded([&] {
-        CS_REQUIRE(cpt == 0 || cpt == 1 || cpt == 2 || cpt == 4, CS_ERR_ARG, "cells per thread must be 0, 1, 2 or 4");
+        CS_REQUIRE(cpt == 0 || cpt == 1 || cpt == 2 || cpt == 4 || cpt == CS_STEP_VECTOR, CS_ERR_ARG,
+                   "cells per thread must be 0, 1, 2, 4 or CS_STEP_VECTOR");
         g_cpt_override = cpt;
     });
 }

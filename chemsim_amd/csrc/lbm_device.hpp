@@ -628,13 +628,24 @@ __device__ __forceinline__ S shfl_elem(S v, int src_lane)
 
 // One row per wave (ROWS rows per block), 64*V columns per wave, VB bytes
 // per lane and plane (V = VB / sizeof(S) cells).  w % V == 0.
-template <class St, int EDGE, bool SR, int VB = 16, int ROWS = 4, int COLL = CS_COLL_BGK>
+//
+// With geometry (MODE_REF, MODE_HALF) the lane reads its V node words as one
+// vector after the fluid gather.  MODE_REF: a solid cell takes the pulled
+// value of the opposite direction (lbm.rs:745), i.e. the pulled pairs swap.
+// MODE_HALF: a bounce-back link takes the cell's own pre-stream population of
+// the opposite direction (+ the lid term), a solid cell is carried unchanged,
+// then Zou-He at inlet/outlet cells -- gather()'s order.  The own values of
+// the planes that stream within the row (cx = 0) are the lane's loaded
+// vectors; the other six planes are read at the own row only by lanes that
+// hold a link or a solid cell.
+template <class St, int EDGE, bool SR, int VB = 16, int ROWS = 4, int COLL = CS_COLL_BGK, int MODE = MODE_FLUID>
 __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T> a)
 {
     using S = typename St::S;
     using T = typename St::T;
     constexpr int V = VB / (int)sizeof(S);
     static_assert(V >= 2, "at least two cells per lane");
+    static_assert(MODE == MODE_FLUID || !SR, "geometry in the vector kernel: 32/64-bit storage only");
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / a.tiles_x;
     const int tx = tile - ty * a.tiles_x;
@@ -671,6 +682,7 @@ __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T>
     const int cright_w = cright >= a.w ? 0 : cright;
 
     T B[V][9];
+    S ownv[9][V];  // MODE_HALF: pre-stream values of the lane's own cells
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const int rs = rows[CX[i] + 1];
@@ -678,6 +690,12 @@ __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T>
         const uint32_t rowbase = (uint32_t)i * P + (uint32_t)rs * (uint32_t)a.pitch;
         VecN<S, VB> v;
         v.raw = __builtin_nontemporal_load((const RawVec<VB>*)at<uint32_t>(fin, rowbase + (uint32_t)cl));
+        if constexpr (MODE == MODE_HALF) {
+            if (CX[i] == 0) {
+#pragma unroll
+                for (int j = 0; j < V; ++j) ownv[i][j] = v.e[j];
+            }
+        }
         const T outside = St::DEV ? -a.k.base(i) : T(0);
         if (CY[i] == 0) {
 #pragma unroll
@@ -701,8 +719,81 @@ __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T>
         }
     }
     if (!active) return;
+    bool carried[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) carried[j] = false;
+    if constexpr (MODE != MODE_FLUID) {
+        // cpitch is a multiple of V and c0 % V == 0: one aligned vector load
+        VecN<uint32_t, 4 * V> nv;
+        nv.raw = *(const RawVec<4 * V>*)(a.node + (uint32_t)r * (uint32_t)a.cpitch + (uint32_t)c0);
+        if constexpr (MODE == MODE_REF) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int type = (int)(nv.e[j] >> NODE_TYPE_SHIFT);
+                if (type == CS_WALL || type == CS_LID) {
+#pragma unroll
+                    for (int i = 1; i < 9; ++i) {
+                        if (OPP[i] > i) {
+                            const T t = B[j][i];
+                            B[j][i] = B[j][OPP[i]];
+                            B[j][OPP[i]] = t;
+                        }
+                    }
+                }
+            }
+        } else {
+            bool need = false;
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int type = (int)(nv.e[j] >> NODE_TYPE_SHIFT);
+                need = need || type == CS_WALL || type == CS_LID || (nv.e[j] & 0x1FFu) != 0u;
+            }
+            if (need) {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    if (CX[i] != 0) {
+                        VecN<S, VB> v;
+                        v.raw = *(const RawVec<VB>*)at<uint32_t>(fin, (uint32_t)i * P + (uint32_t)r * (uint32_t)a.pitch +
+                                                                           (uint32_t)c0);
+#pragma unroll
+                        for (int j = 0; j < V; ++j) ownv[i][j] = v.e[j];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const uint32_t info = nv.e[j];
+                    const int type = (int)(info >> NODE_TYPE_SHIFT);
+                    if (type == CS_WALL || type == CS_LID) {
+                        carried[j] = true;
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) B[j][i] = St::wid(ownv[i][j]);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) {
+                            if ((info >> i) & 1u) {
+                                const T v = St::wid(ownv[OPP[i]][j]);
+                                B[j][i] = ((info >> (9 + i)) & 1u) ? v + a.k.lid(i) : v;
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int type = (int)(nv.e[j] >> NODE_TYPE_SHIFT);
+                if constexpr (St::DEV) {
+                    if (type == CS_INLET) zou_he_inlet_dev<T>(B[j], a.k.rho0, a.k.in_ux, a.k.in_uy);
+                    else if (type == CS_OUTLET) zou_he_outlet_dev<T>(B[j], a.k.rho0, a.k.out_rho);
+                } else {
+                    if (type == CS_INLET) zou_he_inlet<T>(B[j], a.k.in_ux, a.k.in_uy);
+                    else if (type == CS_OUTLET) zou_he_outlet<T>(B[j], a.k.out_rho);
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int j = 0; j < V; ++j) {
+        if (carried[j]) continue;  // MODE_HALF solid cell: stored as read
         if constexpr (COLL == CS_COLL_TRT) trt_collide(a.k, B[j]);
         else if constexpr (COLL == CS_COLL_REGULARIZED) reg_collide(a.k, B[j]);
         else if constexpr (COLL == CS_COLL_KBC) kbc_collide(a.k, B[j]);

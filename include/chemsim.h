@@ -340,10 +340,15 @@ int cs_gemm_bf16(cs_ctx* ctx, int M, int N, int K, const uint16_t* A, const uint
  * the average milliseconds per copy (HIP events).  Calibrates bandwidth and
  * the HBM byte counters. */
 int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy);
-/* Launch shape of the step kernel: cells per thread of the scalar kernel (1,
- * 2 or 4, which also disables the 16-byte vector kernel of lattices without
- * geometry; 0 = the built-in default).  Process-wide; for tests and tuning.
- * No reference counterpart (ArrayFire picks its own launch shapes). */
+/* Launch shape of the step kernel.  Both kernels give the same values: the
+ * scalar kernel (one row per wave, 1, 2 or 4 cells per lane) and the vector
+ * kernel (16 bytes per lane and plane, 4- and 8-byte storage, rows that split
+ * into 16-byte vectors).  0 = the built-in choice by lattice size; 1, 2 or 4
+ * = the scalar kernel with that many cells per lane; CS_STEP_VECTOR = the
+ * vector kernel wherever the row splits, else the built-in choice.
+ * Process-wide; for tests and tuning.  No reference counterpart (ArrayFire
+ * picks its own launch shapes). */
+enum { CS_STEP_VECTOR = -1 };
 int cs_set_cells_per_thread(int cells_per_thread);
 /* Replay of captured HIP graphs of 64 fused steps for lattices of at most
  * 2^20 cells, where launch overhead dominates a step (1 = on, the default;

@@ -428,11 +428,12 @@ def test_init_nonequilibrium_matches_oracle(ctx, storage, edge):
 @pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
 @pytest.mark.parametrize("shape", [(16, 5), (264, 7), (520, 3), (1040, 9), (2048, 16)])
 def test_vector_fluid_step_matches_oracle(ctx, storage, edge, shape):
-    """Lattices without geometry whose rows split into 16-byte vectors take the
-    vector kernel for 4- and 8-byte storage (lane shuffles for the row-wise
-    streaming); partial waves, one- and many-wave rows, row counts that do and
-    do not fill a block: value for value with the oracle, and equal to the
-    scalar kernel.  16-bit storage runs the scalar kernel either way."""
+    """The vector kernel (forced with CS_STEP_VECTOR) on lattices without
+    geometry whose rows split into 16-byte vectors, 4- and 8-byte storage
+    (lane shuffles for the row-wise streaming); partial waves, one- and
+    many-wave rows, row counts that do and do not fill a block: value for value
+    with the oracle, and equal to the scalar kernel.  16-bit storage runs the
+    scalar kernel either way."""
     w, h = shape
     kind = storage.split("_")[0]
     sr = storage.endswith("_sr")
@@ -458,7 +459,7 @@ def test_vector_fluid_step_matches_oracle(ctx, storage, edge, shape):
         f0 = narrow16(okind, O.equilibrium_dev(op, rho, ux, uy))
         ref = O.run_dev16(op, okind, f0, steps, None, rounding=1 if sr else 0, seed=99)
     outs = []
-    for cpt in (0, 1):
+    for cpt in (capi.CS_STEP_VECTOR, 1):
         try:
             capi.call("cs_set_cells_per_thread", cpt)
             with Lattice(ctx, w, h, kind, gp) as lat:
@@ -621,3 +622,42 @@ def test_degenerate_shapes_match_oracle(ctx, shape, edge, storage):
             lat.step(3)
             np.testing.assert_array_equal(lat.download(dt), O.run(op, f, 3, flags),
                                           err_msg=f"{shape} flags={with_flags}")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64", "f32dev"])
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+@pytest.mark.parametrize("halfway", [False, True])
+@pytest.mark.parametrize("shape", [(4, 5), (8, 3), (260, 9), (72, 33)])
+def test_vector_geometry_matches_oracle(ctx, storage, edge, halfway, shape):
+    """The vector kernel (forced with CS_STEP_VECTOR) on lattices with
+    geometry whose width splits into vectors: value for value the oracle's run,
+    and the scalar kernel's, in the reference and half-way modes."""
+    w, h = shape
+    rng = np.random.default_rng(w * 131 + h)
+    flags = random_flags(rng, h, w, lids=halfway)
+    rho = (1 + 0.02 * rng.standard_normal((h, w))).astype(np.float32)
+    ux = (0.03 * rng.standard_normal((h, w))).astype(np.float32)
+    uy = (0.03 * rng.standard_normal((h, w))).astype(np.float32)
+    op = O.params(w, h, tau=0.7, edge=edge, boundary=O.BB_HALFWAY if halfway else O.BB_REFERENCE,
+                  lid=(0.01, 0.04) if halfway else (0.0, 0.0))
+    steps = 6
+    if storage == "f32dev":
+        f0 = O.equilibrium_dev(op, rho, ux, uy)
+        ref = O.run_devf32(op, f0, steps, flags)
+    else:
+        dt = STORE[storage]
+        f0 = O.equilibrium(op, *(a.astype(dt) for a in (rho, ux, uy)))
+        ref = O.run(op, f0, steps, flags)
+    print(f"[vector geometry] {storage} {w}x{h} edge={edge} halfway={halfway}", flush=True)
+    outs = []
+    for cpt in (capi.CS_STEP_VECTOR, 1):
+        try:
+            capi.call("cs_set_cells_per_thread", cpt)
+            with Lattice(ctx, w, h, storage, gpu_params(op), flags) as lat:
+                lat.upload_raw(f0)
+                lat.step(steps)
+                outs.append(lat.download_raw())
+        finally:
+            capi.call("cs_set_cells_per_thread", 0)
+    np.testing.assert_array_equal(outs[0], ref)
+    np.testing.assert_array_equal(outs[1], ref)
