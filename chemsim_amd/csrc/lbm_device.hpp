@@ -591,8 +591,8 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
     }
 }
 
-// ---------------------------------------------- vector step, fluid lattices --
-// Lattices without geometry: a lane owns V consecutive cells of a row, so
+// ----------------------------------------------------------- vector step --
+// A lane owns V consecutive cells of a row, so
 // every population plane is read and written with one 8- or 16-byte access
 // per lane (non-temporal loads and stores: every element is used once).  The
 // populations that stream along the row (cy = +-1) come
