@@ -34,6 +34,12 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def hbm_frac(bytes_per_cell: float, cells: int, kernel_ms: float) -> float:
+    """Algorithmic bytes of one step launch over its duration, as a fraction
+    of the MI355X HBM peak."""
+    return round(bytes_per_cell * cells / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
+
 def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr", trt=None, collision=None, ks_viscosity=0.0):
     from chemsim_amd import Lattice, Params, capi
     from chemsim_amd import workloads as W
@@ -129,6 +135,7 @@ def tgv_8192_run(ctx, tau, u0, steps=200):
         r, x, y = lat.moments_f64()
         lat.close()
         out[st] = {"GLUPS": round(cells * steps / w / 1e9, 3), "kernel_us": round(ms / steps * 1e3, 1),
+                   "hbm_frac": hbm_frac(BYTES_PER_CELL[st], cells, ms / steps),
                    "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
                    "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden)}
         del r, x, y
@@ -185,7 +192,8 @@ def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
     cells = b.cells
     kinds = {capi.CS_LAUNCH_PERSISTENT: "persistent", capi.CS_LAUNCH_TILES: "tiles"}
     out = {"lattices": len(b), "cells": cells, "steps": steps,
-           "GLUPS_batch": round(cells * steps / ms / 1e6, 3), "launch": kinds[best.get("launch", 0)],
+           "GLUPS_batch": round(cells * steps / ms / 1e6, 3), "hbm_frac": hbm_frac(72, cells, ms / steps),
+           "launch": kinds[best.get("launch", 0)],
            "GLUPS_" + kinds[other["launch"]]: round(cells * other_steps / ms_other / 1e6, 3),
            "GLUPS_per_lattice_launches": round(cells * per_lattice_steps / ms_pl / 1e6, 3),
            "seconds": round(ms / 1e3, 3), "tuning": rec["tuning"]}
@@ -271,6 +279,7 @@ def main():
         m1 = ref.sums_f64()[0]
         ref.close()
         extras["f64"] = {"GLUPS": round(cells * args.steps / w64 / 1e9, 3),
+                         "hbm_frac": hbm_frac(144, cells, ms64 / args.steps),
                          "mass_drift_rel": abs(m1 - m0) / m0}
         amp = W.taylor_green_decay(n, args.tau, args.steps)
         rho0, ux0, uy0 = W.taylor_green(n, args.u0)
@@ -289,6 +298,7 @@ def main():
             l2.close()
             key = st if st in ("f32", "f32dev") else f"{st}_{rnd}"
             extras[key] = {"GLUPS": round(cells * args.steps / w2 / 1e9, 3),
+                          "hbm_frac": hbm_frac(BYTES_PER_CELL[st], cells, ms2 / args.steps),
                           "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
                           "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden),
                           "mass_drift_rel": abs(m1 - m0) / m0}
@@ -300,6 +310,7 @@ def main():
         wt, mst = time_steps(lt, args.steps)
         lt.close()
         extras["trt_f32"] = {"GLUPS": round(cells * args.steps / wt / 1e9, 3),
+                             "hbm_frac": hbm_frac(72, cells, mst / args.steps),
                              "kernel_us": round(mst / args.steps * 1e3, 2), "tau_plus": tp, "tau_minus": tm}
         log(f"[bench] trt: {extras['trt_f32']}")
         # Regularized (lbm.rs:596-666) and KBC (lbm.rs:458-590, nu of tau 0.8) on the same workload
@@ -310,6 +321,7 @@ def main():
             wo, mso = time_steps(lo, args.steps)
             lo.close()
             extras[key] = {"GLUPS": round(cells * args.steps / wo / 1e9, 3),
+                           "hbm_frac": hbm_frac(72, cells, mso / args.steps),
                            "kernel_us": round(mso / args.steps * 1e3, 2)}
             log(f"[bench] {key}: {extras[key]}")
         # render pass (render.rs) of the 4096^2 state, each display mode; bytes moved per
