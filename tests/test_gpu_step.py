@@ -674,3 +674,26 @@ def test_vector_geometry_matches_oracle(ctx, storage, edge, halfway, shape):
             capi.call("cs_set_cells_per_thread", 0)
     np.testing.assert_array_equal(outs[0], ref)
     np.testing.assert_array_equal(outs[1], ref)
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_cavity_full_size_matches_oracle(ctx, n):
+    """A lid-driven cavity at 2048^2 (the scalar kernel by the size choice) and
+    4096^2 (the vector kernel with geometry), 30 steps from a perturbed rest
+    state: the FP32 run equals the oracle's value for value."""
+    steps = 30
+    flags = W.cavity_flags(n, n)
+    op = O.params(n, n, tau=0.6, boundary=O.BB_HALFWAY, lid=W.lid_velocity(0.1))
+    rng = np.random.default_rng(n)
+    rho = (1 + 0.001 * rng.standard_normal((n, n))).astype(np.float32)
+    ux = (0.001 * rng.standard_normal((n, n))).astype(np.float32)
+    uy = (0.001 * rng.standard_normal((n, n))).astype(np.float32)
+    f0 = O.equilibrium(op, rho, ux, uy).astype(np.float32)
+    del rho, ux, uy
+    print(f"[cavity full size] {n}x{n} {steps} steps", flush=True)
+    with Lattice(ctx, n, n, "f32", gpu_params(op), flags) as lat:
+        lat.upload(f0)
+        lat.step(steps)
+        got = lat.download(np.float32)
+    ref, _ = O.run_omp(op, f0, steps, flags)
+    np.testing.assert_array_equal(got, ref)
