@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -164,5 +165,5 @@ struct cs_lattice {
     // starting buffer parity, valid for the parameters, geometry and launch
     // shape they were captured with (cleared when any of them changes).
     hipGraphExec_t step_graph[2] = {nullptr, nullptr};
-    int graph_cpt = -1;         // cells-per-thread override at capture
+    int graph_cpt = INT_MIN;    // cells-per-thread override at capture (INT_MIN: none captured)
 };

@@ -2,6 +2,7 @@
 // the reference's unfused phases, moments and FP64 reductions.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -372,7 +373,10 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
-static int g_cpt_override = 0;
+// Process-wide launch switches for tests and tuning (cs_set_cells_per_thread,
+// cs_set_step_graphs).  Atomic: lattices may step on other threads while a
+// switch changes; a lattice stepping during the change may launch either way.
+static std::atomic<int> g_cpt_override{0};
 
 template <class St, int EDGE, int MODE, int CPT, bool SR, int COLL>
 static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
@@ -459,7 +463,7 @@ static bool vector_size_class(const cs_lattice* l)
 template <class St, int EDGE, int MODE, bool SR, int COLL>
 static void launch_step_sr(cs_lattice* l, const void* fin, void* fout)
 {
-    const int ov = g_cpt_override;
+    const int ov = g_cpt_override.load(std::memory_order_relaxed);
     if ((ov == CS_STEP_VECTOR || (ov == 0 && vector_size_class(l))) &&
         launch_step_vec<St, EDGE, MODE, SR, COLL>(l, fin, fout))
         return;
@@ -524,7 +528,7 @@ static void launch_step(cs_lattice* l)
 // rounding always launches step by step.
 constexpr int GRAPH_STEPS = 64;
 constexpr int64_t GRAPH_MAX_CELLS = (int64_t)1 << 20;
-static bool g_step_graphs = true;
+static std::atomic<bool> g_step_graphs{true};
 
 void clear_step_graphs(cs_lattice* l)
 {
@@ -539,7 +543,7 @@ static bool graph_eligible(const cs_lattice* l)
 {
     const bool sr16 = (l->storage == CS_BF16DEV || l->storage == CS_F16DEV) &&
                       l->params.rounding == CS_ROUND_STOCHASTIC;
-    return g_step_graphs && !sr16 && (int64_t)l->w * l->h <= GRAPH_MAX_CELLS;
+    return g_step_graphs.load(std::memory_order_relaxed) && !sr16 && (int64_t)l->w * l->h <= GRAPH_MAX_CELLS;
 }
 
 static void step_once(cs_lattice* l)
@@ -551,7 +555,8 @@ static void step_once(cs_lattice* l)
 // GRAPH_STEPS steps from the current parity, captured (first time) and replayed.
 static void replay_steps(cs_lattice* l)
 {
-    if (l->graph_cpt != g_cpt_override) clear_step_graphs(l);
+    const int ov = g_cpt_override.load(std::memory_order_relaxed);
+    if (l->graph_cpt != ov) clear_step_graphs(l);
     const int parity = l->cur;
     if (!l->step_graph[parity]) {
         const int cur0 = l->cur;
@@ -580,7 +585,7 @@ static void replay_steps(cs_lattice* l)
         (void)hipGraphDestroy(graph);
         CS_HIP(e2);
         l->step_graph[parity] = exec;
-        l->graph_cpt = g_cpt_override;
+        l->graph_cpt = ov;
     }
     CS_HIP(hipGraphLaunch(l->step_graph[parity], l->stream));
     for (int s = 0; s < GRAPH_STEPS; ++s) l->time += l->params.delta_t;  // as step by step
@@ -1398,7 +1403,7 @@ extern "C" int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_pe
 // Graph replay of fused steps on launch-bound lattices (on by default).
 extern "C" int cs_set_step_graphs(int enable)
 {
-    return guarded([&] { g_step_graphs = enable != 0; });
+    return guarded([&] { g_step_graphs.store(enable != 0, std::memory_order_relaxed); });
 }
 
 extern "C" int cs_set_cells_per_thread(int cpt)
@@ -1406,6 +1411,6 @@ extern "C" int cs_set_cells_per_thread(int cpt)
     return guarded([&] {
         CS_REQUIRE(cpt == 0 || cpt == 1 || cpt == 2 || cpt == 4 || cpt == CS_STEP_VECTOR, CS_ERR_ARG,
                    "cells per thread must be 0, 1, 2, 4 or CS_STEP_VECTOR");
-        g_cpt_override = cpt;
+        g_cpt_override.store(cpt, std::memory_order_relaxed);
     });
 }
