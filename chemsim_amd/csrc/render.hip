@@ -84,6 +84,79 @@ __global__ __launch_bounds__(256) void k_render_field(const typename St::S* __re
     }
 }
 
+// The field pass for 4-byte storage with rows that split into 4-cell vectors:
+// a thread takes 4 consecutive cells of a row per iteration, reading each
+// population plane and writing the field (and phase) with one 16-byte access.
+// Per-cell values are those of k_render_field.
+template <class St>
+__global__ __launch_bounds__(256) void k_render_field_vec(const typename St::S* __restrict__ f, int w, int h,
+                                                          int pitch, int64_t P, Consts<typename St::T> k, int mode,
+                                                          float* __restrict__ sval, float* __restrict__ phase,
+                                                          double* __restrict__ partials)
+{
+    using S = typename St::S;
+    using T = typename St::T;
+    static_assert(sizeof(S) == 4, "4-byte storage");
+    __shared__ double sh[2][256];
+    const int wq = w >> 2;  // vectors per row
+    const int64_t n4 = (int64_t)wq * h;
+    double s1 = 0.0, s2 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    const int dr = (int)(stride / wq), dq = (int)(stride - (int64_t)dr * wq);
+    int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int r = (int)(g / wq), q = (int)(g - (int64_t)r * wq);
+    for (; g < n4; g += stride, r += dr, q += dq) {
+        if (q >= wq) {
+            q -= wq;
+            ++r;
+        }
+        const int64_t own = (int64_t)r * pitch + 4 * q;
+        VecN<S, 16> v[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) v[i].raw = __builtin_nontemporal_load((const RawVec<16>*)(f + (int64_t)i * P + own));
+        float sv[4], ph[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T B[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) B[i] = St::wid(v[i].e[j]);
+            T rho, mx, my, ux, uy;
+            moments<St>(k, B, rho, mx, my, ux, uy);
+            if (mode == CS_RENDER_DENSITY) {
+                sv[j] = (float)rho;
+            } else if (mode == CS_RENDER_SPEED) {
+                const float vx = (float)ux, vy = (float)uy;
+                sv[j] = sqrtf(vx * vx + vy * vy);  // lbm.rs:151-154
+            } else {
+                const float vx = mode == CS_RENDER_VELOCITY ? (float)ux : (float)mx;
+                const float vy = mode == CS_RENDER_VELOCITY ? (float)uy : (float)my;
+                sv[j] = vx * vx + vy * vy;         // render.rs:112
+                ph[j] = atan2f(vy, vx);            // af::arg(cplx2(vx, vy)), render.rs:113-114
+            }
+            s1 += (double)sv[j];
+            s2 += (double)sv[j] * (double)sv[j];
+        }
+        const int64_t d = (int64_t)r * w + 4 * q;
+        *reinterpret_cast<float4*>(sval + d) = make_float4(sv[0], sv[1], sv[2], sv[3]);
+        if (mode == CS_RENDER_VELOCITY || mode == CS_RENDER_MOMENTUM)
+            *reinterpret_cast<float4*>(phase + d) = make_float4(ph[0], ph[1], ph[2], ph[3]);
+    }
+    sh[0][threadIdx.x] = s1;
+    sh[1][threadIdx.x] = s2;
+    __syncthreads();
+    for (int qq = 128; qq > 0; qq >>= 1) {
+        if ((int)threadIdx.x < qq) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + qq];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + qq];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sh[0][0];
+        partials[2 * blockIdx.x + 1] = sh[1][0];
+    }
+}
+
 // mean and 1/std as the reference uses them: Matrix::new_filled(avg as f32)
 // and scale(1.0 / std as f32) (render.rs:40-45).
 __global__ __launch_bounds__(256) void k_render_stats(const double* __restrict__ partials, int nblocks, int64_t n,
@@ -273,6 +346,14 @@ void render_enqueue(cs_lattice* l, int mode, int overlay, uint8_t* d_out, int ch
     const RenderScratch r = render_scratch(l);
     const int blocks = (int)std::min<int64_t>(RENDER_BLOCKS, (n + 255) / 256);
     auto field = [&]<class St>() {
+        if constexpr (sizeof(typename St::S) == 4) {
+            if (l->w % 4 == 0) {
+                hipLaunchKernelGGL((k_render_field_vec<St>), dim3((unsigned)blocks), dim3(256), 0, l->stream,
+                                   (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
+                                   render_consts<St>(l), mode, r.sval, r.phase, r.partials);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_render_field<St>), dim3((unsigned)blocks), dim3(256), 0, l->stream,
                            (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane,
                            render_consts<St>(l), mode, r.sval, r.phase, r.partials);
