@@ -216,37 +216,41 @@ __device__ __forceinline__ uint8_t to_byte(float v)
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
 // One pixel's colour (render.rs:7-178): z-score, logistic, HSV -> RGB, the
-// geometry overlay; packed as R | G << 8 | B << 16 | 255 << 24.
-__device__ __forceinline__ uint32_t pixel_rgba(float s, float ph, float avg, float inv, int mode, bool wall)
+// geometry overlay; packed as R | G << 8 | B << 16 | 255 << 24.  HUE: the
+// velocity and momentum modes (hue from the phase, saturation 0.8); the
+// scalar modes have hue 0 and saturation 1, where hsv_to_rgb gives exactly
+// (val, 0, 0).
+template <bool HUE>
+__device__ __forceinline__ uint32_t pixel_rgba(float s, float ph, float avg, float inv, bool wall)
 {
     if (wall) return 0u | (0u << 8) | (255u << 16) | (255u << 24);   // render_geometry: blue
     const float z = (s - avg) * inv;
     const float val = clamp01(1.0f / (1.0f + expf(-z)));  // af::sigmoid
-    float hue = 0.0f, sat = 1.0f;
-    if (mode == CS_RENDER_VELOCITY || mode == CS_RENDER_MOMENTUM) {
+    if constexpr (!HUE) {
+        return (uint32_t)to_byte(val) | (255u << 24);
+    } else {
         const float pi = 3.14159265358979323846f;
-        hue = clamp01((ph + pi) * (0.318309886183790671538f * 0.5f));  // render.rs:121-125
-        sat = 0.8f;
+        const float hue = clamp01((ph + pi) * (0.318309886183790671538f * 0.5f));  // render.rs:121-125
+        float r, g, b;
+        hsv_to_rgb(hue, 0.8f, val, r, g, b);
+        return (uint32_t)to_byte(r) | ((uint32_t)to_byte(g) << 8) | ((uint32_t)to_byte(b) << 16) | (255u << 24);
     }
-    float r, g, b;
-    hsv_to_rgb(hue, sat, val, r, g, b);
-    return (uint32_t)to_byte(r) | ((uint32_t)to_byte(g) << 8) | ((uint32_t)to_byte(b) << 16) | (255u << 24);
 }
 
 // Pixel pass: 4 pixels per thread.  CH = 4 writes RGBA (RGB::to_rgba,
 // display.rs:27-29) as one 16-byte store, CH = 3 packed RGB as three 4-byte
 // stores; `rgb`, `sval` and `phase` are 16-byte aligned.
-template <int CH>
+template <int CH, bool HUE>
 __global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__ sval, const float* __restrict__ phase,
                                                        const float* __restrict__ stats,
                                                        const uint8_t* __restrict__ flags, int w, int h, int cpitch,
-                                                       int mode, int overlay, uint8_t* __restrict__ rgb)
+                                                       int overlay, uint8_t* __restrict__ rgb)
 {
     const int64_t n = (int64_t)w * h;
     const int64_t d0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
     if (d0 >= n) return;
     const bool full = d0 + 4 <= n;
-    const bool vec = mode == CS_RENDER_VELOCITY || mode == CS_RENDER_MOMENTUM;
+    constexpr bool vec = HUE;
     float s[4], ph[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (full) {
         const float4 v = *reinterpret_cast<const float4*>(sval + d0);
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(256) void k_render_pixels(const float* __restrict__
     const float avg = stats[0], inv = stats[1];
     uint32_t px[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) px[j] = pixel_rgba(s[j], ph[j], avg, inv, mode, wall[j]);
+    for (int j = 0; j < 4; ++j) px[j] = pixel_rgba<HUE>(s[j], ph[j], avg, inv, wall[j]);
     if constexpr (CH == 4) {
         if (full) {
             *reinterpret_cast<uint4*>(rgb + 4 * d0) = make_uint4(px[0], px[1], px[2], px[3]);
@@ -369,12 +373,18 @@ void render_enqueue(cs_lattice* l, int mode, int overlay, uint8_t* d_out, int ch
     hipLaunchKernelGGL(k_render_stats, dim3(1), dim3(256), 0, l->stream, r.partials, blocks, n, r.stats);
     const uint8_t* fl = l->has_flags ? (const uint8_t*)l->flags : nullptr;
     const unsigned pblocks = (unsigned)((n + 1023) / 1024);  // 4 pixels per thread
-    if (channels == 4)
-        hipLaunchKernelGGL(k_render_pixels<4>, dim3(pblocks), dim3(256), 0, l->stream, r.sval, r.phase, r.stats, fl,
-                           l->w, l->h, l->cpitch, mode, overlay, d_out);
-    else
-        hipLaunchKernelGGL(k_render_pixels<3>, dim3(pblocks), dim3(256), 0, l->stream, r.sval, r.phase, r.stats, fl,
-                           l->w, l->h, l->cpitch, mode, overlay, d_out);
+    auto pixels = [&]<int CH, bool HUE>() {
+        hipLaunchKernelGGL((k_render_pixels<CH, HUE>), dim3(pblocks), dim3(256), 0, l->stream, r.sval, r.phase,
+                           r.stats, fl, l->w, l->h, l->cpitch, overlay, d_out);
+    };
+    const bool hue = mode == CS_RENDER_VELOCITY || mode == CS_RENDER_MOMENTUM;
+    if (channels == 4) {
+        if (hue) pixels.template operator()<4, true>();
+        else pixels.template operator()<4, false>();
+    } else {
+        if (hue) pixels.template operator()<3, true>();
+        else pixels.template operator()<3, false>();
+    }
     CS_HIP(hipGetLastError());
 }
 
