@@ -353,7 +353,8 @@ struct cs_cnn {
     float* dy32 = nullptr;            // two f32 gradient buffers of act_max elements each
     int64_t act_max = 0;
     float* skip32[3] = {nullptr, nullptr, nullptr};  // accumulated gradients of e1, e2, e3 outputs
-    uint16_t* dz = nullptr;           // bf16 gradient (largest activation)
+    uint16_t* dz = nullptr;           // bf16 gradients of a conv's output and input (largest activation each)
+    uint16_t* dz2 = nullptr;
     uint16_t* dzT = nullptr;          // its transpose [C][P] for the weight gradient
     float* loss_part = nullptr;
     float* targets = nullptr;
@@ -382,9 +383,11 @@ T* dalloc(cs_cnn* net, int64_t n)
 
 void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, const uint16_t* B, int ldb, void* C, int ldc,
           const float* bias, int M, int N, int K, int splits = 1, const Act* gather = nullptr, int sign = 1,
-          int dil = 1)
+          int dil = 1, const uint16_t* mask = nullptr)
 {
     GemmArgs g{};
+    CS_REQUIRE(epi != EPI_BF16_MASK || mask, CS_ERR_ARG, "masked epilogue without a mask");
+    g.mask = mask;
     if (lb == SHIFT) {
         CS_REQUIRE(la != GATHER && gather && gather->C % 8 == 0 && (int64_t)K == gather->P() && dil >= 1, CS_ERR_ARG,
                    "bad shifted operand");
@@ -429,6 +432,7 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
     } else if (la == GATHER && lb == KMAJOR) {
         if (epi == EPI_F32) CS_GEMM(GATHER, KMAJOR, EPI_F32);
         else if (epi == EPI_BF16) CS_GEMM(GATHER, KMAJOR, EPI_BF16);
+        else if (epi == EPI_BF16_MASK) CS_GEMM(GATHER, KMAJOR, EPI_BF16_MASK);
         else CS_GEMM(GATHER, KMAJOR, EPI_BF16_RELU);
     } else {
         throw Error{CS_ERR_ARG, "unsupported gemm layout"};
@@ -459,22 +463,28 @@ void conv_forward(cs_cnn* net, int li, const Act& x, const Act& y, float* y32 = 
              (int)x.P(), c.cout, c.kpad, 1, &x, 1, c.dil);
 }
 
-// Backward of conv li given the f32 gradient dy of its output (post-ReLU).
-// Writes dW, dB and, when dx32 is not null, the f32 input gradient.
-void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float* dy, float* dx32)
+// Backward of conv li.  The gradient of its output comes either as f32 (dy32,
+// then masked by the layer's ReLU into dz) or, with dy32 null, already masked
+// in bf16 in dz.  Writes dW, dB and, when dz_out is not null, the gradient of
+// its input in bf16: masked by the input's own ReLU when mask_in (the input is
+// the previous conv's ReLU output), plain otherwise (pool and concat outputs).
+void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float* dy32, uint16_t* dz,
+                   uint16_t* dz_out, bool mask_in)
 {
     Conv& c = net->convs[(size_t)li];
     const int64_t P = x.P();
-    hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(P * c.cout)), dim3(256), 0, net->stream, dy, y.d, net->dz,
-                       P * c.cout, c.relu ? 1 : 0);
-    CS_HIP(hipGetLastError());
-    CS_HIP(hipGetLastError());
+    CS_REQUIRE(dz != dz_out, CS_ERR_STATE, "gradient buffers alias");
+    if (dy32) {
+        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(P * c.cout)), dim3(256), 0, net->stream, dy32, y.d, dz,
+                           P * c.cout, c.relu ? 1 : 0);
+        CS_HIP(hipGetLastError());
+    }
     CS_REQUIRE(c.cout >= 8 && c.cout <= 256 && 256 % c.cout == 0 && P % 8 == 0, CS_ERR_STATE,
                "unsupported layer shape");
     {
         const int64_t blocks = std::min<int64_t>(1024, (P + 255) / 256);
         const int64_t rows = (P + blocks - 1) / blocks;
-        hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)blocks), dim3(256), 0, net->stream, net->dz, net->part, P,
+        hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)blocks), dim3(256), 0, net->stream, dz, net->part, P,
                            c.cout, rows);
         hipLaunchKernelGGL(k_colsum, dim3((unsigned)c.cout), dim3(256), 0, net->stream, net->part, c.gb, (int)blocks,
                            c.cout);
@@ -484,7 +494,7 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
     // dz transposed so that p (the reduction) is contiguous, colT read from
     // the three column-shifted transposed copies of x
     hipLaunchKernelGGL(k_transpose16, dim3((unsigned)((P + 63) / 64), (unsigned)((c.cout + 63) / 64)), dim3(256), 0,
-                       net->stream, net->dz, net->dzT, P, c.cout);
+                       net->stream, dz, net->dzT, P, c.cout);
     CS_HIP(hipGetLastError());
     shift3_t(net, x, c.dil);
     // enough K ranges for ~1024 workgroups, each at least 1024 pixels long
@@ -498,11 +508,13 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
     hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream, net->part,
                        c.gw, c.nw(), splits);
     CS_HIP(hipGetLastError());
-    if (!dx32) return;
-    // data gradient: dx[p][ci] = sum_k colT(dz)[p][k] wd[ci][k]
-    Act dzact{net->dz, x.N, x.H, x.W, c.cout};
-    gemm(net, GATHER, KMAJOR, EPI_F32, net->dz, 0, c.wd, c.kpad_d, dx32, c.cin, nullptr, (int)P, c.cin, c.kpad_d, 1,
-         &dzact, -1, c.dil);
+    if (!dz_out) return;
+    // data gradient: dx[p][ci] = sum_k colT(dz)[p][k] wd[ci][k], stored in
+    // bf16 (masked by x's ReLU when mask_in) -- the conversion the next layer's
+    // backward would make from f32, done in the epilogue
+    Act dzact{dz, x.N, x.H, x.W, c.cout};
+    gemm(net, GATHER, KMAJOR, mask_in ? EPI_BF16_MASK : EPI_BF16, dz, 0, c.wd, c.kpad_d, dz_out, c.cin, nullptr,
+         (int)P, c.cin, c.kpad_d, 1, &dzact, -1, c.dil, mask_in ? x.d : nullptr);
 }
 
 void forward(cs_cnn* net, int n)
@@ -554,101 +566,65 @@ void forward(cs_cnn* net, int n)
 }
 
 // Backward from the f32 gradient of the head output (net->dy32 holds it on entry).
+// Between consecutive convs the gradient stays in bf16 (two buffers used in
+// turn); where gradients add up (concat and pool skips) it is accumulated in
+// f32 and masked by the next conv's backward.
 void backward(cs_cnn* net, int n)
 {
     auto& ci = net->conv_in;
     auto& co = net->conv_out;
     hipStream_t s = net->stream;
     const int H = net->H, W = net->W;
-    float* g = net->dy32;       // gradient of the current conv output
-    float* gin = nullptr;       // gradient of the current conv input
-    float* buf[2] = {net->dy32, net->dy32 + net->act_max};
-    int cur = 0;
-    auto step = [&](int li, bool need_dx) {
-        gin = need_dx ? buf[cur ^ 1] : nullptr;
-        conv_backward(net, li, ci[(size_t)li], co[(size_t)li], g, gin);
-        cur ^= 1;
-        g = gin;
+    uint16_t* zb[2] = {net->dz, net->dz2};
+    int zc = 0;                     // zb[zc]: bf16 gradient of the current conv's output
+    const float* g32 = net->dy32;   // ... or, when not null, its f32 gradient
+    float* dcoarse = net->dy32 + net->act_max;
+    // input gradient kinds: 0 none, 1 masked by the previous conv's ReLU, 2 plain
+    auto step = [&](int li, int kind) {
+        const bool mask = kind == 1 && net->convs[(size_t)li - 1].relu;
+        conv_backward(net, li, ci[(size_t)li], co[(size_t)li], g32, zb[zc], kind ? zb[zc ^ 1] : nullptr, mask);
+        g32 = nullptr;
+        zc ^= 1;
+    };
+    auto concat_back = [&](int C1, int C2, int Hc, int Wc, float* skip, int64_t nskip) {
+        // zb[zc]: bf16 gradient of the concat; coarse part -> dcoarse (f32),
+        // skip part accumulated into the skip gradient
+        CS_HIP(hipMemsetAsync(skip, 0, sizeof(float) * (size_t)nskip, s));
+        const int64_t nmax = std::max<int64_t>((int64_t)n * (Hc / 2) * (Wc / 2) * C1, nskip);
+        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, zb[zc], dcoarse, skip, n, Hc,
+                           Wc, C1, C2);
+        CS_HIP(hipGetLastError());
+        g32 = dcoarse;
+    };
+    auto pool_back = [&](int Hc, int Wc, int C, float* skip) {
+        // zb[zc]: bf16 gradient of the pooled tensor, added to the skip gradient
+        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for((int64_t)n * Hc * Wc * C)), dim3(256), 0, s, zb[zc], skip, n,
+                           Hc, Wc, C);
+        CS_HIP(hipGetLastError());
+        g32 = skip;
     };
     // head and d1
-    step(14, true);
-    step(13, true);
-    step(12, true);  // gradient of cat1 (f32, channels co[11].C + co[1].C)
-    // split cat1: coarse part -> gradient of co[11] (f32), skip part -> e1 skip accumulator
-    CS_HIP(hipMemsetAsync(net->skip32[0], 0, sizeof(float) * (size_t)co[1].n(), s));
-    {
-        float* dcoarse = buf[cur ^ 1];
-        Act cat1 = net->cat1;
-        // k_up_concat_back reads bf16 gradients: convert the f32 gradient first
-        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(cat1.n())), dim3(256), 0, s, g, cat1.d, net->dz, cat1.n(), 0);
-        const int64_t nmax = std::max<int64_t>((int64_t)n * (H / 2) * (W / 2) * co[11].C, co[1].n());
-        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, net->dz, dcoarse, net->skip32[0],
-                           n, H, W, co[11].C, co[1].C);
-        CS_HIP(hipGetLastError());
-        cur ^= 1;
-        g = dcoarse;
-    }
-    step(11, true);
-    step(10, true);  // gradient of cat2
-    CS_HIP(hipMemsetAsync(net->skip32[1], 0, sizeof(float) * (size_t)co[3].n(), s));
-    {
-        float* dcoarse = buf[cur ^ 1];
-        Act cat2 = net->cat2;
-        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(cat2.n())), dim3(256), 0, s, g, cat2.d, net->dz, cat2.n(), 0);
-        const int64_t nmax = std::max<int64_t>((int64_t)n * (H / 4) * (W / 4) * co[9].C, co[3].n());
-        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, net->dz, dcoarse, net->skip32[1],
-                           n, H / 2, W / 2, co[9].C, co[3].C);
-        CS_HIP(hipGetLastError());
-        cur ^= 1;
-        g = dcoarse;
-    }
-    step(9, true);
-    step(8, true);  // gradient of cat3
-    CS_HIP(hipMemsetAsync(net->skip32[2], 0, sizeof(float) * (size_t)co[5].n(), s));
-    {
-        float* dcoarse = buf[cur ^ 1];
-        Act cat3 = net->cat3;
-        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(cat3.n())), dim3(256), 0, s, g, cat3.d, net->dz, cat3.n(), 0);
-        const int64_t nmax = std::max<int64_t>((int64_t)n * (H / 8) * (W / 8) * co[7].C, co[5].n());
-        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, net->dz, dcoarse, net->skip32[2],
-                           n, H / 4, W / 4, co[7].C, co[5].C);
-        CS_HIP(hipGetLastError());
-        cur ^= 1;
-        g = dcoarse;
-    }
-    step(7, true);
-    step(6, true);  // gradient of p3 (f32)
-    // pool backward into the e3 skip accumulator -> total gradient of co[5]
-    {
-        Act p3 = net->p3;
-        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(p3.n())), dim3(256), 0, s, g, p3.d, net->dz, p3.n(), 0);
-        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for(co[5].n())), dim3(256), 0, s, net->dz, net->skip32[2], n,
-                           H / 4, W / 4, co[5].C);
-        CS_HIP(hipGetLastError());
-        g = net->skip32[2];
-    }
-    step(5, true);
-    step(4, true);  // gradient of p2
-    {
-        Act p2 = net->p2;
-        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(p2.n())), dim3(256), 0, s, g, p2.d, net->dz, p2.n(), 0);
-        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for(co[3].n())), dim3(256), 0, s, net->dz, net->skip32[1], n,
-                           H / 2, W / 2, co[3].C);
-        CS_HIP(hipGetLastError());
-        g = net->skip32[1];
-    }
-    step(3, true);
-    step(2, true);  // gradient of p1
-    {
-        Act p1 = net->p1;
-        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(p1.n())), dim3(256), 0, s, g, p1.d, net->dz, p1.n(), 0);
-        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for(co[1].n())), dim3(256), 0, s, net->dz, net->skip32[0], n, H,
-                           W, co[1].C);
-        CS_HIP(hipGetLastError());
-        g = net->skip32[0];
-    }
-    step(1, true);
-    step(0, false);
+    step(14, 1);
+    step(13, 1);
+    step(12, 2);  // gradient of cat1 (channels co[11].C + co[1].C)
+    concat_back(co[11].C, co[1].C, H, W, net->skip32[0], co[1].n());
+    step(11, 1);
+    step(10, 2);  // gradient of cat2
+    concat_back(co[9].C, co[3].C, H / 2, W / 2, net->skip32[1], co[3].n());
+    step(9, 1);
+    step(8, 2);   // gradient of cat3
+    concat_back(co[7].C, co[5].C, H / 4, W / 4, net->skip32[2], co[5].n());
+    step(7, 1);
+    step(6, 2);   // gradient of p3 -> the e3 skip accumulator, total gradient of co[5]
+    pool_back(H / 4, W / 4, co[5].C, net->skip32[2]);
+    step(5, 1);
+    step(4, 2);   // gradient of p2
+    pool_back(H / 2, W / 2, co[3].C, net->skip32[1]);
+    step(3, 1);
+    step(2, 2);   // gradient of p1
+    pool_back(H, W, co[1].C, net->skip32[0]);
+    step(1, 1);
+    step(0, 0);
 }
 
 void prepare_weights(cs_cnn* net)
@@ -803,6 +779,7 @@ extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int b
             net->act_max = amax2;
             net->dy32 = dalloc<float>(net, 2 * amax2);
             net->dz = dalloc<uint16_t>(net, amax2);
+            net->dz2 = dalloc<uint16_t>(net, amax2);
             net->dzT = dalloc<uint16_t>(net, amax2);
             net->skip32[0] = dalloc<float>(net, net->conv_out[1].n());
             net->skip32[1] = dalloc<float>(net, net->conv_out[3].n());

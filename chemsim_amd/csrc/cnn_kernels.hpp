@@ -46,7 +46,8 @@ enum Layout : int { KMAJOR = 0, GATHER = 2, SHIFT = 3 };
 enum Epi : int {
     EPI_F32 = 0,        // C (+ bias) stored as f32
     EPI_BF16 = 1,       // C (+ bias) stored as bf16
-    EPI_BF16_RELU = 2   // max(C + bias, 0) stored as bf16
+    EPI_BF16_RELU = 2,  // max(C + bias, 0) stored as bf16
+    EPI_BF16_MASK = 3   // C (+ bias) where mask > 0, else 0, stored as bf16 (ReLU backward)
 };
 
 struct GemmArgs {
@@ -60,6 +61,7 @@ struct GemmArgs {
     int64_t split_stride;  // elements between the partial outputs of consecutive z
     int gH, gW, gC, gsign;  // GATHER: input height, width, channels (multiple of 8), tap sign (+1 / -1)
     int gdil;               // GATHER: tap spacing (dilation)
+    const uint16_t* mask;   // EPI_BF16_MASK: [M][ldc] bf16 ReLU output whose zeros mask C
 };
 
 constexpr int BK = 32;
@@ -310,6 +312,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
                     ((float*)g.C)[(int64_t)blockIdx.z * g.split_stride + (int64_t)row * g.ldc + col] = v;
                 } else {
                     if (EPI == EPI_BF16_RELU) v = v > 0.f ? v : 0.f;
+                    if (EPI == EPI_BF16_MASK && !(bf16_f32(g.mask[(int64_t)row * g.ldc + col]) > 0.f)) v = 0.f;
                     ((uint16_t*)g.C)[(int64_t)row * g.ldc + col] = bf16_rne(v);
                 }
             }
