@@ -407,8 +407,9 @@ static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
 // Lattices with 4- or 8-byte storage, with or without geometry, take the
 // vector kernel (16 bytes per lane and plane, 4 rows per block) when a row
 // splits into vectors and the buffers allow 32-bit offsets.  16-bit storage
-// keeps the scalar kernel with 4 cells per lane, which measured faster (124
-// vs 144 us per 4096^2 step with stochastic rounding).
+// keeps the scalar kernel with 4 cells per lane, which measured faster (4096^2
+// FP16 with stochastic rounding: 114 us against 132-171 us for vector
+// variants with 4-, 8- and 16-byte lanes).
 template <class St, int EDGE, int MODE, bool SR, int COLL>
 static bool launch_step_vec(cs_lattice* l, const void* fin, void* fout)
 {
