@@ -592,15 +592,14 @@ __global__ __launch_bounds__(256) void k_step(StepArgs<typename St::T> a)
 }
 
 // ----------------------------------------------------------- vector step --
-// A lane owns V consecutive cells of a row, so
-// every population plane is read and written with one 8- or 16-byte access
-// per lane (non-temporal loads and stores: every element is used once).  The
-// populations that stream along the row (cy = +-1) come
-// from the lane's own vector shifted by one cell; the cell that crosses into
-// the neighbouring lane is exchanged with a lane shuffle, and the first and
-// last lane of a wave (and the last active lane of a row) read their outside
-// cell directly.  The per-cell arithmetic and the stored values are those of
-// gather/finish, so the results equal k_step bit for bit.
+// A lane owns V consecutive cells of a row, so every population plane is read
+// and written with one 8- or 16-byte access per lane (non-temporal loads and
+// stores: every element is used once).  The populations that stream along the
+// row (cy = +-1) come from the lane's own vector shifted by one cell; the cell
+// that crosses into the neighbouring lane is exchanged with a lane shuffle,
+// and the first and last lane of a wave (and the last active lane of a row)
+// read their outside cell directly.  The per-cell arithmetic and the stored
+// values are those of gather/finish, so the results equal k_step bit for bit.
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
 template <int VB>
