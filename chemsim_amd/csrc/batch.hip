@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <functional>
 #include <numeric>
 #include <type_traits>
@@ -281,10 +282,14 @@ struct cs_batch {
     LatState* d_state = nullptr;
     bool lats_dirty = true;
     std::vector<LatState> state;      // per lattice: current buffer, steps taken
-    // host copies behind asynchronous uploads (kept alive until the next upload,
-    // which follows the stream's earlier work)
+    // Host sources of the asynchronous uploads.  A source must stay alive and
+    // unchanged until the copy has run; every batch call synchronises its
+    // stream before it returns, and each source is rewritten only by the next
+    // call's upload.
     std::vector<int> host_order;
     std::vector<TileRef> host_tiles;
+    std::vector<unsigned char> host_lats;  // BatchLat<T> descriptors
+    std::vector<LatState> host_state;     // `state` as of the launch
     cs_batch_tuning tuning{256, 2, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_PERSISTENT};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -348,11 +353,14 @@ static void upload_descriptors(cs_batch* b)
     if (!b->lats_dirty) return;
     dispatch_batch_storage(b->storage, [&]<class St>() {
         using T = typename St::T;
-        std::vector<BatchLat<T>> host((size_t)b->n);
-        for (int i = 0; i < b->n; ++i) host[(size_t)i] = describe<T>(b, i);
+        b->host_lats.assign(sizeof(BatchLat<T>) * (size_t)b->n, 0);
+        for (int i = 0; i < b->n; ++i) {
+            const BatchLat<T> d = describe<T>(b, i);
+            std::memcpy(b->host_lats.data() + sizeof d * (size_t)i, &d, sizeof d);
+        }
         if (!b->d_lats) CS_HIP(hipMalloc(&b->d_lats, sizeof(BatchLat<T>) * (size_t)b->n));
-        CS_HIP(hipMemcpyAsync(b->d_lats, host.data(), sizeof(BatchLat<T>) * (size_t)b->n, hipMemcpyHostToDevice,
-                              b->stream));
+        CS_HIP(hipMemcpyAsync(b->d_lats, b->host_lats.data(), sizeof(BatchLat<T>) * (size_t)b->n,
+                              hipMemcpyHostToDevice, b->stream));
     });
     b->lats_dirty = false;
 }
@@ -421,9 +429,12 @@ static void launch_group(cs_batch* b, int group, int first, int count, int steps
     else go_mode.template operator()<CS_EDGE_ZERO>();
 }
 
+// The launch's per-lattice state goes up from its own copy: advance_state
+// changes `state` right after the launch is enqueued.
 static void upload_state(cs_batch* b)
 {
-    CS_HIP(hipMemcpyAsync(b->d_state, b->state.data(), sizeof(LatState) * (size_t)b->n, hipMemcpyHostToDevice,
+    b->host_state = b->state;
+    CS_HIP(hipMemcpyAsync(b->d_state, b->host_state.data(), sizeof(LatState) * (size_t)b->n, hipMemcpyHostToDevice,
                           b->stream));
 }
 
@@ -631,6 +642,11 @@ extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const i
                 throw Error{CS_ERR_ARG, "bad lattice size"};
             }
             const int p = (w[i] + align - 1) / align * align;
+            // the batch kernels address a lattice's buffer with 32-bit byte offsets
+            if ((int64_t)9 * p * h[i] * (int64_t)b->esize >= ((int64_t)1 << 32)) {
+                delete b;
+                throw Error{CS_ERR_UNSUPPORTED, "lattice too large for a batch (one population buffer >= 4 GiB)"};
+            }
             b->w.push_back(w[i]);
             b->h.push_back(h[i]);
             b->pitch.push_back(9 * p);  // row-interleaved populations (as the single lattice)

@@ -2,6 +2,7 @@
 // the reference's unfused phases, moments and FP64 reductions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -16,9 +17,20 @@
 namespace cs {
 
 // ------------------------------------------------------------------ errors --
-static thread_local std::string g_error;
-void set_error(const std::string& msg) { g_error = msg; }
-void clear_error() { g_error.clear(); }
+// The message of the calling thread's last failed call.  A plain byte array:
+// thread-local objects with destructors are destroyed when their thread ends,
+// before static objects, so a std::string here would be written to after its
+// buffer was freed by any call made from a static destructor at process exit
+// (such as a host layer's device singleton destroying its context) -- that
+// write lands in a freed heap chunk and corrupts the allocator's free lists.
+static thread_local char g_error[512];
+void set_error(const std::string& msg)
+{
+    const size_t n = std::min(msg.size(), sizeof g_error - 1);
+    std::memcpy(g_error, msg.data(), n);
+    g_error[n] = '\0';
+}
+void clear_error() { g_error[0] = '\0'; }
 
 size_t storage_bytes(int storage)
 {
@@ -687,7 +699,7 @@ using namespace cs;
 
 // ======================================================================= ABI
 
-extern "C" const char* cs_last_error(void) { return g_error.c_str(); }
+extern "C" const char* cs_last_error(void) { return g_error; }
 extern "C" int cs_abi_version(void) { return CS_ABI_VERSION; }
 
 extern "C" int cs_ctx_create(int device, cs_ctx** out)
@@ -814,10 +826,10 @@ extern "C" int cs_lattice_set_params(cs_lattice* l, const cs_params* p)
             p->rho0 != l->params.rho0) {
             const int64_t n = (int64_t)9 * l->w * l->h;
             std::vector<float> tmp((size_t)n);
-            if (cs_lattice_download_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};
+            if (cs_lattice_download_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, std::string(g_error)};
             l->params = *p;
             set_consts(l, *p);
-            if (cs_lattice_upload_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};
+            if (cs_lattice_upload_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, std::string(g_error)};
         }
         l->params = *p;
         set_consts(l, *p);
@@ -916,7 +928,7 @@ extern "C" int cs_lattice_upload_f64(cs_lattice* l, const double* soa)
         const size_t n = (size_t)9 * l->w * l->h;
         std::vector<float> tmp(n);
         for (size_t j = 0; j < n; ++j) tmp[j] = (float)soa[j];
-        if (cs_lattice_upload_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};
+        if (cs_lattice_upload_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, std::string(g_error)};
     });
 }
 
@@ -932,7 +944,7 @@ extern "C" int cs_lattice_download_f64(cs_lattice* l, double* soa)
         }
         const size_t n = (size_t)9 * l->w * l->h;
         std::vector<float> tmp(n);
-        if (cs_lattice_download_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, g_error};
+        if (cs_lattice_download_f32(l, tmp.data()) != CS_OK) throw Error{CS_ERR_HIP, std::string(g_error)};
         for (size_t j = 0; j < n; ++j) soa[j] = (double)tmp[j];
     });
 }

@@ -563,8 +563,10 @@ public:
     {
         if (device_operator()) {
             prepare();
+            // handle() uploads host edits first; only then is the host copy stale
+            cs_lattice* h = lattice->handle();
             lattice->invalidate_host();
-            check(cs_lattice_step(lattice->handle(), 1));
+            check(cs_lattice_step(h, 1));
         } else {
             stream();
             bounce_back();
@@ -622,6 +624,24 @@ public:
         int u = 0;
         check(cs_lattice_is_unstable(lattice->handle(), &u));
         return u != 0;
+    }
+
+    // The device lattice with this state's parameters, geometry and host
+    // edits uploaded, for device work that only reads the populations (the
+    // render pass).  The host copy of the populations stays valid.
+    cs_lattice* device_view() const
+    {
+        prepare();
+        return lattice->handle();
+    }
+    // The same for device work that changes the populations (frame
+    // recording): the host copy is stale from here on and is downloaded again
+    // on the next host read.
+    cs_lattice* device_lattice()
+    {
+        cs_lattice* h = device_view();
+        lattice->invalidate_host();
+        return h;
     }
 
 private:

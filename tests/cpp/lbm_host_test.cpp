@@ -14,7 +14,15 @@
 //   trt_tau.f32   (tau_plus, tau_minus)
 //   reg.f32, reg_eval.f32  the same with Regularized::new(BGK::new(0.6))
 //   kbc.f32, kbc_eval.f32  the same with KBC::new(0.05)
+//   edit.f32      5 fused steps, populations_mut() edited, 5 more
+//   render.u8     display::render of the four modes with the geometry, after
+//                 the 20 fused steps (4 x h x w x 3)
+//   record.u8     display::record: 3 frames of 4 steps, velocity, depth 2
+//                 (3 x h x w x 4); record.f32 the populations afterwards
+//   stop.txt      frames delivered before a throwing write_frame stopped a
+//                 recording, and the steps the state advanced
 // `lbm_host_test cpu` runs only checks that need no GPU.
+#include <display.hpp>
 #include <lbm.hpp>
 
 #include <cstdio>
@@ -123,6 +131,12 @@ static void write(const std::string& path, const std::vector<float>& v)
     f.write((const char*)v.data(), (std::streamsize)(v.size() * sizeof(float)));
 }
 
+static void write_bytes(const std::string& path, const std::vector<uint8_t>& v)
+{
+    std::ofstream f(path, std::ios::binary);
+    f.write((const char*)v.data(), (std::streamsize)v.size());
+}
+
 static std::vector<float> flat(const lbm::Populations& p)
 {
     std::vector<float> out;
@@ -176,6 +190,57 @@ int main(int argc, char** argv)
             for (const auto& f : {s.density(), s.velocity().first, s.velocity().second, s.speed(), s.pressure()})
                 m.insert(m.end(), f.data().begin(), f.data().end());
             write(out + "/moments.f32", m);
+            std::vector<uint8_t> img;
+            for (auto mode : {display::DisplayMode::Density, display::DisplayMode::Speed,
+                              display::DisplayMode::Velocity, display::DisplayMode::MomentumDensity}) {
+                const auto rgb = display::render(s, mode, true);
+                img.insert(img.end(), rgb.begin(), rgb.end());
+            }
+            write_bytes(out + "/render.u8", img);
+            // rendering reads the device state only: the host copy is still valid
+            write(out + "/fused_after_render.f32", flat(s.populations()));
+        }
+        {
+            // host edits between steps reach the device before the next step
+            auto s = make_state(w, h, std::make_unique<lbm::BGK>(0.6f));
+            for (int i = 0; i < 5; ++i) s.step();
+            auto& pm = s.lattice->populations_mut();
+            pm[2].second.at(30, 20) += 0.01f;
+            pm[5].second.at(31, 21) = 0.0f;
+            for (int i = 0; i < 5; ++i) s.step();
+            write(out + "/edit.f32", flat(s.populations()));
+        }
+        {
+            auto s = make_state(w, h, std::make_unique<lbm::BGK>(0.6f));
+            std::vector<uint8_t> frames;
+            const int64_t got = display::record(s, 3, 4, display::DisplayMode::Velocity, true, 2,
+                                                [&](int64_t k, const uint8_t* rgba, int fw, int fh) {
+                                                    EXPECT(k == (int64_t)frames.size() / (4 * (int64_t)w * h));
+                                                    EXPECT(fw == (int)w && fh == (int)h);
+                                                    frames.insert(frames.end(), rgba, rgba + 4 * (size_t)fw * fh);
+                                                });
+            EXPECT(got == 3 && s.time == 12.0f);
+            write_bytes(out + "/record.u8", frames);
+            write(out + "/record.f32", flat(s.populations()));
+        }
+        {
+            // a write_frame that throws stops the recording; the exception
+            // comes back once the frames in flight are done
+            auto s = make_state(w, h, std::make_unique<lbm::BGK>(0.6f));
+            int delivered = 0;
+            bool caught = false;
+            try {
+                display::record(s, 6, 1, display::DisplayMode::Density, false, 3,
+                                [&](int64_t k, const uint8_t*, int, int) {
+                                    if (k == 2) throw std::runtime_error("encoder full");
+                                    ++delivered;
+                                });
+            } catch (const std::runtime_error& e) {
+                caught = std::string(e.what()) == "encoder full";
+            }
+            EXPECT(caught && delivered == 2);
+            std::ofstream f(out + "/stop.txt");
+            f << delivered << " " << s.time << "\n";
         }
         {
             auto s = make_state(w, h, std::make_unique<lbm::BGK>(0.6f));

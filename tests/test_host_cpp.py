@@ -1,9 +1,12 @@
 """The C++ drop-in layer (chemsim_amd/host/lbm.hpp), driven by
 tests/cpp/lbm_host_test.cpp the way main.rs drives lbm.rs, against the
 oracle: the fused BGK step, the three phases called one by one, a user
-collision operator on the plugin path, BGK::evaluate, the moment fields and a
-geometry edit between steps, and the TRT operator (fused and evaluate) --
-value for value."""
+collision operator on the plugin path, BGK::evaluate, the moment fields, a
+geometry edit and a population edit between steps, and the TRT, Regularized
+and KBC operators (fused and evaluate) -- value for value; the display layer's
+render and record of a State against the render restatement of the oracle's
+state (to one unit on a few pixels, as the render pass itself), and a
+recording stopped by a throwing frame function."""
 import os
 import subprocess
 
@@ -11,6 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as O
+from oracle import render_oracle as R
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "..", "chemsim_amd", "lib", "lbm_host_test")
@@ -71,3 +75,30 @@ def test_host_layer_matches_oracle(tmp_path):
     kop = O.params(W_, H_, kbc=0.05)
     np.testing.assert_array_equal(load("kbc_eval.f32"), O.kbc(kop, f0))
     np.testing.assert_array_equal(load("kbc.f32"), O.run(kop, f0, 20, flags))
+
+    # populations_mut() edits between two fused BGK steps reach the device
+    mid = O.run(op, f0, 5, flags).copy()
+    mid[2, 20, 30] = np.float32(mid[2, 20, 30] + np.float32(0.01))
+    mid[5, 21, 31] = 0.0
+    np.testing.assert_array_equal(load("edit.f32"), O.run(op, mid, 5, flags))
+    # display::render after the fused run; the host copy stays valid
+    np.testing.assert_array_equal(load("fused_after_render.f32"), ref)
+    img = np.fromfile(tmp_path / "render.u8", np.uint8).reshape(4, H_, W_, 3)
+    for mode in range(4):
+        close_image(img[mode], R.render(ref, mode, flags, True), mode)
+    # display::record: 3 frames of 4 steps, velocity, depth 2
+    rec = np.fromfile(tmp_path / "record.u8", np.uint8).reshape(3, H_, W_, 4)
+    f = f0
+    for k in range(3):
+        f = O.run(op, f, 4, flags)
+        close_image(rec[k, :, :, :3], R.render(f, R.VELOCITY, flags, True), k)
+        assert (rec[k, :, :, 3] == 255).all()
+    np.testing.assert_array_equal(load("record.f32"), f)
+    delivered, t = (tmp_path / "stop.txt").read_text().split()
+    assert int(delivered) == 2 and 2 <= float(t) <= 5, (delivered, t)
+
+
+def close_image(got, want, what):
+    d = np.abs(got.astype(int) - want.astype(int))
+    assert d.max() <= 1, (what, d.max())
+    assert (d > 0).sum() <= max(3, d.size // 1000), (what, (d > 0).sum())
