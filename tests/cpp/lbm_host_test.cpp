@@ -215,7 +215,7 @@ int main(int argc, char** argv)
             std::vector<uint8_t> frames;
             const int64_t got = display::record(s, 3, 4, display::DisplayMode::Velocity, true, 2,
                                                 [&](int64_t k, const uint8_t* rgba, int fw, int fh) {
-                                                    EXPECT(k == (int64_t)frames.size() / (4 * (int64_t)w * h));
+                                                    EXPECT(k == (int64_t)(frames.size() / (4 * w * h)));
                                                     EXPECT(fw == (int)w && fh == (int)h);
                                                     frames.insert(frames.end(), rgba, rgba + 4 * (size_t)fw * fh);
                                                 });
