@@ -55,6 +55,43 @@ def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr", trt=None, collision=Non
     return lat
 
 
+def warmstart_leg(ctx, count=16):
+    """Time to steady state (velocity residual 1e-6 over 100 steps) of the
+    first `count` held-out channels (512 x 128, Zou-He inlet/outlet, 1-4
+    obstacles, Re 20-100; the evaluation seed of the shipped network), from the
+    cold start and from the network's prediction with the Chapman-Enskog
+    non-equilibrium part.  Warm wall time includes the network's forward pass
+    and the feature computation on the host."""
+    from chemsim_amd import warmstart as WS
+    from chemsim_amd.cnn import Net
+    p, meta = WS.load_params(WS.WEIGHTS_PATH)
+    chans = WS.make_channels(meta.get("seed_test", 2), count)
+    out = {"channels": count, "weights": os.path.basename(WS.WEIGHTS_PATH), "tol": 1e-6}
+    with Net(ctx, 128, 512, base=meta["base"], batch=8) as net:
+        net.set_params(p)
+        cold_s = cold_t = warm_s = warm_t = 0.0
+        errs_u, errs_r = [], []
+        for ch in chans:
+            s, dt, _, ref = WS.time_to_steady(ctx, ch, WS.cold_state(ch))
+            cold_s += s
+            cold_t += dt
+            t0 = time.perf_counter()
+            pred = WS.predict(net, ch)
+            t_net = time.perf_counter() - t0
+            s, dt, _, _ = WS.time_to_steady(ctx, ch, pred, neq=True)
+            warm_s += s
+            warm_t += dt + t_net
+            e_r, e_u = WS.relative_error(ch, pred, ref)
+            errs_u.append(e_u)
+            errs_r.append(e_r)
+    out.update({"cold_steps_total": int(cold_s), "warm_steps_total": int(warm_s),
+                "cold_seconds": round(cold_t, 3), "warm_seconds_incl_network": round(warm_t, 3),
+                "step_ratio": round(cold_s / warm_s, 3), "time_ratio": round(cold_t / warm_t, 3),
+                "error_u_median": round(float(np.median(errs_u)), 4),
+                "error_rho_median": round(float(np.median(errs_r)), 4)})
+    return out
+
+
 def time_steps(lat, steps):
     lat.sync()
     t0 = time.perf_counter()
@@ -366,6 +403,8 @@ def main():
         from chemsim_amd.warmstart_run import time_step
         extras["cnn_train_step"] = time_step(ctx, iters=10)
         log(f"[bench] cnn: {extras['cnn_train_step']}")
+        extras["warmstart_16"] = warmstart_leg(ctx)
+        log(f"[bench] warm start: {extras['warmstart_16']}")
 
     cpu = None
     if not args.no_cpu:

@@ -26,6 +26,8 @@ from .lattice import Context, Lattice, Params
 from . import workloads as W
 
 RE_MID, RE_HALF = 60.0, 40.0
+# the shipped network weights (bf16) and their metadata
+WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_c32.npz")
 DIST_SCALE = 16.0         # signed distance feature = distance / DIST_SCALE, clipped to [-1, 1]
 
 
@@ -239,3 +241,21 @@ def load_params(path: str):
     p = (d["params_bf16"].astype(np.uint32) << 16).view(np.float32)
     meta = json.loads(bytes(d["meta"]).decode())
     return p, meta
+
+
+def predict(net, ch: W.Channel):
+    """The network's (rho, ux, uy) for one channel (features on the host,
+    forward pass on the GPU)."""
+    x = features(ch)
+    y = net.forward(x[None])
+    return decode_prediction(ch, y[0], x)
+
+
+def relative_error(ch: W.Channel, pred, ref):
+    """L2 errors of a prediction over fluid cells: (rho - 1) and |u|, relative."""
+    fluid = ch.flags() != W.WALL
+    r_p, ux_p, uy_p = (np.asarray(a, np.float64)[fluid] for a in pred)
+    r_t, ux_t, uy_t = (np.asarray(a, np.float64)[fluid] for a in ref)
+    e_rho = np.linalg.norm(r_p - r_t) / max(np.linalg.norm(r_t - 1.0), 1e-30)
+    e_u = np.sqrt(((ux_p - ux_t) ** 2 + (uy_p - uy_t) ** 2).sum() / (ux_t ** 2 + uy_t ** 2).sum())
+    return float(e_rho), float(e_u)

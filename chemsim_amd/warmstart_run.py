@@ -35,14 +35,7 @@ def log(msg: str) -> None:
     print(msg, flush=True)
 
 
-def relative_error(ch, pred, ref):
-    """L2 errors of the prediction over fluid cells: (rho - 1) and |u| relative."""
-    fluid = ch.flags() != 1
-    r_p, ux_p, uy_p = (np.asarray(a, np.float64)[fluid] for a in pred)
-    r_t, ux_t, uy_t = (np.asarray(a, np.float64)[fluid] for a in ref)
-    e_rho = np.linalg.norm(r_p - r_t) / max(np.linalg.norm(r_t - 1.0), 1e-30)
-    e_u = np.sqrt(((ux_p - ux_t) ** 2 + (uy_p - uy_t) ** 2).sum() / (ux_t ** 2 + uy_t ** 2).sum())
-    return float(e_rho), float(e_u)
+relative_error = WS.relative_error
 
 
 def predict(net: Net, chans):
