@@ -6,6 +6,8 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -613,6 +615,29 @@ void do_steps(cs_lattice* l, int n)
     CS_HIP(hipGetLastError());
 }
 
+// Staging area of cs_lattice_moments_f64_noalloc: the moment pass writes its
+// planes here instead of into a buffer allocated per call.  Each GPU the
+// library runs on holds its own copy of this array until the process ends.
+// In host code `scratch` names a host-side stand-in, never the device memory:
+// only the per-device address from hipGetSymbolAddress is passed to kernels
+// and copies.
+static __device__ __align__(256) double scratch[CS_NOALLOC_MAX_VALUES];
+constexpr int MAX_DEVICES = 64;
+static double* g_scratch_ptr[MAX_DEVICES];  // device address of `scratch`, per device
+static std::mutex g_scratch_mu;             // the calls that use it are serialised per process
+
+// Looks up this device's address of `scratch` once (the device is current).
+static void init_noalloc_scratch(int device)
+{
+    CS_REQUIRE(device >= 0 && device < MAX_DEVICES, CS_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    double* p = nullptr;
+    CS_HIP(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+    CS_REQUIRE(p != nullptr, CS_ERR_HIP, "no device address for the moment scratch");
+    g_scratch_ptr[device] = p;
+}
+
 // Runs k_moments and copies the requested planes (in mask bit order) to host.
 static void moments_to_host(cs_lattice* l, unsigned mask, double* host)
 {
@@ -710,10 +735,11 @@ extern "C" int cs_ctx_create(int device, cs_ctx** out)
         CS_HIP(hipGetDeviceCount(&count));
         CS_REQUIRE(device >= 0 && device < count, CS_ERR_ARG, "device index out of range");
         CS_HIP(hipSetDevice(device));
-        auto* c = new cs_ctx;
+        auto c = std::make_unique<cs_ctx>();
         c->device = device;
         CS_HIP(hipGetDeviceProperties(&c->prop, device));
-        *out = c;
+        init_noalloc_scratch(device);
+        *out = c.release();
     });
 }
 
@@ -1276,6 +1302,39 @@ extern "C" int cs_lattice_moments_f64(cs_lattice* l, double* rho, double* ux, do
         if (rho) std::memcpy(rho, tmp.data(), sizeof(double) * n);
         if (ux) std::memcpy(ux, tmp.data() + n, sizeof(double) * n);
         if (uy) std::memcpy(uy, tmp.data() + 2 * n, sizeof(double) * n);
+    });
+}
+
+extern "C" int cs_lattice_moments_f64_noalloc(cs_lattice* l, double* rho, double* ux, double* uy)
+{
+    return guarded([&] {
+        check_lat(l);
+        const int64_t n = (int64_t)l->w * l->h;
+        const unsigned mask = (rho ? M_RHO : 0u) | (ux ? M_UX : 0u) | (uy ? M_UY : 0u);
+        const int nplanes = __builtin_popcount(mask);
+        if (nplanes == 0) return;
+        CS_REQUIRE(n * nplanes <= (int64_t)CS_NOALLOC_MAX_VALUES, CS_ERR_ARG,
+                   "lattice too large for the no-allocation moment pass: planes x cells must be at most "
+                   "CS_NOALLOC_MAX_VALUES (3 x 2^20)");
+        const int dev = l->ctx->device;
+        CS_HIP(hipSetDevice(dev));
+        CS_REQUIRE(dev >= 0 && dev < MAX_DEVICES && g_scratch_ptr[dev] != nullptr, CS_ERR_STATE,
+                   "moment scratch not set up for this device");
+        double* const ptr = g_scratch_ptr[dev];
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        dispatch_storage(l->storage, [&]<class St>() {
+            hipLaunchKernelGGL((k_moments<St>), cell_grid(l), dim3(256), 0, l->stream,
+                               (const typename St::S*)l->buf[l->cur], l->w, l->h, l->pitch, l->plane, consts_of<St>(l),
+                               mask, ptr);
+        });
+        CS_HIP(hipGetLastError());
+        int slot = 0;
+        for (double* o : {rho, ux, uy})
+            if (o) {
+                CS_HIP(hipMemcpyAsync(o, ptr + (int64_t)(slot++) * n, sizeof(double) * (size_t)n,
+                                      hipMemcpyDeviceToHost, l->stream));
+            }
+        CS_HIP(hipStreamSynchronize(l->stream));
     });
 }
 

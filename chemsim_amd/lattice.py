@@ -273,6 +273,13 @@ class Lattice:
         call("cs_lattice_moments_f64", self.handle, *[_ptr(o) for o in outs])
         return tuple(outs)
 
+    def moments_f64_noalloc(self, planes=(True, True, True)):
+        """(rho, ux, uy) through the entry point that allocates nothing; a
+        False in `planes` passes a null pointer for that plane (None back)."""
+        outs = [np.empty((self.h, self.w), np.float64) if want else None for want in planes]
+        call("cs_lattice_moments_f64_noalloc", self.handle, *[None if o is None else _ptr(o) for o in outs])
+        return tuple(outs)
+
     def sums_f64(self) -> np.ndarray:
         out = (C.c_double * 3)()
         call("cs_lattice_sums_f64", self.handle, out)

@@ -238,6 +238,17 @@ int cs_bgk_evaluate(cs_ctx* ctx, int64_t n, const float* f, const float* feq, fl
 int cs_trt_new(float lambda, float ks_viscosity, float delta_x, float delta_t, float* tau_plus, float* tau_minus);
 /* Moments in the working precision, widened to f64 (any null output is skipped). */
 int cs_lattice_moments_f64(cs_lattice* lat, double* rho, double* ux, double* uy);
+/* The same moment pass with no device or host allocation: no counterpart in
+ * the reference; for callers that cannot allocate during the call (serving
+ * loops and harnesses that forbid hipMalloc).  The planes are staged in a
+ * static device array of CS_NOALLOC_MAX_VALUES doubles per GPU (24 MiB, held
+ * until the process ends) instead of a buffer allocated per call, so the
+ * requested planes x w x h must be at most CS_NOALLOC_MAX_VALUES (three planes
+ * up to 2^20 cells, one plane up to 3 x 2^20); larger lattices get CS_ERR_ARG.
+ * Synchronous; the calls of one process are serialised on the shared array
+ * (calls from several processes use separate copies). */
+enum { CS_NOALLOC_MAX_VALUES = 3 << 20 };
+int cs_lattice_moments_f64_noalloc(cs_lattice* lat, double* rho, double* ux, double* uy);
 
 /* ---- FP64 reductions (extension) -------------------------------------- */
 /* out = {sum rho, sum m_x, sum m_y} over non-solid cells, f64 accumulation. */

@@ -164,6 +164,10 @@ static void lattice_api(cs_ctx* ctx, int w, int h, int storage)
         std::vector<double> r(n), x(n), y(n);
         OK(cs_lattice_moments_f64(l, r.data(), x.data(), y.data()));
         OK(cs_lattice_moments_f64(l, nullptr, x.data(), nullptr));
+        OK(cs_lattice_moments_f64_noalloc(l, r.data(), x.data(), y.data()));
+        OK(cs_lattice_moments_f64_noalloc(l, nullptr, x.data(), nullptr));
+        OK(cs_lattice_moments_f64_noalloc(l, r.data(), nullptr, y.data()));
+        OK(cs_lattice_moments_f64_noalloc(l, nullptr, nullptr, nullptr));
         double sums[3];
         OK(cs_lattice_sums_f64(l, sums));
         double res = 0;

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <set>
 #include <vector>
@@ -40,6 +41,19 @@ StubStream g_null;                      // the null stream
 std::set<StubStream*> g_streams{&g_null};
 std::set<void*> g_device, g_host;       // live allocations
 bool g_capturing = false;
+// Registered __device__ variables (host shadow -> bytes) and their "device"
+// memory.  Constructed on first use: registration runs from the library's
+// static constructors, possibly before this file's globals exist.
+std::map<const void*, size_t>& var_size()
+{
+    static std::map<const void*, size_t> m;
+    return m;
+}
+std::map<const void*, void*>& var_mem()
+{
+    static std::map<const void*, void*> m;
+    return m;
+}
 
 struct CallConfig {
     dim3 grid, block;
@@ -311,5 +325,24 @@ void** __hipRegisterFatBinary(const void*)
 }
 void __hipRegisterFunction(void**, const void*, char*, const char*, unsigned int, void*, void*, void*, void*, int*) {}
 void __hipUnregisterFatBinary(void**) {}
+void __hipRegisterVar(void**, void* var, char*, char*, int, size_t size, int, int)
+{
+    var_size()[var] = size;
+}
+// A __device__ variable's address: exactly-sized zeroed heap memory per
+// variable, kept until the process ends (as a code object's globals are).
+hipError_t hipGetSymbolAddress(void** ptr, const void* symbol)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    auto it = var_size().find(symbol);
+    if (it == var_size().end()) return hipErrorInvalidSymbol;
+    void*& mem = var_mem()[symbol];
+    if (!mem) {
+        if (posix_memalign(&mem, 256, it->second ? it->second : 1) != 0) return hipErrorOutOfMemory;
+        std::memset(mem, 0, it->second);
+    }
+    *ptr = mem;
+    return hipSuccess;
+}
 
 }  // extern "C"
