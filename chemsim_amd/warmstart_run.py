@@ -73,6 +73,39 @@ def evaluate(ctx: Context, net: Net, chans, storage: str = "f32dev", tol: float 
     return rows
 
 
+def error_gain_curve(ctx: Context, net: Net, chans, alphas, storage: str = "f32dev", tol: float = 1e-6,
+                     max_steps: int = 400000):
+    """Steps to the residual tol from starts between the converged field and
+    the network's prediction: start = exact + alpha (prediction - exact), with
+    the Chapman-Enskog non-equilibrium part, for every alpha.  This keeps the
+    spatial structure of the network's own error and scales its size, so it
+    shows how the gain over the cold start grows as the prediction improves."""
+    preds, _ = predict(net, chans)
+    per = []
+    for i, (ch, pred) in enumerate(zip(chans, preds)):
+        s_cold, _, _, ref = WS.time_to_steady(ctx, ch, WS.cold_state(ch), tol, 100, max_steps, storage)
+        exact = [np.asarray(a, np.float64) for a in ref]
+        p = [np.asarray(a, np.float64) for a in pred]
+        row = {"re": ch.re, "cold": s_cold, "error_u": relative_error(ch, pred, ref)[1],
+               "error_rho": relative_error(ch, pred, ref)[0], "steps": {}}
+        for a in alphas:
+            start = tuple((e + a * (q - e)).astype(np.float32) for e, q in zip(exact, p))
+            s, _, _, _ = WS.time_to_steady(ctx, ch, start, tol, 100, max_steps, storage, neq=True)
+            row["steps"][str(a)] = s
+        per.append(row)
+        log(f"[curve {i + 1}/{len(chans)}] Re {ch.re:5.1f} cold {s_cold} " +
+            " ".join(f"a={a}:{row['steps'][str(a)]}" for a in alphas))
+    cold = sum(r["cold"] for r in per)
+    summary = {"cases": len(per), "cold_steps_total": cold, "error_u_median": float(np.median([r["error_u"] for r in per])),
+               "error_rho_median": float(np.median([r["error_rho"] for r in per])), "curve": []}
+    for a in alphas:
+        tot = sum(r["steps"][str(a)] for r in per)
+        summary["curve"].append({"alpha": a, "error_u_median": a * summary["error_u_median"],
+                                 "error_rho_median": a * summary["error_rho_median"],
+                                 "steps_total": tot, "step_ratio": cold / max(tot, 1)})
+    return {"summary": summary, "cases": per}
+
+
 def summarize(rows, max_steps: int):
     """Ratios cold / warm of total steps and total seconds over the cases
     where both runs reached the tolerance, plus medians of per-case ratios."""
@@ -138,7 +171,9 @@ def dataset(ctx: Context, chans, args):
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("cmd", choices=["all", "evaluate", "time"])
+    ap.add_argument("cmd", choices=["all", "evaluate", "time", "curve"])
+    ap.add_argument("--alphas", default="1,0.5,0.25,0.1,0.03,0.01,0",
+                    help="error scales of the curve command (start = exact + alpha (network - exact))")
     ap.add_argument("--iters", type=int, default=10, help="forward+backward passes to time (cmd time)")
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=200)
@@ -205,8 +240,17 @@ def main(argv=None) -> int:
                      time.perf_counter() - t1})
         WS.save_params(os.path.join(args.out, f"warmstart_c{args.base}.npz"), net.get_params(), meta)
     else:
-        params, meta = WS.load_params(args.weights)
+        params, meta = WS.load_params(args.weights or WS.WEIGHTS_PATH)
         net.set_params(params)
+    if args.cmd == "curve":
+        alphas = [float(a) for a in args.alphas.split(",")]
+        out = error_gain_curve(ctx, net, test, alphas, args.eval_storage, args.tol, args.max_steps)
+        with open(os.path.join(args.out, "warmstart_curve.json"), "w") as fh:
+            json.dump(out, fh, indent=1)
+        log(json.dumps(out["summary"]))
+        net.close()
+        ctx.close()
+        return 0
     rows = evaluate(ctx, net, test, args.eval_storage, args.tol, 100, args.max_steps, bound=not args.no_bound)
     summary = summarize(rows, args.max_steps)
     summary.update({"storage": args.eval_storage, "tol": args.tol, "seed_test": args.seed_test})
