@@ -17,6 +17,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "chemsim.h")
 CS_OK, CS_ERR_ARG, CS_ERR_HIP, CS_ERR_UNSUPPORTED, CS_ERR_STATE, CS_ERR_NOMEM, CS_ERR_CALLBACK = range(7)
 CS_RECORD_MAX_DEPTH = 16
 CS_STEP_VECTOR = -1
+CS_STEP_VECTOR_WIDE = -2
 CS_NOALLOC_MAX_VALUES = 3 << 20
 CS_F32, CS_F64, CS_BF16DEV, CS_F16DEV, CS_F32DEV = range(5)
 CS_EDGE_ZERO, CS_EDGE_PERIODIC = range(2)

@@ -421,16 +421,20 @@ static void launch_step_cpt(cs_lattice* l, const void* fin, void* fout)
 // Lattices with 4- or 8-byte storage, with or without geometry, take the
 // vector kernel (16 bytes per lane and plane, 4 rows per block) when a row
 // splits into vectors and the buffers allow 32-bit offsets.  16-bit storage
-// keeps the scalar kernel with 4 cells per lane, which measured faster (4096^2
-// FP16 with stochastic rounding: 114 us against 132-171 us for vector
-// variants with 4-, 8- and 16-byte lanes).
-template <class St, int EDGE, int MODE, bool SR, int COLL>
+// keeps the scalar kernel with 4 cells per lane, which measured faster: 4096^2
+// Taylor-Green with stochastic rounding, FP16 114.9 us against 128.4 us with
+// 8-byte and 134.6 us with 16-byte vector lanes (BF16 113.5 / 122.8 / 137.4),
+// with the row-wise neighbours exchanged by DPP lane shifts; the vector kernel
+// stays available for 16-bit storage through cs_set_cells_per_thread.
+template <class St, int EDGE, int MODE, bool SR, int COLL, int VB = 16>
 static bool launch_step_vec(cs_lattice* l, const void* fin, void* fout)
 {
-    if constexpr (sizeof(typename St::S) < 4) {
+    if constexpr (sizeof(typename St::S) < 4 && VB == 16 && MODE != MODE_FLUID) {
+        return false;  // node words of 8 cells do not fit one vector
+    } else if constexpr (SR && MODE != MODE_FLUID) {
         return false;
     } else {
-        constexpr int VB = 16, ROWS = 4;
+        constexpr int ROWS = 4;
         constexpr int V = VB / (int)sizeof(typename St::S);
         if (l->w % V != 0 || pop_elems(l) * (int64_t)sizeof(typename St::S) >= ((int64_t)1 << 32)) return false;
         StepArgs<typename St::T> a{};
@@ -479,9 +483,14 @@ template <class St, int EDGE, int MODE, bool SR, int COLL>
 static void launch_step_sr(cs_lattice* l, const void* fin, void* fout)
 {
     const int ov = g_cpt_override.load(std::memory_order_relaxed);
-    if ((ov == CS_STEP_VECTOR || (ov == 0 && vector_size_class(l))) &&
-        launch_step_vec<St, EDGE, MODE, SR, COLL>(l, fin, fout))
+    if constexpr (is_16bit<St>()) {
+        // 16-bit storage: the vector kernel only when asked for (8- or 16-byte lanes)
+        if (ov == CS_STEP_VECTOR && launch_step_vec<St, EDGE, MODE, SR, COLL, 8>(l, fin, fout)) return;
+        if (ov == CS_STEP_VECTOR_WIDE && launch_step_vec<St, EDGE, MODE, SR, COLL, 16>(l, fin, fout)) return;
+    } else if ((ov == CS_STEP_VECTOR || ov == CS_STEP_VECTOR_WIDE || (ov == 0 && vector_size_class(l))) &&
+               launch_step_vec<St, EDGE, MODE, SR, COLL>(l, fin, fout)) {
         return;
+    }
     int cpt = ov > 0 ? ov : 0;
     // 16-bit storage: 4 cells per lane (with stochastic rounding on the
     // Taylor-Green benchmark 143.6 against 140.4 GLUPS for one cell per lane at
@@ -1481,8 +1490,8 @@ extern "C" int cs_set_step_graphs(int enable)
 extern "C" int cs_set_cells_per_thread(int cpt)
 {
     return guarded([&] {
-        CS_REQUIRE(cpt == 0 || cpt == 1 || cpt == 2 || cpt == 4 || cpt == CS_STEP_VECTOR, CS_ERR_ARG,
-                   "cells per thread must be 0, 1, 2, 4 or CS_STEP_VECTOR");
+        CS_REQUIRE(cpt == 0 || cpt == 1 || cpt == 2 || cpt == 4 || cpt == CS_STEP_VECTOR || cpt == CS_STEP_VECTOR_WIDE,
+                   CS_ERR_ARG, "cells per thread must be 0, 1, 2, 4, CS_STEP_VECTOR or CS_STEP_VECTOR_WIDE");
         g_cpt_override.store(cpt, std::memory_order_relaxed);
     });
 }

@@ -610,18 +610,32 @@ union VecN {
     S e[VB / sizeof(S)];
 };
 
-template <class S>
-__device__ __forceinline__ S shfl_elem(S v, int src_lane)
+// Neighbour-lane exchange of a row vector's edge element by DPP wave shifts
+// (one VALU move; no LDS crossbar).  lane_prev: each lane receives lane - 1's
+// value, lane_next: lane + 1's; lane 0 / lane 63 receive 0 (the callers
+// replace those with a direct load).
+__device__ __forceinline__ uint32_t dpp_prev(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t dpp_next(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130 /* wave_shl:1 */, 0xF, 0xF, false);
+}
+template <class S, bool NEXT>
+__device__ __forceinline__ S lane_shift(S v)
 {
     if constexpr (sizeof(S) == 8) {
         const uint64_t u = __builtin_bit_cast(uint64_t, v);
-        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, src_lane, 64);
-        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), src_lane, 64);
+        const uint32_t lo = NEXT ? dpp_next((uint32_t)u) : dpp_prev((uint32_t)u);
+        const uint32_t hi = NEXT ? dpp_next((uint32_t)(u >> 32)) : dpp_prev((uint32_t)(u >> 32));
         return __builtin_bit_cast(S, ((uint64_t)hi << 32) | lo);
     } else if constexpr (sizeof(S) == 4) {
-        return __builtin_bit_cast(S, __shfl(__builtin_bit_cast(int, v), src_lane, 64));
+        const uint32_t u = __builtin_bit_cast(uint32_t, v);
+        return __builtin_bit_cast(S, NEXT ? dpp_next(u) : dpp_prev(u));
     } else {
-        return (S)__shfl((int)v, src_lane, 64);
+        const uint32_t u = (uint32_t)v;
+        return (S)(NEXT ? dpp_next(u) : dpp_prev(u));
     }
 }
 
@@ -701,7 +715,7 @@ __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T>
             for (int j = 0; j < V; ++j) B[j][i] = rvalid ? St::wid(v.e[j]) : outside;
         } else if (CY[i] == 1) {
             // source column c - 1: the previous cell, across the lane boundary for j = 0
-            S left = shfl_elem(v.e[V - 1], lane == 0 ? 0 : lane - 1);
+            S left = lane_shift<S, false>(v.e[V - 1]);
             if (lane == 0 && active) left = *at<uint32_t>(fin, rowbase + (uint32_t)cleft_w);
             const bool lvalid = rvalid && (lane != 0 || vleft);
             B[0][i] = lvalid ? St::wid(left) : outside;
@@ -709,7 +723,7 @@ __global__ __launch_bounds__(64 * ROWS) void k_step_vec(StepArgs<typename St::T>
             for (int j = 1; j < V; ++j) B[j][i] = rvalid ? St::wid(v.e[j - 1]) : outside;
         } else {
             // source column c + 1: the next cell, across the lane boundary for j = V - 1
-            S right = shfl_elem(v.e[0], lane == 63 ? 63 : lane + 1);
+            S right = lane_shift<S, true>(v.e[0]);
             if (last) right = *at<uint32_t>(fin, rowbase + (uint32_t)cright_w);
             const bool rtvalid = rvalid && (!last || vright);
 #pragma unroll

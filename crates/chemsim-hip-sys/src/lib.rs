@@ -136,6 +136,7 @@ pub struct cs_batch_tuning {
 }
 
 pub const CS_STEP_VECTOR: c_int = -1;
+pub const CS_STEP_VECTOR_WIDE: c_int = -2;
 
 extern "C" {
     // errors, context

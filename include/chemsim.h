@@ -353,13 +353,16 @@ int cs_gemm_bf16(cs_ctx* ctx, int M, int N, int K, const uint16_t* A, const uint
 int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy);
 /* Launch shape of the step kernel.  Both kernels give the same values: the
  * scalar kernel (one row per wave, 1, 2 or 4 cells per lane) and the vector
- * kernel (16 bytes per lane and plane, 4- and 8-byte storage, rows that split
- * into 16-byte vectors).  0 = the built-in choice by lattice size; 1, 2 or 4
- * = the scalar kernel with that many cells per lane; CS_STEP_VECTOR = the
- * vector kernel wherever the row splits, else the built-in choice.
+ * kernel (a lane owns consecutive cells of a row, one 8- or 16-byte access per
+ * plane).  0 = the built-in choice by lattice size; 1, 2 or 4 = the scalar
+ * kernel with that many cells per lane; CS_STEP_VECTOR = the vector kernel
+ * wherever the row splits into vectors (16-byte lanes for 4- and 8-byte
+ * storage, 8-byte lanes for 16-bit storage), else the built-in choice;
+ * CS_STEP_VECTOR_WIDE = the same with 16-byte lanes for 16-bit storage too
+ * (no geometry).
  * Process-wide; for tests and tuning.  No reference counterpart (ArrayFire
  * picks its own launch shapes). */
-enum { CS_STEP_VECTOR = -1 };
+enum { CS_STEP_VECTOR = -1, CS_STEP_VECTOR_WIDE = -2 };
 int cs_set_cells_per_thread(int cells_per_thread);
 /* Replay of captured HIP graphs of 64 fused steps for lattices of at most
  * 2^20 cells, where launch overhead dominates a step (1 = on, the default;

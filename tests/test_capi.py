@@ -44,6 +44,7 @@ def test_abi_version_and_error_path():
     assert capi.lib.cs_lattice_step(None, 1) == capi.CS_ERR_ARG
     assert "null" in capi.last_error()
     assert capi.lib.cs_set_cells_per_thread(3) == capi.CS_ERR_ARG
-    assert capi.lib.cs_set_cells_per_thread(-2) == capi.CS_ERR_ARG
+    assert capi.lib.cs_set_cells_per_thread(-3) == capi.CS_ERR_ARG
+    assert capi.lib.cs_set_cells_per_thread(capi.CS_STEP_VECTOR_WIDE) == capi.CS_OK
     assert capi.lib.cs_set_cells_per_thread(capi.CS_STEP_VECTOR) == capi.CS_OK
     assert capi.lib.cs_set_cells_per_thread(0) == capi.CS_OK

@@ -441,12 +441,12 @@ def test_init_nonequilibrium_matches_oracle(ctx, storage, edge):
 @pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
 @pytest.mark.parametrize("shape", [(16, 5), (264, 7), (520, 3), (1040, 9), (2048, 16)])
 def test_vector_fluid_step_matches_oracle(ctx, storage, edge, shape):
-    """The vector kernel (forced with CS_STEP_VECTOR) on lattices without
-    geometry whose rows split into 16-byte vectors, 4- and 8-byte storage
-    (lane shuffles for the row-wise streaming); partial waves, one- and
-    many-wave rows, row counts that do and do not fill a block: value for value
-    with the oracle, and equal to the scalar kernel.  16-bit storage runs the
-    scalar kernel either way."""
+    """The vector kernel (forced with CS_STEP_VECTOR, and CS_STEP_VECTOR_WIDE)
+    on lattices without geometry whose rows split into vectors (16-byte lanes
+    for 4- and 8-byte storage; 8- and 16-byte lanes for 16-bit storage; DPP
+    lane shifts for the row-wise streaming); partial waves, one- and many-wave
+    rows, row counts that do and do not fill a block: value for value with the
+    oracle, and equal to the scalar kernel."""
     w, h = shape
     kind = storage.split("_")[0]
     sr = storage.endswith("_sr")
@@ -472,7 +472,7 @@ def test_vector_fluid_step_matches_oracle(ctx, storage, edge, shape):
         f0 = narrow16(okind, O.equilibrium_dev(op, rho, ux, uy))
         ref = O.run_dev16(op, okind, f0, steps, None, rounding=1 if sr else 0, seed=99)
     outs = []
-    for cpt in (capi.CS_STEP_VECTOR, 1):
+    for cpt in (capi.CS_STEP_VECTOR, capi.CS_STEP_VECTOR_WIDE, 1):
         try:
             capi.call("cs_set_cells_per_thread", cpt)
             with Lattice(ctx, w, h, kind, gp) as lat:
@@ -481,8 +481,8 @@ def test_vector_fluid_step_matches_oracle(ctx, storage, edge, shape):
                 outs.append(lat.download_raw())
         finally:
             capi.call("cs_set_cells_per_thread", 0)
-    np.testing.assert_array_equal(outs[0], ref)
-    np.testing.assert_array_equal(outs[1], ref)
+    for o in outs:
+        np.testing.assert_array_equal(o, ref)
 
 
 
@@ -637,7 +637,7 @@ def test_degenerate_shapes_match_oracle(ctx, shape, edge, storage):
                                           err_msg=f"{shape} flags={with_flags}")
 
 
-@pytest.mark.parametrize("storage", ["f32", "f64", "f32dev"])
+@pytest.mark.parametrize("storage", ["f32", "f64", "f32dev", "f16", "bf16"])
 @pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
 @pytest.mark.parametrize("halfway", [False, True])
 @pytest.mark.parametrize("shape", [(4, 5), (8, 3), (260, 9), (72, 33)])
@@ -657,6 +657,10 @@ def test_vector_geometry_matches_oracle(ctx, storage, edge, halfway, shape):
     if storage == "f32dev":
         f0 = O.equilibrium_dev(op, rho, ux, uy)
         ref = O.run_devf32(op, f0, steps, flags)
+    elif storage in ("f16", "bf16"):
+        okind = O.BF16 if storage == "bf16" else O.F16
+        f0 = narrow16(okind, O.equilibrium_dev(op, rho, ux, uy))
+        ref = O.run_dev16(op, okind, f0, steps, flags)
     else:
         dt = STORE[storage]
         f0 = O.equilibrium(op, *(a.astype(dt) for a in (rho, ux, uy)))
