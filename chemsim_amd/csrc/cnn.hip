@@ -94,31 +94,9 @@ __global__ void k_pool_back(const uint16_t* __restrict__ dy, float* __restrict__
     dx[i] += 0.25f * bf16_f32(dy[((n * Ho + y / 2) * Wo + x / 2) * C + c]);
 }
 
-// Nearest 2x up-sampling of x (C1 channels) into channels [0, C1) of the
-// concatenation buffer y (Ct channels), and copy of the skip tensor s (C2
-// channels) into channels [C1, C1 + C2).
-__global__ void k_up_concat(const uint16_t* __restrict__ x, const uint16_t* __restrict__ s, uint16_t* __restrict__ y,
-                            int N, int H, int W, int C1, int C2)
-{
-    const int Ct = C1 + C2;
-    const int64_t n_out = (int64_t)N * H * W * Ct;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_out) return;
-    const int c = (int)(i % Ct);
-    const int64_t p = i / Ct;
-    if (c < C1) {
-        const int xx = (int)(p % W);
-        const int64_t t = p / W;
-        const int yy = (int)(t % H);
-        const int64_t n = t / H;
-        y[i] = x[((n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C1 + c];
-    } else {
-        y[i] = s[p * C2 + (c - C1)];
-    }
-}
-
-// Backward of k_up_concat: dx (f32, coarse) = sum of the 4 children of the
-// first C1 channels; ds (f32) += the last C2 channels.
+// Backward of a virtual concatenation (up-sampled coarse ++ skip) from the
+// gradient dy of the concatenation: dx (f32, coarse) = sum over the 4
+// children of the first C1 channels; ds (f32) += the last C2 channels.
 __global__ void k_up_concat_back(const uint16_t* __restrict__ dy, float* __restrict__ dx, float* __restrict__ ds,
                                  int N, int H, int W, int C1, int C2)
 {
@@ -205,26 +183,6 @@ __global__ void k_colsum(const float* __restrict__ part, float* __restrict__ out
         __syncthreads();
     }
     if (t == 0) out[c] = sh[0];
-}
-
-// dst[c][r] = src[r][c] for a bf16 [R][C] matrix, 64 x 64 tiles through LDS.
-__global__ void k_transpose16(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int64_t R, int C)
-{
-    __shared__ uint16_t tile[64][65];
-    const int64_t r0 = (int64_t)blockIdx.x * 64;
-    const int c0 = blockIdx.y * 64;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int i = ty; i < 64; i += 4) {
-        const int64_t r = r0 + i;
-        const int c = c0 + tx;
-        tile[i][tx] = (r < R && c < C) ? src[r * C + c] : (uint16_t)0;
-    }
-    __syncthreads();
-    for (int i = ty; i < 64; i += 4) {
-        const int c = c0 + i;
-        const int64_t r = r0 + tx;
-        if (c < C && r < R) dst[(int64_t)c * R + r] = tile[tx][i];
-    }
 }
 
 // Sum of split-K partials in a fixed order: a block takes 64 outputs, 16
@@ -326,9 +284,14 @@ struct Conv {
     int64_t nw() const { return (int64_t)cout * kpad; }
 };
 
-struct Act {  // NHWC bf16 activation and its shape
+// NHWC bf16 activation and its shape.  A virtual concatenation (lo not null)
+// is the nearest 2x up-sampling of lo (C1 channels at H/2 x W/2) followed by
+// the C - C1 channels of d; the convolutions read it in place.
+struct Act {
     uint16_t* d = nullptr;
     int N = 0, H = 0, W = 0, C = 0;
+    const uint16_t* lo = nullptr;
+    int C1 = 0;
     int64_t P() const { return (int64_t)N * H * W; }
     int64_t n() const { return P() * C; }
 };
@@ -346,8 +309,6 @@ struct cs_cnn {
     Act x_in, p1, p2, p3, cat3, cat2, cat1;
     float* head_out = nullptr;        // [P][8] f32
     // scratch
-    uint16_t* col = nullptr;
-    int64_t col_elems = 0;
     float* part = nullptr;
     int64_t part_elems = 0;
     float* dy32 = nullptr;            // two f32 gradient buffers of act_max elements each
@@ -355,7 +316,6 @@ struct cs_cnn {
     float* skip32[3] = {nullptr, nullptr, nullptr};  // accumulated gradients of e1, e2, e3 outputs
     uint16_t* dz = nullptr;           // bf16 gradients of a conv's output and input (largest activation each)
     uint16_t* dz2 = nullptr;
-    uint16_t* dzT = nullptr;          // its transpose [C][P] for the weight gradient
     float* loss_part = nullptr;
     float* targets = nullptr;
     float* weights = nullptr;
@@ -388,14 +348,16 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
     GemmArgs g{};
     CS_REQUIRE(epi != EPI_BF16_MASK || mask, CS_ERR_ARG, "masked epilogue without a mask");
     g.mask = mask;
-    if (lb == SHIFT) {
+    if (lb == GATHER_T) {
         CS_REQUIRE(la != GATHER && gather && gather->C % 8 == 0 && (int64_t)K == gather->P() && dil >= 1, CS_ERR_ARG,
-                   "bad shifted operand");
+                   "bad gathered operand of the weight gradient");
         g.gH = gather->H;
         g.gW = gather->W;
         g.gC = gather->C;
         g.gsign = 1;
         g.gdil = dil;
+        g.glo = gather->lo;
+        g.gC1 = gather->C1;
     }
     if (la == GATHER) {
         CS_REQUIRE(gather && gather->C % 8 == 0 && (int64_t)M == gather->P() && dil >= 1, CS_ERR_ARG,
@@ -405,6 +367,10 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
         g.gC = gather->C;
         g.gsign = sign;
         g.gdil = dil;
+        g.glo = gather->lo;
+        g.gC1 = gather->C1;
+        CS_REQUIRE(gather->lo == nullptr || (gather->C1 % 8 == 0 && gather->H % 2 == 0 && gather->W % 2 == 0),
+                   CS_ERR_ARG, "bad virtual concatenation");
     }
     g.A = A;
     g.B = B;
@@ -427,8 +393,8 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
 #define CS_GEMM(LA, LB, EPI) launch_gemm<LA, LB, EPI>(cfg, grid, net->stream, g)
     if (la == KMAJOR && lb == KMAJOR && epi == EPI_F32) {
         CS_GEMM(KMAJOR, KMAJOR, EPI_F32);
-    } else if (la == KMAJOR && lb == SHIFT && epi == EPI_F32) {
-        CS_GEMM(KMAJOR, SHIFT, EPI_F32);
+    } else if (la == PMAJOR_T && lb == GATHER_T && epi == EPI_F32) {
+        CS_GEMM(PMAJOR_T, GATHER_T, EPI_F32);
     } else if (la == GATHER && lb == KMAJOR) {
         if (epi == EPI_F32) CS_GEMM(GATHER, KMAJOR, EPI_F32);
         else if (epi == EPI_BF16) CS_GEMM(GATHER, KMAJOR, EPI_BF16);
@@ -441,21 +407,55 @@ void gemm(cs_cnn* net, int la, int lb, int epi, const uint16_t* A, int lda, cons
     CS_HIP(hipGetLastError());
 }
 
-// The three column-shifted transposed copies of x that the weight gradient's
-// SHIFT operand reads: col[s][c][p] (3 x C x P).
-void shift3_t(cs_cnn* net, const Act& x, int dil)
+// The halo kernel takes 3x3 convolutions with dilation 1 on images whose
+// width is a multiple of 64 and height a multiple of 4.
+bool halo_ok(const Act& x, int dil) { return dil == 1 && x.W % 64 == 0 && x.H % 4 == 0 && x.C % 8 == 0; }
+
+void conv_halo(cs_cnn* net, int epi, const Act& x, const uint16_t* w, int ldw, const float* bias, void* out, int ldc,
+               int N, int sign, const uint16_t* mask, int kflops)
 {
-    const int64_t P = x.P();
-    CS_REQUIRE(3 * P * x.C <= net->col_elems && P % 8 == 0 && x.C % 8 == 0 && dil >= 1 && dil <= 8, CS_ERR_STATE,
-               "shift scratch too small or unsupported shape");
-    hipLaunchKernelGGL(k_shift3_t, dim3((unsigned)((P + 63) / 64), (unsigned)((x.C + 63) / 64)), dim3(256), 0,
-                       net->stream, x.d, net->col, P, x.W, x.C, dil);
+    HaloArgs a{};
+    a.in = x.d;
+    a.glo = x.lo;
+    a.C1 = x.C1;
+    a.w = w;
+    a.ldw = ldw;
+    a.bias = bias;
+    a.out = out;
+    a.ldc = ldc;
+    a.mask = mask;
+    a.N = N;
+    a.C = x.C;
+    a.H = x.H;
+    a.W = x.W;
+    a.sign = sign;
+    CS_REQUIRE(epi != EPI_BF16_MASK || mask, CS_ERR_ARG, "masked epilogue without a mask");
+    const int bn = N <= 16 ? 16 : (N <= 32 ? 32 : 64);
+    const dim3 grid((unsigned)((int64_t)x.N * (x.H / 4) * (x.W / 64)), (unsigned)((N + bn - 1) / bn));
+    net->gemm_flops += 2.0 * (double)x.P() * N * kflops;
+    auto go = [&]<int EPI>() {
+        if (bn == 16) hipLaunchKernelGGL((k_conv_halo<EPI, 16>), grid, dim3(256), 0, net->stream, a);
+        else if (bn == 32) hipLaunchKernelGGL((k_conv_halo<EPI, 32>), grid, dim3(256), 0, net->stream, a);
+        else hipLaunchKernelGGL((k_conv_halo<EPI, 64>), grid, dim3(256), 0, net->stream, a);
+    };
+    switch (epi) {
+    case EPI_F32: go.template operator()<EPI_F32>(); break;
+    case EPI_BF16: go.template operator()<EPI_BF16>(); break;
+    case EPI_BF16_RELU: go.template operator()<EPI_BF16_RELU>(); break;
+    default: go.template operator()<EPI_BF16_MASK>(); break;
+    }
     CS_HIP(hipGetLastError());
 }
 
 void conv_forward(cs_cnn* net, int li, const Act& x, const Act& y, float* y32 = nullptr)
 {
     Conv& c = net->convs[(size_t)li];
+    if (halo_ok(x, c.dil)) {
+        if (y32) conv_halo(net, EPI_F32, x, c.wb, c.kpad, c.b, y32, 8, c.cout, 1, nullptr, c.kpad);
+        else conv_halo(net, c.relu ? EPI_BF16_RELU : EPI_BF16, x, c.wb, c.kpad, c.b, y.d, c.cout, c.cout, 1, nullptr,
+                       c.kpad);
+        return;
+    }
     if (y32)
         gemm(net, GATHER, KMAJOR, EPI_F32, x.d, 0, c.wb, c.kpad, y32, 8, c.b, (int)x.P(), c.cout, c.kpad, 1, &x, 1, c.dil);
     else
@@ -490,21 +490,17 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
                            c.cout);
         CS_HIP(hipGetLastError());
     }
-    // weight gradient: dW[co][k] = sum_p dzT[co][p] colT[k][p], split over p;
-    // dz transposed so that p (the reduction) is contiguous, colT read from
-    // the three column-shifted transposed copies of x
-    hipLaunchKernelGGL(k_transpose16, dim3((unsigned)((P + 63) / 64), (unsigned)((c.cout + 63) / 64)), dim3(256), 0,
-                       net->stream, dz, net->dzT, P, c.cout);
-    CS_HIP(hipGetLastError());
-    shift3_t(net, x, c.dil);
+    // weight gradient: dW[co][k] = sum_p dz[p][co] x[p + tap offset][ci],
+    // k = tap*cin + ci, split over p; both operands are read from their NHWC
+    // tensors and transposed into LDS by the GEMM's loaders
     // enough K ranges for ~1024 workgroups, each at least 1024 pixels long
     int bm, bn;
     tile_dims(pick_tile(c.cout, c.kpad), bm, bn);
     const int64_t tiles = (int64_t)((c.cout + bm - 1) / bm) * ((c.kpad + bn - 1) / bn);
     int splits = (int)std::max<int64_t>(1, std::min<int64_t>({256, P / 1024, (1024 + tiles - 1) / tiles}));
     while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
-    gemm(net, KMAJOR, SHIFT, EPI_F32, net->dzT, (int)P, net->col, (int)P, net->part, c.kpad, nullptr, c.cout, c.kpad,
-         (int)P, splits, &x, 1, c.dil);
+    gemm(net, PMAJOR_T, GATHER_T, EPI_F32, dz, c.cout, x.d, 0, net->part, c.kpad, nullptr, c.cout, c.kpad, (int)P,
+         splits, &x, 1, c.dil);
     hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream, net->part,
                        c.gw, c.nw(), splits);
     CS_HIP(hipGetLastError());
@@ -513,6 +509,11 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
     // bf16 (masked by x's ReLU when mask_in) -- the conversion the next layer's
     // backward would make from f32, done in the epilogue
     Act dzact{dz, x.N, x.H, x.W, c.cout};
+    if (halo_ok(dzact, c.dil)) {
+        conv_halo(net, mask_in ? EPI_BF16_MASK : EPI_BF16, dzact, c.wd, c.kpad_d, nullptr, dz_out, c.cin, c.cin, -1,
+                  mask_in ? x.d : nullptr, c.kpad_d);
+        return;
+    }
     gemm(net, GATHER, KMAJOR, mask_in ? EPI_BF16_MASK : EPI_BF16, dz, 0, c.wd, c.kpad_d, dz_out, c.cin, nullptr,
          (int)P, c.cin, c.kpad_d, 1, &dzact, -1, c.dil, mask_in ? x.d : nullptr);
 }
@@ -549,16 +550,10 @@ void forward(cs_cnn* net, int n)
     conv_forward(net, 6, ci[6], co[6]);
     conv_forward(net, 7, ci[7], co[7]);
     // decoder
-    hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(net->cat3.n())), dim3(256), 0, s, co[7].d, co[5].d, net->cat3.d, n,
-                       H / 4, W / 4, co[7].C, co[5].C);
     conv_forward(net, 8, ci[8], co[8]);
     conv_forward(net, 9, ci[9], co[9]);
-    hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(net->cat2.n())), dim3(256), 0, s, co[9].d, co[3].d, net->cat2.d, n,
-                       H / 2, W / 2, co[9].C, co[3].C);
     conv_forward(net, 10, ci[10], co[10]);
     conv_forward(net, 11, ci[11], co[11]);
-    hipLaunchKernelGGL(k_up_concat, dim3(blocks_for(net->cat1.n())), dim3(256), 0, s, co[11].d, co[1].d, net->cat1.d,
-                       n, H, W, co[11].C, co[1].C);
     conv_forward(net, 12, ci[12], co[12]);
     conv_forward(net, 13, ci[13], co[13]);
     conv_forward(net, 14, ci[14], co[14], net->head_out);
@@ -754,24 +749,28 @@ extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int b
             net->p1 = act(H / 2, W / 2, c);
             net->p2 = act(H / 4, W / 4, 2 * c);
             net->p3 = act(H / 8, W / 8, 4 * c);
-            net->cat3 = act(H / 4, W / 4, 12 * c);
-            net->cat2 = act(H / 2, W / 2, 6 * c);
-            net->cat1 = act(H, W, 3 * c);
+            // decoder inputs: up-sampled coarse output ++ encoder skip, read in place
+            auto cat = [&](const Act& coarse, const Act& skip) {
+                Act a{skip.d, n, skip.H, skip.W, coarse.C + skip.C};
+                a.lo = coarse.d;
+                a.C1 = coarse.C;
+                return a;
+            };
+            net->cat3 = cat(net->conv_out[7], net->conv_out[5]);
+            net->cat2 = cat(net->conv_out[9], net->conv_out[3]);
+            net->cat1 = cat(net->conv_out[11], net->conv_out[1]);
             const Act ins[15] = {net->x_in,        net->conv_out[0], net->p1,          net->conv_out[2], net->p2,
                                  net->conv_out[4], net->p3,          net->conv_out[6], net->cat3,        net->conv_out[8],
                                  net->cat2,        net->conv_out[10], net->cat1,       net->conv_out[12], net->conv_out[13]};
             for (int i = 0; i < 15; ++i) net->conv_in.push_back(ins[i]);
             net->head_out = dalloc<float>(net, (int64_t)n * H * W * 8);
-            // scratch: the weight gradient's three shifted copies of the largest input (3 x cin x pixels)
-            int64_t col = 0, part = 0;
+            // scratch: split-K partials of the weight gradient and the bias partials
+            int64_t part = 0;
             for (int i = 0; i < 15; ++i) {
                 const int64_t P = net->conv_in[(size_t)i].P();
-                col = std::max(col, 3 * P * net->convs[(size_t)i].cin);
                 part = std::max(part, (int64_t)64 * net->convs[(size_t)i].nw());
                 part = std::max(part, P * net->convs[(size_t)i].cin);
             }
-            net->col = dalloc<uint16_t>(net, col);
-            net->col_elems = col;
             net->part = dalloc<float>(net, part);
             net->part_elems = part;
             const int64_t amax = (int64_t)n * H * W * 96;  // >= every activation (3c <= 96 for c <= 32)
@@ -780,7 +779,6 @@ extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int b
             net->dy32 = dalloc<float>(net, 2 * amax2);
             net->dz = dalloc<uint16_t>(net, amax2);
             net->dz2 = dalloc<uint16_t>(net, amax2);
-            net->dzT = dalloc<uint16_t>(net, amax2);
             net->skip32[0] = dalloc<float>(net, net->conv_out[1].n());
             net->skip32[1] = dalloc<float>(net, net->conv_out[3].n());
             net->skip32[2] = dalloc<float>(net, net->conv_out[5].n());

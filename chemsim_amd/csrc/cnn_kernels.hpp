@@ -6,8 +6,8 @@
 // forward pass, the data gradient and the weight gradient:
 //   forward  out[p][co] = sum_k col(in)[p][k] * W[co][k]    gather inside the GEMM
 //   dgrad    din[p][ci] = sum_k colT(dout)[p][k] * Wd[ci][k] gather (flipped taps) inside the GEMM
-//   wgrad    dW[co][k]  = sum_p doutT[co][p] * colT[k][p]   doutT transposed in memory, colT read
-//                                                          from three column-shifted copies, split over p
+//   wgrad    dW[co][k]  = sum_p dout[p][co] * col(in)[p][k]  both read pixel-major and transposed
+//                                                          into LDS by the loaders, split over p
 // GEMM tiles (BM x BN x 32, chosen by shape) are double-buffered in LDS;
 // 4 waves of 64 lanes each own a sub-tile of v_mfma_f32_16x16x32_bf16
 // fragments with FP32 accumulation.
@@ -37,11 +37,14 @@ __device__ __forceinline__ float bf16_f32(uint16_t b) { return __uint_as_float((
 // GATHER: the A operand of a 3x3 convolution (taps `dil` cells apart) read
 // straight from the NHWC input (implicit im2col): row r = pixel,
 // k = tap*C + c, zero outside the image and for k >= 9C.
-// SHIFT: the B operand of the weight gradient, colT[k][p] with k = tap*C + c,
-// read from three column-shifted transposed copies of the input,
-// X3[s][c][p] = in[p + (s - 1) dil][c] (zero across the row ends), at the
-// row offset dy*W of the tap; zero outside the image and for k >= 9C.
-enum Layout : int { KMAJOR = 0, GATHER = 2, SHIFT = 3 };
+// PMAJOR_T: the A operand of the weight gradient, rows m = channels of an
+// NHWC tensor [P][lda] (the output gradient dz), K = pixels: element (m, p)
+// at A[p*lda + m].  GATHER_T: its B operand, rows n = tap*C + c of the input
+// x read at the tap's neighbour of pixel p (zero outside the image and for
+// n >= 9C), K = pixels.  Both are read as 16-byte vectors of 8 channels of
+// one pixel and written transposed into LDS, so neither needs a transposed
+// or shifted copy in memory.
+enum Layout : int { KMAJOR = 0, GATHER = 2, PMAJOR_T = 4, GATHER_T = 5 };
 
 enum Epi : int {
     EPI_F32 = 0,        // C (+ bias) stored as f32
@@ -61,11 +64,30 @@ struct GemmArgs {
     int64_t split_stride;  // elements between the partial outputs of consecutive z
     int gH, gW, gC, gsign;  // GATHER: input height, width, channels (multiple of 8), tap sign (+1 / -1)
     int gdil;               // GATHER: tap spacing (dilation)
+    // GATHER / GATHER_T of a virtual concatenation (glo not null): channels
+    // [0, gC1) are the nearest 2x up-sampling of glo ([gH/2][gW/2][gC1] per
+    // image), channels [gC1, gC) the operand tensor itself, which then holds
+    // gC - gC1 channels per pixel (the skip tensor).  No concatenated copy exists.
+    const uint16_t* glo;
+    int gC1;
     const uint16_t* mask;   // EPI_BF16_MASK: [M][ldc] bf16 ReLU output whose zeros mask C
 };
 
 constexpr int BK = 32;
 constexpr int LDS_PITCH = BK + 8;  // bf16 elements per LDS row (padding against bank conflicts)
+
+// Channels c..c+7 of pixel q = (img, yy, xx) of the gathered tensor `in`,
+// through the virtual concatenation when g.glo is set.
+__device__ __forceinline__ u32x4 gathered8(const uint16_t* __restrict__ in, const GemmArgs& g, int64_t q, int img,
+                                           int yy, int xx, int c)
+{
+    if (g.glo == nullptr) return *(const u32x4*)(in + q * g.gC + c);
+    if (c < g.gC1) {
+        const int64_t qc = ((int64_t)img * (g.gH >> 1) + (yy >> 1)) * (g.gW >> 1) + (xx >> 1);
+        return *(const u32x4*)(g.glo + qc * g.gC1 + c);
+    }
+    return *(const u32x4*)(in + q * (g.gC - g.gC1) + (c - g.gC1));
+}
 
 // One BK-wide slice of an R-row operand tile: global -> registers (NV 16-byte
 // vectors per thread, zero outside `rows`), registers -> LDS as [row][k].
@@ -109,8 +131,8 @@ struct GatherLoader {
     static constexpr int VECS = 4 * R;
     static constexpr int NV = (VECS + 255) / 256;
     u32x4 v[NV];
-    int y[NV], x[NV];
-    int64_t base[NV];  // element offset of channel 0 of the slot's pixel
+    int y[NV], x[NV], img[NV];
+    int64_t base[NV];  // the slot's pixel
 
     __device__ __forceinline__ void init(const GemmArgs& g, int r0)
     {
@@ -121,12 +143,14 @@ struct GatherLoader {
             const int p = r0 + (e >> 2);
             y[i] = -(1 << 28);  // marks an unused slot: every tap falls outside
             x[i] = 0;
+            img[i] = 0;
             base[i] = 0;
             if ((VECS % 256 == 0 || e < VECS) && p < g.M) {
                 x[i] = p % g.gW;
                 const int ny = p / g.gW;
                 y[i] = ny % g.gH;
-                base[i] = (int64_t)p * g.gC;
+                img[i] = ny / g.gH;
+                base[i] = p;
             }
         }
     }
@@ -142,8 +166,12 @@ struct GatherLoader {
                 const int tap = k / g.gC, c = k - tap * g.gC;
                 const int dy = g.gsign * g.gdil * (tap / 3 - 1), dx = g.gsign * g.gdil * (tap % 3 - 1);
                 const int yy = y[i] + dy, xx = x[i] + dx;
-                if (yy >= 0 && yy < g.gH && xx >= 0 && xx < g.gW)
-                    v[i] = *(const u32x4*)(in + base[i] + (int64_t)(dy * g.gW + dx) * g.gC + c);
+                if (yy >= 0 && yy < g.gH && xx >= 0 && xx < g.gW) {
+                    if (g.glo == nullptr)
+                        v[i] = *(const u32x4*)(in + base[i] * g.gC + (int64_t)(dy * g.gW + dx) * g.gC + c);
+                    else
+                        v[i] = gathered8(in, g, base[i] + (int64_t)(dy * g.gW + dx), img[i], yy, xx, c);
+                }
             }
         }
     }
@@ -159,44 +187,14 @@ struct GatherLoader {
     }
 };
 
-// The SHIFT operand (rows n of colT, K = pixels): a vector of 8 pixels lies in
-// one image row when W is a multiple of 8 (one validity test, one 16-byte
-// load); otherwise the pixels are tested and loaded one by one.
+// A BK-pixel slice of a pixel-major operand: vector e holds channels
+// 8*(e / BK) .. +7 of pixel k0 + e % BK; stored transposed, lds[channel][pixel].
 template <int R>
-struct ShiftLoader {
-    static constexpr int VECS = 4 * R;
+struct PixelVecs {
+    static_assert(R % 8 == 0, "rows in groups of 8 channels");
+    static constexpr int VECS = BK * R / 8;
     static constexpr int NV = (VECS + 255) / 256;
     u32x4 v[NV];
-
-    __device__ __forceinline__ void load(const uint16_t* __restrict__ x3, const GemmArgs& g, int n0, int k0, int kend)
-    {
-        const int t = threadIdx.x;
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int e = t + 256 * i;
-            v[i] = u32x4{0u, 0u, 0u, 0u};
-            if (VECS % 256 != 0 && e >= VECS) continue;
-            const int n = n0 + (e >> 2), p0 = k0 + 8 * (e & 3);
-            if (n >= g.N || n >= 9 * g.gC || p0 >= kend) continue;
-            const int tap = n / g.gC, c = n - tap * g.gC;
-            const int dy = g.gdil * (tap / 3 - 1);
-            const uint16_t* row = x3 + ((int64_t)(tap % 3) * g.gC + c) * g.K;
-            if (g.gW % 8 == 0) {
-                const int yy = (p0 / g.gW) % g.gH + dy;
-                if (yy >= 0 && yy < g.gH) v[i] = *(const u32x4*)(row + p0 + dy * g.gW);
-            } else {
-                uint32_t w[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int p = p0 + j;
-                    const int yy = (p / g.gW) % g.gH + dy;
-                    const uint32_t b = (yy >= 0 && yy < g.gH) ? (uint32_t)row[p + dy * g.gW] : 0u;
-                    w[j >> 1] |= b << (16 * (j & 1));
-                }
-                v[i] = u32x4{w[0], w[1], w[2], w[3]};
-            }
-        }
-    }
     __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const
     {
         const int t = threadIdx.x;
@@ -204,7 +202,58 @@ struct ShiftLoader {
         for (int i = 0; i < NV; ++i) {
             const int e = t + 256 * i;
             if (VECS % 256 != 0 && e >= VECS) continue;
-            *(u32x4*)(&lds[e >> 2][8 * (e & 3)]) = v[i];
+            const int j = e % BK, r = 8 * (e / BK);
+            const uint32_t w[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                lds[r + 2 * q][j] = (uint16_t)w[q];
+                lds[r + 2 * q + 1][j] = (uint16_t)(w[q] >> 16);
+            }
+        }
+    }
+};
+
+// PMAJOR_T: rows m0.. of A = channels of dz [P][lda] (lda a multiple of 8).
+template <int R>
+struct PixelMajorLoader : PixelVecs<R> {
+    using PixelVecs<R>::v;
+    static constexpr int VECS = PixelVecs<R>::VECS, NV = PixelVecs<R>::NV;
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ src, int ld, int rows, int r0, int k0, int kend)
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            v[i] = u32x4{0u, 0u, 0u, 0u};
+            if (VECS % 256 != 0 && e >= VECS) continue;
+            const int p = k0 + e % BK, c = r0 + 8 * (e / BK);
+            if (p < kend && c < rows) v[i] = *(const u32x4*)(src + (int64_t)p * ld + c);
+        }
+    }
+};
+
+// GATHER_T: rows n0.. of B = (tap, channel) pairs of the NHWC input x, read
+// at the tap's neighbour (dilation gdil) of each pixel of the slice.
+template <int R>
+struct GatherTLoader : PixelVecs<R> {
+    using PixelVecs<R>::v;
+    static constexpr int VECS = PixelVecs<R>::VECS, NV = PixelVecs<R>::NV;
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ x, const GemmArgs& g, int n0, int k0, int kend)
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int e = t + 256 * i;
+            v[i] = u32x4{0u, 0u, 0u, 0u};
+            if (VECS % 256 != 0 && e >= VECS) continue;
+            const int p = k0 + e % BK, n = n0 + 8 * (e / BK);
+            if (p >= kend || n >= g.N || n >= 9 * g.gC) continue;
+            const int tap = n / g.gC, c = n - tap * g.gC;  // gC % 8 == 0: the 8 rows share the tap
+            const int dy = g.gdil * (tap / 3 - 1), dx = g.gdil * (tap % 3 - 1);
+            const int py = p / g.gW;
+            const int xx = p - py * g.gW + dx, yy = py % g.gH + dy;
+            if (yy >= 0 && yy < g.gH && xx >= 0 && xx < g.gW)
+                v[i] = gathered8(x, g, (int64_t)p + dy * g.gW + dx, py / g.gH, yy, xx, c);
         }
     }
 };
@@ -216,8 +265,8 @@ struct LoaderB {
     __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const { l.store(lds); }
 };
 template <int R>
-struct LoaderB<SHIFT, R> {
-    ShiftLoader<R> l;
+struct LoaderB<GATHER_T, R> {
+    GatherTLoader<R> l;
     __device__ __forceinline__ void load(const GemmArgs& g, int n0, int k0, int kend) { l.load(g.B, g, n0, k0, kend); }
     __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const { l.store(lds); }
 };
@@ -225,6 +274,13 @@ struct LoaderB<SHIFT, R> {
 template <int L, int R>
 struct LoaderA {
     TileLoader<L, R> l;
+    __device__ __forceinline__ void init(const GemmArgs&, int) {}
+    __device__ __forceinline__ void load(const GemmArgs& g, int r0, int k0, int kend) { l.load(g.A, g.lda, g.M, r0, k0, kend); }
+    __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const { l.store(lds); }
+};
+template <int R>
+struct LoaderA<PMAJOR_T, R> {
+    PixelMajorLoader<R> l;
     __device__ __forceinline__ void init(const GemmArgs&, int) {}
     __device__ __forceinline__ void load(const GemmArgs& g, int r0, int k0, int kend) { l.load(g.A, g.lda, g.M, r0, k0, kend); }
     __device__ __forceinline__ void store(uint16_t (*lds)[LDS_PITCH]) const { l.store(lds); }
@@ -356,44 +412,123 @@ inline void launch_gemm(int cfg, dim3 grid, hipStream_t s, const GemmArgs& g)
     }
 }
 
-// Column-shifted transposed copies of an NHWC input for the SHIFT operand:
-// x3[s][c][p] = in[p + (s - 1) dil][c] when the shifted pixel lies in p's
-// image row, else 0 (s = 0, 1, 2; dil <= 8).  A block loads 64 + 16 pixels x
-// 64 channels into LDS with 16-byte loads and writes the three shifted
-// 64-pixel rows of each channel with 16-byte stores.  P and C are multiples
-// of 8.
-__global__ __launch_bounds__(256) void k_shift3_t(const uint16_t* __restrict__ in, uint16_t* __restrict__ x3,
-                                                  int64_t P, int W, int C, int dil)
+// Direct 3x3 convolution (dilation 1) on MFMA with the input tile and its
+// one-pixel halo staged in LDS once per chunk of 32 input channels, so the
+// nine taps read LDS only (the GATHER GEMM fetches every tap from memory).
+// Forward: out[p][n] = sum over taps and channels of in[p + tap][c] W[n][tap*C + c]
+// (+ bias, epilogue); with sign -1 (flipped taps) and the transposed weights
+// it is the data gradient.  A block is 4 waves over a tile of 4 image rows x
+// 64 columns of one image (W % 64 == 0, H % 4 == 0); wave w computes row w
+// (64 pixels) x BN output channels n0 = blockIdx.y * BN.  The input may be a
+// virtual concatenation (in.glo set, as in GemmArgs).
+struct HaloArgs {
+    const uint16_t* in;     // gathered input, NHWC with C channels (or the skip part)
+    const uint16_t* glo;    // virtual concatenation: up-sampled part (C1 channels at H/2 x W/2), or null
+    int C1;
+    const uint16_t* w;      // weights [N][ldw], k = tap*C + c
+    int ldw;
+    const float* bias;      // [N] or null
+    void* out;              // [P][ldc]
+    int ldc;
+    const uint16_t* mask;   // EPI_BF16_MASK: [P][ldc] ReLU output whose zeros mask the result
+    int N, C, H, W, sign;
+};
+
+constexpr int HALO_CP = 40;                 // bf16 per halo pixel: 32 channels + pad (conflict-free 16-byte reads)
+constexpr int HALO_PIX = 6 * 66;            // (4 + 2) rows x (64 + 2) columns
+constexpr int HALO_WP = 9 * 32 + 8;         // bf16 per weight row: 9 taps x 32 channels + pad
+
+template <int EPI, int BN>
+__global__ __launch_bounds__(256) void k_conv_halo(HaloArgs a)
 {
-    __shared__ uint16_t tile[80][65];  // odd pitch: the 8 pixel groups of a channel hit 8 banks
-    const int64_t p0 = (int64_t)blockIdx.x * 64;
-    const int c0 = blockIdx.y * 64;
-    const int t = threadIdx.x;
-    for (int e = t; e < 80 * 8; e += 256) {
-        const int j = e >> 3, cv = e & 7;  // pixel p0 - 8 + j, channels c0 + 8cv .. +7
-        const int64_t q = p0 - 8 + j;
-        const int c = c0 + 8 * cv;
-        u32x4 v = {0u, 0u, 0u, 0u};
-        if (q >= 0 && q < P && c < C) v = *(const u32x4*)(in + q * C + c);
-        const uint16_t* h = (const uint16_t*)&v;
+    constexpr int TN = BN / 16;
+    __shared__ __align__(16) uint16_t halo[HALO_PIX][HALO_CP];
+    __shared__ __align__(16) uint16_t wts[BN][HALO_WP];
+    const int tiles_x = a.W / 64, tiles_y = a.H / 4;
+    int t = blockIdx.x;
+    const int tx0 = (t % tiles_x) * 64;
+    t /= tiles_x;
+    const int ty0 = (t % tiles_y) * 4;
+    const int img = t / tiles_y;
+    const int n0 = blockIdx.y * BN;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    const int C2 = a.C - a.C1;
+    f32x4 acc[4][TN];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) tile[j][8 * cv + i] = h[i];
-    }
-    __syncthreads();
-    for (int e = t; e < 3 * 64 * 8; e += 256) {
-        const int s = e / 512, rem = e - s * 512;
-        const int c = rem >> 3, pv = rem & 7;  // channel c0 + c, pixels p0 + 8pv .. +7
-        const int64_t p = p0 + 8 * pv;
-        if (c0 + c >= C || p >= P) continue;
-        const int dx = (s - 1) * dil;
-        uint32_t w[4] = {0u, 0u, 0u, 0u};
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int xx = (int)((p + i) % W) + dx;
-            const uint32_t b = (xx >= 0 && xx < W) ? (uint32_t)tile[8 + 8 * pv + i + dx][c] : 0u;
-            w[i >> 1] |= b << (16 * (i & 1));
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int c0 = 0; c0 < a.C; c0 += 32) {
+        // halo: pixel (ty0 - 1 + hy, tx0 - 1 + hx), channels c0 .. c0 + 31
+        for (int e = threadIdx.x; e < HALO_PIX * 4; e += 256) {
+            const int hp = e >> 2, v = e & 3;
+            const int hy = hp / 66, hx = hp - hy * 66;
+            const int yy = ty0 - 1 + hy, xx = tx0 - 1 + hx, c = c0 + 8 * v;
+            u32x4 val = {0u, 0u, 0u, 0u};
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && c < a.C) {
+                const int64_t q = ((int64_t)img * a.H + yy) * a.W + xx;
+                if (a.glo == nullptr) {
+                    val = *(const u32x4*)(a.in + q * a.C + c);
+                } else if (c < a.C1) {
+                    const int64_t qc = ((int64_t)img * (a.H / 2) + yy / 2) * (a.W / 2) + xx / 2;
+                    val = *(const u32x4*)(a.glo + qc * a.C1 + c);
+                } else {
+                    val = *(const u32x4*)(a.in + q * C2 + (c - a.C1));
+                }
+            }
+            *(u32x4*)(&halo[hp][8 * v]) = val;
         }
-        *(u32x4*)(x3 + ((int64_t)s * C + c0 + c) * P + p) = u32x4{w[0], w[1], w[2], w[3]};
+        // weights of the chunk: wts[n][tap*32 + k] = W[n0 + n][tap*C + c0 + k]
+        for (int e = threadIdx.x; e < BN * 36; e += 256) {
+            const int n = e / 36, r = e - n * 36;
+            const int tap = r >> 2, v = r & 3;
+            const int c = c0 + 8 * v;
+            u32x4 val = {0u, 0u, 0u, 0u};
+            if (n0 + n < a.N && c < a.C) val = *(const u32x4*)(a.w + (int64_t)(n0 + n) * a.ldw + tap * a.C + c);
+            *(u32x4*)(&wts[n][32 * tap + 8 * v]) = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = a.sign * (tap / 3 - 1), dx = a.sign * (tap % 3 - 1);
+            bf16x8 af[4], bf[TN];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                af[i] = *(const bf16x8*)(&halo[(wave + 1 + dy) * 66 + 16 * i + fr + 1 + dx][fk]);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bf[j] = *(const bf16x8*)(&wts[16 * j + fr][32 * tap + fk]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // D layout: lane holds rows 4*(lane/16) + 0..3 (pixels) of column lane % 16 (channel)
+    const int64_t prow = ((int64_t)img * a.H + ty0 + wave) * a.W + tx0;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int col = n0 + 16 * j + fr;
+        if (col >= a.N) continue;
+        const float bias = a.bias ? a.bias[col] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t p = prow + 16 * i + 4 * (lane >> 4) + q;
+                float v = acc[i][j][q] + bias;
+                if (EPI == EPI_F32) {
+                    ((float*)a.out)[p * a.ldc + col] = v;
+                } else {
+                    if (EPI == EPI_BF16_RELU) v = v > 0.f ? v : 0.f;
+                    if (EPI == EPI_BF16_MASK && !(bf16_f32(a.mask[p * a.ldc + col]) > 0.f)) v = 0.f;
+                    ((uint16_t*)a.out)[p * a.ldc + col] = bf16_rne(v);
+                }
+            }
     }
 }
 
