@@ -491,19 +491,38 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
         CS_HIP(hipGetLastError());
     }
     // weight gradient: dW[co][k] = sum_p dz[p][co] x[p + tap offset][ci],
-    // k = tap*cin + ci, split over p; both operands are read from their NHWC
-    // tensors and transposed into LDS by the GEMM's loaders
-    // enough K ranges for ~1024 workgroups, each at least 1024 pixels long
-    int bm, bn;
-    tile_dims(pick_tile(c.cout, c.kpad), bm, bn);
-    const int64_t tiles = (int64_t)((c.cout + bm - 1) / bm) * ((c.kpad + bn - 1) / bn);
-    int splits = (int)std::max<int64_t>(1, std::min<int64_t>({256, P / 1024, (1024 + tiles - 1) / tiles}));
-    while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
-    gemm(net, PMAJOR_T, GATHER_T, EPI_F32, dz, c.cout, x.d, 0, net->part, c.kpad, nullptr, c.cout, c.kpad, (int)P,
-         splits, &x, 1, c.dil);
-    hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream, net->part,
-                       c.gw, c.nw(), splits);
-    CS_HIP(hipGetLastError());
+    // k = tap*cin + ci, split over p.  Dilation 1: the LDS-tile kernel;
+    // otherwise the GEMM, whose loaders read both operands from their NHWC
+    // tensors and transpose them into LDS.
+    if (halo_ok(x, c.dil) && c.cout % 8 == 0) {
+        const int bm = c.cout <= 16 ? 16 : (c.cout <= 32 ? 32 : 64);
+        const int cob = (c.cout + bm - 1) / bm, cib = (x.C + WG_CI - 1) / WG_CI;
+        const int tiles = x.N * (x.H / 4) * (x.W / 64);
+        int runs = std::max(1, std::min(tiles, 1024 / (cob * cib)));
+        while (runs > 1 && (int64_t)runs * c.cout * c.kpad > net->part_elems) --runs;
+        WgradArgs wa{dz, x.d, x.lo, x.C1, net->part, c.kpad, c.cout, x.C, x.H, x.W, tiles};
+        const dim3 grid((unsigned)cob, (unsigned)cib, (unsigned)runs);
+        if (bm == 16) hipLaunchKernelGGL((k_wgrad_halo<16>), grid, dim3(256), 0, net->stream, wa);
+        else if (bm == 32) hipLaunchKernelGGL((k_wgrad_halo<32>), grid, dim3(256), 0, net->stream, wa);
+        else hipLaunchKernelGGL((k_wgrad_halo<64>), grid, dim3(256), 0, net->stream, wa);
+        CS_HIP(hipGetLastError());
+        net->gemm_flops += 2.0 * c.cout * c.kpad * (double)P;
+        hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream,
+                           net->part, c.gw, c.nw(), runs);
+        CS_HIP(hipGetLastError());
+    } else {
+        // enough K ranges for ~1024 workgroups, each at least 1024 pixels long
+        int bm, bn;
+        tile_dims(pick_tile(c.cout, c.kpad), bm, bn);
+        const int64_t tiles = (int64_t)((c.cout + bm - 1) / bm) * ((c.kpad + bn - 1) / bn);
+        int splits = (int)std::max<int64_t>(1, std::min<int64_t>({256, P / 1024, (1024 + tiles - 1) / tiles}));
+        while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
+        gemm(net, PMAJOR_T, GATHER_T, EPI_F32, dz, c.cout, x.d, 0, net->part, c.kpad, nullptr, c.cout, c.kpad, (int)P,
+             splits, &x, 1, c.dil);
+        hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream,
+                           net->part, c.gw, c.nw(), splits);
+        CS_HIP(hipGetLastError());
+    }
     if (!dz_out) return;
     // data gradient: dx[p][ci] = sum_k colT(dz)[p][k] wd[ci][k], stored in
     // bf16 (masked by x's ReLU when mask_in) -- the conversion the next layer's

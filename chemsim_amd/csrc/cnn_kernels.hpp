@@ -532,5 +532,137 @@ __global__ __launch_bounds__(256) void k_conv_halo(HaloArgs a)
     }
 }
 
+// Weight gradient of a 3x3 convolution with dilation 1 from LDS tiles:
+// dW[co][tap*C + ci] = sum over pixels p of dz[p][co] x[p + tap][ci].  A
+// block owns BM output channels (blockIdx.x), 16 input channels (blockIdx.y)
+// and a run of 4 x 64 pixel tiles (blockIdx.z of gridDim.z runs); per tile it
+// stages dz transposed (dzT[co][pixel]) and the input tile with its halo as
+// three copies shifted by one column (xs[dx + 1][ci][row][col]), so every
+// MFMA operand is an aligned 16-byte LDS read and all nine taps reuse the
+// staged tile.  The sums of a run go to part[blockIdx.z][co][k] (padding
+// columns k >= 9C written as 0); a fixed-order reduction over the runs follows.
+struct WgradArgs {
+    const uint16_t* dz;    // [P][cout]
+    const uint16_t* in;    // input, NHWC with C channels (or the skip part of a virtual concatenation)
+    const uint16_t* glo;   // virtual concatenation: up-sampled part (C1 channels at H/2 x W/2), or null
+    int C1;
+    float* part;           // [gridDim.z][cout][ldc]
+    int ldc;
+    int cout, C, H, W, tiles;
+};
+
+constexpr int WG_CI = 16;          // input channels per block
+constexpr int WG_DZP = 256 + 8;    // bf16 per dzT row: 256 pixels + pad (conflict-free A reads)
+constexpr int WG_XR = 68;          // bf16 per xs row: 64 columns + pad (conflict-free B reads)
+
+template <int BM>
+__global__ __launch_bounds__(256) void k_wgrad_halo(WgradArgs a)
+{
+    constexpr int NF = (BM / 16) * 9;       // 16x16 output fragments: (output-channel block, tap)
+    constexpr int FPW = (NF + 3) / 4;       // per wave
+    __shared__ __align__(16) uint16_t dzT[BM][WG_DZP];
+    __shared__ __align__(16) uint16_t xs[3][WG_CI][6][WG_XR];
+    const int co0 = blockIdx.x * BM, ci0 = blockIdx.y * WG_CI;
+    const int z = blockIdx.z, runs = gridDim.z;
+    const int t0 = (int)((int64_t)a.tiles * z / runs), t1 = (int)((int64_t)a.tiles * (z + 1) / runs);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    const int tiles_x = a.W / 64, tiles_y = a.H / 4;
+    const int C2 = a.C - a.C1;
+    f32x4 acc[FPW];
+#pragma unroll
+    for (int i = 0; i < FPW; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int t = t0; t < t1; ++t) {
+        const int tx0 = (t % tiles_x) * 64;
+        const int ty0 = ((t / tiles_x) % tiles_y) * 4;
+        const int img = t / (tiles_x * tiles_y);
+        // dz of the tile's 256 pixels, transposed: dzT[co][64 r + c]
+        for (int e = threadIdx.x; e < 256 * (BM / 8); e += 256) {
+            const int px = e & 255, v = e >> 8;
+            const int co = co0 + 8 * v;
+            u32x4 val = {0u, 0u, 0u, 0u};
+            if (co < a.cout) {
+                const int64_t q = ((int64_t)img * a.H + ty0 + (px >> 6)) * a.W + tx0 + (px & 63);
+                val = *(const u32x4*)(a.dz + q * a.cout + co);
+            }
+            const uint32_t w[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                dzT[8 * v + 2 * j][px] = (uint16_t)w[j];
+                dzT[8 * v + 2 * j + 1][px] = (uint16_t)(w[j] >> 16);
+            }
+        }
+        // input halo (6 rows x 66 columns), channels ci0 .. ci0 + 15, three shifted copies
+        for (int e = threadIdx.x; e < 6 * 66 * 2; e += 256) {
+            const int hp = e >> 1, v = e & 1;
+            const int hy = hp / 66, hx = hp - hy * 66;
+            const int yy = ty0 - 1 + hy, xx = tx0 - 1 + hx, c = ci0 + 8 * v;
+            u32x4 val = {0u, 0u, 0u, 0u};
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && c < a.C) {
+                const int64_t q = ((int64_t)img * a.H + yy) * a.W + xx;
+                if (a.glo == nullptr) {
+                    val = *(const u32x4*)(a.in + q * a.C + c);
+                } else if (c < a.C1) {
+                    const int64_t qc = ((int64_t)img * (a.H / 2) + yy / 2) * (a.W / 2) + xx / 2;
+                    val = *(const u32x4*)(a.glo + qc * a.C1 + c);
+                } else {
+                    val = *(const u32x4*)(a.in + q * C2 + (c - a.C1));
+                }
+            }
+            const uint32_t w[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+            for (int sft = 0; sft < 3; ++sft) {
+                const int col = hx - sft;
+                if (col < 0 || col >= 64) continue;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    xs[sft][8 * v + 2 * j][hy][col] = (uint16_t)w[j];
+                    xs[sft][8 * v + 2 * j + 1][hy][col] = (uint16_t)(w[j] >> 16);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {  // 32-pixel slices: tile row ks / 2, columns 32 (ks % 2) ..
+            const int r = ks >> 1, cx = 32 * (ks & 1);
+#pragma unroll
+            for (int i = 0; i < FPW; ++i) {
+                const int f = wave + 4 * i;
+                if (f >= NF) break;
+                const int cb = f / 9, tap = f - 9 * cb;
+                const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+                const bf16x8 av = *(const bf16x8*)(&dzT[16 * cb + fr][64 * r + cx + fk]);
+                const bf16x8 bv = *(const bf16x8*)(&xs[dx + 1][fr][r + 1 + dy][cx + fk]);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // D layout: lane holds rows 4*(lane/16) + 0..3 (output channels) of column lane % 16 (input channel)
+    float* out = a.part + (int64_t)z * a.cout * a.ldc;
+    const int ci = ci0 + fr;
+#pragma unroll
+    for (int i = 0; i < FPW; ++i) {
+        const int f = wave + 4 * i;
+        if (f >= NF) break;
+        const int cb = f / 9, tap = f - 9 * cb;
+        if (ci >= a.C) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int co = co0 + 16 * cb + 4 * (lane >> 4) + q;
+            if (co < a.cout) out[(int64_t)co * a.ldc + tap * a.C + ci] = acc[i][q];
+        }
+    }
+    if (blockIdx.y == 0) {
+        const int pad = a.ldc - 9 * a.C;
+        for (int e = threadIdx.x; e < BM * pad; e += 256) {
+            const int co = co0 + e / pad;
+            if (co < a.cout) out[(int64_t)co * a.ldc + 9 * a.C + e % pad] = 0.f;
+        }
+    }
+}
+
 }  // namespace cnn
 }  // namespace cs

@@ -52,9 +52,14 @@ def test_forward_matches_oracle(ctx, h, w):
     assert err < 1e-2, err
 
 
-@pytest.mark.parametrize("h,w", [(16, 32), (64, 128)])
-def test_gradients_match_oracle(ctx, h, w):
-    c, (p, x, t, wt) = 8, small(h=h, w=w, seed=12)
+@pytest.mark.parametrize("h,w,c", [(16, 32, 8), (64, 128, 8), (64, 128, 32)])
+def test_gradients_match_oracle(ctx, h, w, c):
+    """(16, 32): every convolution on the GEMM path; (64, 128): the LDS-tile
+    kernels at the levels whose width is a multiple of 64 (forward, data and
+    weight gradient), the GEMM below; base 32 adds output-channel blocks of
+    32 and 64, several 32-channel chunks per tap and the virtual concatenations
+    of 96 and 192 channels on the tile kernels."""
+    _, (p, x, t, wt) = c, small(c=c, h=h, w=w, seed=12)
     with Net(ctx, h, w, base=c, batch=2) as net:
         net.set_params(p)
         loss = net.loss_grad(x, t, wt)
