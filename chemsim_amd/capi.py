@@ -18,6 +18,7 @@ CS_OK, CS_ERR_ARG, CS_ERR_HIP, CS_ERR_UNSUPPORTED, CS_ERR_STATE, CS_ERR_NOMEM, C
 CS_RECORD_MAX_DEPTH = 16
 CS_STEP_VECTOR = -1
 CS_STEP_VECTOR_WIDE = -2
+CS_TEMPORAL_AUTO = -1
 CS_NOALLOC_MAX_VALUES = 3 << 20
 CS_F32, CS_F64, CS_BF16DEV, CS_F16DEV, CS_F32DEV = range(5)
 CS_EDGE_ZERO, CS_EDGE_PERIODIC = range(2)
@@ -182,6 +183,7 @@ SIGNATURES = {
     "cs_bench_copy": [_P, _I64, _I, _FP],
     "cs_set_cells_per_thread": [_I],
     "cs_set_step_graphs": [_I],
+    "cs_set_temporal_steps": [_I],
 }
 
 OPTIONAL = set()

@@ -166,4 +166,5 @@ struct cs_lattice {
     // shape they were captured with (cleared when any of them changes).
     hipGraphExec_t step_graph[2] = {nullptr, nullptr};
     int graph_cpt = INT_MIN;    // cells-per-thread override at capture (INT_MIN: none captured)
+    int graph_tb = 0;           // steps per temporally blocked launch at capture
 };

@@ -1,0 +1,142 @@
+// Temporally blocked fused step: K steps of the D2Q9 update per launch.
+//
+// The one-step kernels read and write every population once per step, so
+// they are bound by HBM at 72 bytes per cell update (FP32).  Here a block
+// loads a 64 x 64 region of the lattice (its (64-2K) x (64-2K) interior and a
+// halo of K cells on every side) into LDS once, runs K steps on it there and
+// stores the interior: step s updates the cells at least s cells inside the
+// region, which pull only from cells that step s-1 updated, so the interior
+// after K steps is exact.  HBM moves (64^2 + (64-2K)^2) * 36 bytes per K
+// steps of (64-2K)^2 cells -- 12.5 bytes per cell update at K = 8 -- and the
+// update runs at LDS and VALU speed.
+//
+// Cells outside a zero-edged lattice hold the outside value (f = 0, or
+// g = -w_i rho0 for deviation storage) and are never updated, which is the
+// one-step kernels' rule for a population pulled from outside the edge; a
+// periodic lattice is unrolled into the region by wrapping the loads (any
+// number of times, so lattices smaller than a region work too).  The
+// per-cell arithmetic is the one-step kernels' (same pull order, same
+// collision functions), so K steps here equal K launches of k_step bit for
+// bit.  No geometry (every cell fluid), 32-bit storage.
+#pragma once
+
+#include "lbm_device.hpp"
+
+namespace cs {
+
+constexpr int TB_EDGE = 64;        // region side (cells); lane l owns region column l
+constexpr int TB_THREADS = 1024;   // 16 waves; wave w owns region rows w, w + 16, w + 32, w + 48
+constexpr int TB_SLOTS = TB_EDGE * TB_EDGE / TB_THREADS;
+
+// Collision of one fluid cell: the operator finish() applies.
+template <class St, int COLL>
+__device__ __forceinline__ void collide_cell(const Consts<typename St::T>& k, typename St::T (&B)[9])
+{
+    if constexpr (COLL != CS_COLL_BGK) static_assert(!St::DEV, "TRT, Regularized and KBC run in the reference arithmetic");
+    if constexpr (COLL == CS_COLL_TRT) trt_collide(k, B);
+    else if constexpr (COLL == CS_COLL_REGULARIZED) reg_collide(k, B);
+    else if constexpr (COLL == CS_COLL_KBC) kbc_collide(k, B);
+    else if constexpr (St::DEV) bgk_collide_dev(k, B);
+    else bgk_collide(k, B);
+}
+
+// Region of tile t: rows ty*(64-2K) - K .., columns tx*(64-2K) - K ..
+// (a.tiles_x = ceil(w / (64-2K))); blocks of TB_THREADS threads, one block
+// per CU (the region takes 144 KiB of LDS).  A wave's LDS accesses are 64
+// consecutive words of one region row (a pull from column -1 or 64 wraps
+// within the row and belongs to a lane that discards its result), so they
+// are free of bank conflicts, and each lane's nine pull addresses per row
+// slot stay the same from step to step.
+template <class St, int EDGE, int K, int COLL>
+__global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T> a)
+{
+    using S = typename St::S;
+    using T = typename St::T;
+    static_assert(sizeof(S) == 4 && sizeof(T) == 4, "32-bit storage and arithmetic");
+    static_assert(K >= 1 && 2 * K < TB_EDGE, "halo");
+    constexpr int IN = TB_EDGE - 2 * K;  // interior side
+    __shared__ T L[9][TB_EDGE][TB_EDGE];
+
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = tile / a.tiles_x;
+    const int tx = tile - ty * a.tiles_x;
+    const int r0 = ty * IN - K, c0 = tx * IN - K;
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const S* __restrict__ fin = (const S*)a.fin;
+    S* __restrict__ fout = (S*)a.fout;
+    const uint32_t P = (uint32_t)a.plane;
+
+    // this lane's lattice column, and whether it lies inside the lattice
+    int gc = c0 + lane;
+    bool cin = true;
+    if (EDGE == CS_EDGE_PERIODIC) {
+        gc %= a.w;
+        gc += gc < 0 ? a.w : 0;
+    } else {
+        cin = gc >= 0 && gc < a.w;
+        gc = cin ? gc : 0;
+    }
+    bool inside[TB_SLOTS];
+#pragma unroll
+    for (int j = 0; j < TB_SLOTS; ++j) {
+        const int rr = wave + 16 * j;
+        int gr = r0 + rr;
+        bool rin = true;
+        if (EDGE == CS_EDGE_PERIODIC) {
+            gr %= a.h;
+            gr += gr < 0 ? a.h : 0;
+        } else {
+            rin = gr >= 0 && gr < a.h;
+            gr = rin ? gr : 0;
+        }
+        inside[j] = cin && rin;
+        const uint32_t o = (uint32_t)gr * (uint32_t)a.pitch + (uint32_t)gc;
+        T v[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) v[i] = St::wid(*at<uint32_t>(fin, (uint32_t)i * P + o));
+#pragma unroll
+        for (int i = 0; i < 9; ++i) L[i][rr][lane] = inside[j] ? v[i] : (St::DEV ? -a.k.base(i) : T(0));
+    }
+
+#pragma unroll 1
+    for (int s = 1; s <= K; ++s) {
+        __syncthreads();  // step s - 1 fully in LDS
+        T B[TB_SLOTS][9];
+#pragma unroll
+        for (int j = 0; j < TB_SLOTS; ++j) {
+            const int rr = wave + 16 * j;
+            if (rr >= s && rr < TB_EDGE - s) {  // uniform over the wave
+#pragma unroll
+                for (int i = 0; i < 9; ++i) B[j][i] = L[i][rr + CX[i]][(lane - CY[i]) & (TB_EDGE - 1)];
+                collide_cell<St, COLL>(a.k, B[j]);
+            }
+        }
+        __syncthreads();  // every read of step s done before any write
+        const bool cols = lane >= s && lane < TB_EDGE - s;
+#pragma unroll
+        for (int j = 0; j < TB_SLOTS; ++j) {
+            const int rr = wave + 16 * j;
+            if (rr >= s && rr < TB_EDGE - s && cols && inside[j]) {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) L[i][rr][lane] = B[j][i];
+            }
+        }
+    }
+    __syncthreads();
+
+    // interior: rows ty*IN .. and columns tx*IN .. of the lattice (no wrap)
+    const int oc = c0 + lane;
+    const bool ocol = lane >= K && lane < TB_EDGE - K && oc < a.w;
+#pragma unroll
+    for (int j = 0; j < TB_SLOTS; ++j) {
+        const int rr = wave + 16 * j;
+        const int orow = r0 + rr;
+        if (ocol && rr >= K && rr < TB_EDGE - K && orow < a.h) {
+            const uint32_t o = (uint32_t)orow * (uint32_t)a.pitch + (uint32_t)oc;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) St::st(at<uint32_t>(fout, (uint32_t)i * P + o), 0, L[i][rr][lane]);
+        }
+    }
+}
+
+}  // namespace cs

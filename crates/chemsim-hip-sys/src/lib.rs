@@ -137,6 +137,7 @@ pub struct cs_batch_tuning {
 
 pub const CS_STEP_VECTOR: c_int = -1;
 pub const CS_STEP_VECTOR_WIDE: c_int = -2;
+pub const CS_TEMPORAL_AUTO: c_int = -1;
 
 extern "C" {
     // errors, context
@@ -257,4 +258,5 @@ extern "C" {
     pub fn cs_bench_copy(ctx: *mut cs_ctx, bytes: i64, iters: c_int, ms_per_copy: *mut f32) -> c_int;
     pub fn cs_set_cells_per_thread(cells_per_thread: c_int) -> c_int;
     pub fn cs_set_step_graphs(enable: c_int) -> c_int;
+    pub fn cs_set_temporal_steps(k: c_int) -> c_int;
 }

@@ -370,6 +370,14 @@ int cs_set_cells_per_thread(int cells_per_thread);
  * with stochastic rounding always launches step by step.  Process-wide; for
  * tests and tuning.  No reference counterpart. */
 int cs_set_step_graphs(int enable);
+/* Steps per launch of the temporally blocked step kernel, which lattices
+ * without geometry in FP32 or FP32-deviation storage take: CS_TEMPORAL_AUTO
+ * (the default, 8 steps), 4 or 8 steps, or 0 for one step per launch.  Results
+ * are identical either way.  A launch-shape override of
+ * cs_set_cells_per_thread selects the one-step kernels instead.
+ * Process-wide; for tests and tuning.  No reference counterpart. */
+enum { CS_TEMPORAL_AUTO = -1 };
+int cs_set_temporal_steps(int k);
 
 #ifdef __cplusplus
 }

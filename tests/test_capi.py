@@ -48,3 +48,7 @@ def test_abi_version_and_error_path():
     assert capi.lib.cs_set_cells_per_thread(capi.CS_STEP_VECTOR_WIDE) == capi.CS_OK
     assert capi.lib.cs_set_cells_per_thread(capi.CS_STEP_VECTOR) == capi.CS_OK
     assert capi.lib.cs_set_cells_per_thread(0) == capi.CS_OK
+    for k in (2, 3, 6, -2):
+        assert capi.lib.cs_set_temporal_steps(k) == capi.CS_ERR_ARG
+    for k in (0, 4, 8, capi.CS_TEMPORAL_AUTO):
+        assert capi.lib.cs_set_temporal_steps(k) == capi.CS_OK

@@ -40,17 +40,21 @@ def gpu_params(op: O.Params) -> Params:
                   ks_viscosity=op.ks_viscosity)
 
 
-@pytest.fixture(params=["auto", "vector"])
+@pytest.fixture(params=["auto", "one_step", "vector"])
 def kernel(request):
-    """The step kernel the lattice takes: the built-in choice by size (small
-    test lattices: the scalar kernel) or the vector kernel, forced wherever
-    rows split into 16-byte vectors."""
+    """The step kernel the lattice takes: the built-in choice (FP32 and FP32
+    deviation lattices without geometry: the temporally blocked kernel), the
+    one-step kernels' choice by size (small test lattices: the scalar kernel),
+    or the vector kernel, forced wherever rows split into 16-byte vectors."""
     if request.param == "vector":
         capi.call("cs_set_cells_per_thread", capi.CS_STEP_VECTOR)
+    elif request.param == "one_step":
+        capi.call("cs_set_temporal_steps", 0)
     try:
         yield request.param
     finally:
         capi.call("cs_set_cells_per_thread", 0)
+        capi.call("cs_set_temporal_steps", capi.CS_TEMPORAL_AUTO)
 
 
 @pytest.mark.parametrize("storage", ["f32", "f64"])

@@ -1,0 +1,144 @@
+"""The temporally blocked step (k_step_tb: K steps per launch out of LDS)
+against the CPU oracle, value for value, for lattices without geometry in
+FP32 and FP32 deviation storage: zero and periodic edges, every collision
+operator, lattices smaller than one 64 x 64 region and ragged ones, step
+counts that leave a remainder for the one-step kernels, graph replay, and a
+full-size lattice against the one-step kernels."""
+import numpy as np
+import pytest
+
+from chemsim_amd import Lattice, Params, capi
+from chemsim_amd import workloads as W
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_state(rng, h, w):
+    rho = 1 + 0.02 * rng.standard_normal((h, w))
+    ux = 0.03 * rng.standard_normal((h, w))
+    uy = 0.03 * rng.standard_normal((h, w))
+    p = O.params(w, h)
+    f = O.equilibrium(p, rho, ux, uy) + 0.002 * rng.standard_normal((9, h, w))
+    return f.astype(np.float32)
+
+
+def gpu_params(op: O.Params) -> Params:
+    return Params(tau=op.tau, delta_x=op.delta_x, delta_t=op.delta_t, edge=op.edge, boundary=op.boundary,
+                  lid_ux=op.lid_ux, lid_uy=op.lid_uy, inlet_ux=op.inlet_ux, inlet_uy=op.inlet_uy,
+                  outlet_rho=op.outlet_rho, rho0=op.rho0, collision=op.collision, tau_minus=op.tau_minus,
+                  ks_viscosity=op.ks_viscosity)
+
+
+@pytest.fixture(params=[capi.CS_TEMPORAL_AUTO, 4, 8])
+def steps_per_launch(request):
+    capi.call("cs_set_temporal_steps", request.param)
+    try:
+        yield request.param
+    finally:
+        capi.call("cs_set_temporal_steps", capi.CS_TEMPORAL_AUTO)
+
+
+SHAPES = [(1, 1), (37, 29), (200, 130), (96, 48), (13, 300), (300, 7)]
+
+
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{w}x{h}" for w, h in SHAPES])
+def test_bgk_f32_matches_oracle(ctx, edge, shape, steps_per_launch):
+    w, h = shape
+    rng = np.random.default_rng(41)
+    f = rand_state(rng, h, w)
+    op = O.params(w, h, tau=0.62, edge=edge)
+    with Lattice(ctx, w, h, "f32", gpu_params(op)) as lat:
+        lat.upload(f)
+        ref = f
+        for n in (19, 8):  # K-step launches, then the remainder one step at a time
+            lat.step(n)
+            ref = O.run(op, ref, n)
+            np.testing.assert_array_equal(lat.download(), ref)
+        assert lat.time == 27.0
+
+
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+@pytest.mark.parametrize("shape", [(37, 29), (200, 130)], ids=["37x29", "200x130"])
+def test_bgk_f32dev_matches_oracle(ctx, edge, shape, steps_per_launch):
+    w, h = shape
+    rng = np.random.default_rng(42)
+    op = O.params(w, h, tau=0.7, edge=edge)
+    rho = (1 + 0.01 * rng.standard_normal((h, w))).astype(np.float32)
+    ux = (0.03 * rng.standard_normal((h, w))).astype(np.float32)
+    uy = (0.03 * rng.standard_normal((h, w))).astype(np.float32)
+    g0 = O.equilibrium_dev(op, rho, ux, uy)
+    with Lattice(ctx, w, h, "f32dev", gpu_params(op)) as lat:
+        lat.init_equilibrium(rho, ux, uy)
+        lat.step(21)
+        np.testing.assert_array_equal(lat.download_raw(), O.run_devf32(op, g0, 21))
+
+
+OPERATORS = {"trt": dict(trt=(0.9, 0.6)), "regularized": dict(regularized=True, tau=0.8),
+             "kbc": dict(kbc=0.05, tau=0.8)}
+
+
+@pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
+@pytest.mark.parametrize("coll", list(OPERATORS))
+def test_operators_match_oracle(ctx, edge, coll, steps_per_launch):
+    w, h = 70, 45
+    rng = np.random.default_rng(43)
+    f = rand_state(rng, h, w)
+    op = O.params(w, h, edge=edge, **OPERATORS[coll])
+    with Lattice(ctx, w, h, "f32", gpu_params(op)) as lat:
+        lat.upload(f)
+        lat.step(16)
+        np.testing.assert_array_equal(lat.download(), O.run(op, f, 16))
+
+
+def test_graph_replay(ctx):
+    """Launch-bound lattices replay captured runs of 64 steps, here eight
+    K = 8 launches each, then take a remainder."""
+    w, h = 128, 96
+    rng = np.random.default_rng(44)
+    f = rand_state(rng, h, w)
+    op = O.params(w, h, tau=0.8, edge=O.EDGE_PERIODIC)
+    with Lattice(ctx, w, h, "f32", gpu_params(op)) as lat:
+        lat.upload(f)
+        lat.step(64 * 3 + 13)
+        np.testing.assert_array_equal(lat.download(), O.run(op, f, 64 * 3 + 13))
+        assert lat.time == 64 * 3 + 13
+
+
+def test_switching_between_kernels(ctx):
+    """Turning the temporal blocking off and on between calls continues the
+    same trajectory (the one-step and K-step launches share the buffers)."""
+    w, h = 150, 90
+    rng = np.random.default_rng(45)
+    f = rand_state(rng, h, w)
+    op = O.params(w, h, tau=0.75, edge=O.EDGE_PERIODIC)
+    try:
+        with Lattice(ctx, w, h, "f32", gpu_params(op)) as lat:
+            lat.upload(f)
+            total = 0
+            for k, n in ((8, 17), (0, 5), (4, 9), (capi.CS_TEMPORAL_AUTO, 64)):
+                capi.call("cs_set_temporal_steps", k)
+                lat.step(n)
+                total += n
+            np.testing.assert_array_equal(lat.download(), O.run(op, f, total))
+    finally:
+        capi.call("cs_set_temporal_steps", capi.CS_TEMPORAL_AUTO)
+
+
+def test_full_size_equals_one_step_kernels(ctx):
+    """4096^2 periodic Taylor-Green (the benchmark lattice), 16 steps: the
+    temporally blocked launches equal the one-step vector kernel bit for bit."""
+    n = 4096
+    rho, ux, uy = W.taylor_green(n, 0.05, np.float32)
+    out = {}
+    try:
+        for k in (8, 0):
+            capi.call("cs_set_temporal_steps", k)
+            with Lattice(ctx, n, n, "f32", Params(tau=0.8, edge=capi.CS_EDGE_PERIODIC)) as lat:
+                lat.init_equilibrium(rho, ux, uy)
+                lat.step(16)
+                out[k] = lat.download_raw()
+    finally:
+        capi.call("cs_set_temporal_steps", capi.CS_TEMPORAL_AUTO)
+    np.testing.assert_array_equal(out[8].view(np.uint32), out[0].view(np.uint32))
