@@ -1,16 +1,20 @@
 """Benchmark of the D2Q9 BGK hot path on one MI355X.
 
 Default workload: periodic Taylor-Green vortex, 4096 x 4096, FP32, tau = 0.8,
-u0 = 0.05, wavenumber 2 pi / 4096; K timed steps after W warm-up steps.  A
-step is one fused pull launch (stream + bounce-back + BGK collide) over the
-whole lattice, with the populations resident in HBM.
+u0 = 0.05, wavenumber 2 pi / 4096; K timed steps after W warm-up steps, with
+the populations resident in HBM.  A step is one full update (stream +
+bounce-back + BGK collide) of every cell.  FP32 lattices without geometry
+take the temporally blocked kernel (8 steps per launch out of LDS); the
+one-step kernel of the same workload is timed beside it in `extras`.
 
 Prints ONE JSON line.  `value` is lattice updates per second (GLUPS) over the
 timed steps (wall clock around a synchronised region); the roofline object
 prices the step kernel's average duration (HIP events on the lattice's own
-stream) against 72 algorithmic bytes per cell update.  The CPU baseline is
-the oracle's fused FP32 step (OpenMP over rows) on a bounded sample of the
-same lattice.
+stream) against the bytes that kernel must move: for the temporally blocked
+kernel each 64 x 64 region (interior and halo) read once and each interior
+cell written once per launch, for the one-step kernels 72 bytes per cell
+update.  The CPU baseline is the oracle's fused FP32 step (OpenMP over rows)
+on a bounded sample of the same lattice.
 """
 from __future__ import annotations
 
@@ -38,6 +42,25 @@ def hbm_frac(bytes_per_cell: float, cells: int, kernel_ms: float) -> float:
     """Algorithmic bytes of one step launch over its duration, as a fraction
     of the MI355X HBM peak."""
     return round(bytes_per_cell * cells / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
+
+TB_EDGE, TB_STEPS = 64, 8  # region side and steps per launch of the temporally blocked kernel
+
+
+def tb_bytes_per_step(w: int, h: int, esize: int = 4, k: int = TB_STEPS) -> float:
+    """Bytes one launch of the temporally blocked kernel must move, per step:
+    every 64 x 64 region (interior of side 64 - 2k plus halo) read once, every
+    lattice cell written once, 9 populations of `esize` bytes."""
+    inner = TB_EDGE - 2 * k
+    regions = -(-w // inner) * -(-h // inner)
+    return (regions * TB_EDGE * TB_EDGE + w * h) * 9 * esize / k
+
+
+def step_bytes_per_cell(storage: str, n: int, temporal: bool) -> float:
+    """Algorithmic bytes per cell update of the kernel the lattice takes."""
+    if temporal and storage in ("f32", "f32dev"):
+        return tb_bytes_per_step(n, n) / (n * n)
+    return BYTES_PER_CELL[storage]
 
 
 def tgv_lattice(ctx, n, storage, tau, u0, rounding="sr", trt=None, collision=None, ks_viscosity=0.0):
@@ -172,12 +195,25 @@ def tgv_8192_run(ctx, tau, u0, steps=200):
         r, x, y = lat.moments_f64()
         lat.close()
         out[st] = {"GLUPS": round(cells * steps / w / 1e9, 3), "kernel_us": round(ms / steps * 1e3, 1),
-                   "hbm_frac": hbm_frac(BYTES_PER_CELL[st], cells, ms / steps),
+                   "hbm_frac": hbm_frac(step_bytes_per_cell(st, n, True), cells, ms / steps),
                    "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
                    "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden)}
         del r, x, y
+    # the one-step FP32 kernel (72 B per cell update) on the same lattice
+    from chemsim_amd import capi
+    lat = tgv_lattice(ctx, n, "f32", tau, u0, "rn")
+    capi.call("cs_set_temporal_steps", 0)
+    try:
+        lat.step(5)
+        w, ms = time_steps(lat, steps)
+    finally:
+        capi.call("cs_set_temporal_steps", capi.CS_TEMPORAL_AUTO)
+        lat.close()
+    out["f32_one_step"] = {"GLUPS": round(cells * steps / w / 1e9, 3), "kernel_us": round(ms / steps * 1e3, 1),
+                           "hbm_frac": hbm_frac(72, cells, ms / steps)}
     out["bf16_over_f32"] = round(out["bf16"]["GLUPS"] / out["f32"]["GLUPS"], 3)
     out["f16_over_f32"] = round(out["f16"]["GLUPS"] / out["f32"]["GLUPS"], 3)
+    out["f16_over_f32_one_step"] = round(out["f16"]["GLUPS"] / out["f32_one_step"]["GLUPS"], 3)
     return out
 
 
@@ -283,7 +319,8 @@ def main():
     s_final = lat.sums_f64()
     glups = cells * args.steps / wall / 1e9
     kernel_ms = ms / args.steps
-    bpc = BYTES_PER_CELL[storage]
+    temporal = storage in ("f32", "f32dev")
+    bpc = step_bytes_per_cell(storage, n, temporal)
     achieved = bpc * cells / (kernel_ms * 1e-3) / 1e9
     log(f"[bench] {glups:.2f} GLUPS wall, {kernel_ms * 1e3:.1f} us/step (events), {achieved:.0f} GB/s algorithmic")
     # four more timed regions of the same length, for the spread (value stays the first)
@@ -304,6 +341,23 @@ def main():
               "mass_final": float(s_final[0]),
               "repeats_GLUPS": [round(r, 3) for r in reps],
               "median_GLUPS": round(float(np.median(reps)), 3)}
+    if temporal:
+        # the one-step kernel (vector lanes, 72 B per cell update) on the same lattice and steps
+        from chemsim_amd import capi as _cap
+        _cap.call("cs_set_temporal_steps", 0)
+        try:
+            w1, ms1 = time_steps(lat, args.steps)
+        finally:
+            _cap.call("cs_set_temporal_steps", _cap.CS_TEMPORAL_AUTO)
+        extras["one_step_kernel"] = {"GLUPS": round(cells * args.steps / w1 / 1e9, 3),
+                                     "kernel_us": round(ms1 / args.steps * 1e3, 2),
+                                     "hbm_frac": hbm_frac(BYTES_PER_CELL[storage], cells, ms1 / args.steps),
+                                     "bytes_per_cell": BYTES_PER_CELL[storage]}
+        extras["temporal_blocking"] = {"steps_per_launch": TB_STEPS, "region": [TB_EDGE, TB_EDGE],
+                                       "bytes_per_cell_update": round(bpc, 3),
+                                       "speedup_vs_one_step": round(extras["one_step_kernel"]["kernel_us"]
+                                                                    / (kernel_ms * 1e3), 3)}
+        log(f"[bench] one-step kernel: {extras['one_step_kernel']}")
 
     if not args.no_extras and storage == "f32":
         # FP64 oracle run and 16-bit storage on the same workload, same steps
@@ -335,7 +389,7 @@ def main():
             l2.close()
             key = st if st in ("f32", "f32dev") else f"{st}_{rnd}"
             extras[key] = {"GLUPS": round(cells * args.steps / w2 / 1e9, 3),
-                          "hbm_frac": hbm_frac(BYTES_PER_CELL[st], cells, ms2 / args.steps),
+                          "hbm_frac": hbm_frac(step_bytes_per_cell(st, n, True), cells, ms2 / args.steps),
                           "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
                           "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden),
                           "mass_drift_rel": abs(m1 - m0) / m0}
@@ -347,7 +401,7 @@ def main():
         wt, mst = time_steps(lt, args.steps)
         lt.close()
         extras["trt_f32"] = {"GLUPS": round(cells * args.steps / wt / 1e9, 3),
-                             "hbm_frac": hbm_frac(72, cells, mst / args.steps),
+                             "hbm_frac": hbm_frac(step_bytes_per_cell("f32", n, True), cells, mst / args.steps),
                              "kernel_us": round(mst / args.steps * 1e3, 2), "tau_plus": tp, "tau_minus": tm}
         log(f"[bench] trt: {extras['trt_f32']}")
         # Regularized (lbm.rs:596-666) and KBC (lbm.rs:458-590, nu of tau 0.8) on the same workload
@@ -358,7 +412,7 @@ def main():
             wo, mso = time_steps(lo, args.steps)
             lo.close()
             extras[key] = {"GLUPS": round(cells * args.steps / wo / 1e9, 3),
-                           "hbm_frac": hbm_frac(72, cells, mso / args.steps),
+                           "hbm_frac": hbm_frac(step_bytes_per_cell("f32", n, True), cells, mso / args.steps),
                            "kernel_us": round(mso / args.steps * 1e3, 2)}
             log(f"[bench] {key}: {extras[key]}")
         # render pass (render.rs) of the 4096^2 state, each display mode; bytes moved per
@@ -418,7 +472,8 @@ def main():
         cpu["host"] = host_cpu()
         log(f"[bench] cpu baseline {cg:.4f} GLUPS on {threads} threads ({csteps} steps)")
 
-    traffic = load_profile_traffic(f"k_step_{storage}_periodic_fluid_{n}")
+    traffic = load_profile_traffic(f"k_step_tb_{storage}_periodic_fluid_{n}" if temporal
+                                   else f"k_step_{storage}_periodic_fluid_{n}")
     line = {
         "metric": METRIC,
         "value": round(glups, 3),
@@ -439,7 +494,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "bytes_per_cell": bpc, "kernel_us": round(kernel_ms * 1e3, 2)},
+                     "bytes_per_cell": round(bpc, 3), "kernel_us": round(kernel_ms * 1e3, 2),
+                     "kernel": "k_step_tb (8 steps per launch)" if temporal else "k_step / k_step_vec",
+                     "per": "launch" if temporal else "step"},
         "cpu_baseline": cpu,
         "extras": extras,
     }
