@@ -56,10 +56,11 @@ def tb_bytes_per_step(w: int, h: int, esize: int = 4, k: int = TB_STEPS) -> floa
     return (regions * TB_EDGE * TB_EDGE + w * h) * 9 * esize / k
 
 
-def step_bytes_per_cell(storage: str, n: int, temporal: bool) -> float:
-    """Algorithmic bytes per cell update of the kernel the lattice takes."""
-    if temporal and storage in ("f32", "f32dev"):
-        return tb_bytes_per_step(n, n) / (n * n)
+def step_bytes_per_cell(storage: str, n: int, k: int) -> float:
+    """Algorithmic bytes per cell update of the kernel an n x n lattice takes
+    (k: steps per launch, Lattice.temporal_steps(); 1 = a one-step kernel)."""
+    if k > 1:
+        return tb_bytes_per_step(n, n, k=k) / (n * n)
     return BYTES_PER_CELL[storage]
 
 
@@ -187,6 +188,7 @@ def tgv_8192_run(ctx, tau, u0, steps=200):
     out = {"steps": steps, "f64_GLUPS": round(cells * steps / w64 / 1e9, 3)}
     for st, rnd in (("f32", "rn"), ("bf16", "sr"), ("f16", "sr")):
         lat = tgv_lattice(ctx, n, st, tau, u0, rnd)
+        k = lat.temporal_steps()
         lat.step(5)
         w, ms = time_steps(lat, steps)
         lat.close()
@@ -195,7 +197,7 @@ def tgv_8192_run(ctx, tau, u0, steps=200):
         r, x, y = lat.moments_f64()
         lat.close()
         out[st] = {"GLUPS": round(cells * steps / w / 1e9, 3), "kernel_us": round(ms / steps * 1e3, 1),
-                   "hbm_frac": hbm_frac(step_bytes_per_cell(st, n, True), cells, ms / steps),
+                   "hbm_frac": hbm_frac(step_bytes_per_cell(st, n, k), cells, ms / steps), "steps_per_launch": k,
                    "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
                    "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden)}
         del r, x, y
@@ -319,8 +321,9 @@ def main():
     s_final = lat.sums_f64()
     glups = cells * args.steps / wall / 1e9
     kernel_ms = ms / args.steps
-    temporal = storage in ("f32", "f32dev")
-    bpc = step_bytes_per_cell(storage, n, temporal)
+    k_launch = lat.temporal_steps()
+    temporal = k_launch > 1
+    bpc = step_bytes_per_cell(storage, n, k_launch)
     achieved = bpc * cells / (kernel_ms * 1e-3) / 1e9
     log(f"[bench] {glups:.2f} GLUPS wall, {kernel_ms * 1e3:.1f} us/step (events), {achieved:.0f} GB/s algorithmic")
     # four more timed regions of the same length, for the spread (value stays the first)
@@ -353,7 +356,7 @@ def main():
                                      "kernel_us": round(ms1 / args.steps * 1e3, 2),
                                      "hbm_frac": hbm_frac(BYTES_PER_CELL[storage], cells, ms1 / args.steps),
                                      "bytes_per_cell": BYTES_PER_CELL[storage]}
-        extras["temporal_blocking"] = {"steps_per_launch": TB_STEPS, "region": [TB_EDGE, TB_EDGE],
+        extras["temporal_blocking"] = {"steps_per_launch": k_launch, "region": [TB_EDGE, TB_EDGE],
                                        "bytes_per_cell_update": round(bpc, 3),
                                        "speedup_vs_one_step": round(extras["one_step_kernel"]["kernel_us"]
                                                                     / (kernel_ms * 1e3), 3)}
@@ -382,6 +385,7 @@ def main():
         for st, rnd in (("f32", "rn"), ("f32dev", "rn"), ("bf16", "sr"), ("f16", "sr"), ("bf16", "rn"),
                         ("f16", "rn")):
             l2 = tgv_lattice(ctx, n, st, args.tau, args.u0, rnd)
+            k2 = l2.temporal_steps()
             m0 = l2.sums_f64()[0]
             w2, ms2 = time_steps(l2, args.steps)
             r, x, y = l2.moments_f64()
@@ -389,7 +393,8 @@ def main():
             l2.close()
             key = st if st in ("f32", "f32dev") else f"{st}_{rnd}"
             extras[key] = {"GLUPS": round(cells * args.steps / w2 / 1e9, 3),
-                          "hbm_frac": hbm_frac(step_bytes_per_cell(st, n, True), cells, ms2 / args.steps),
+                          "hbm_frac": hbm_frac(step_bytes_per_cell(st, n, k2), cells, ms2 / args.steps),
+                          "steps_per_launch": k2,
                           "u_rel_L2_vs_f64": float(np.sqrt(((x - x64) ** 2 + (y - y64) ** 2).sum()) / den),
                           "rho_dev_rel_L2_vs_f64": float(np.sqrt(((r - r64) ** 2).sum()) / rden),
                           "mass_drift_rel": abs(m1 - m0) / m0}
@@ -398,10 +403,12 @@ def main():
         from chemsim_amd.lattice import trt_new
         tp, tm = trt_new(0.25, (args.tau - 0.5) / 3.0)
         lt = tgv_lattice(ctx, n, "f32", args.tau, args.u0, trt=(tp, tm))
+        kt = lt.temporal_steps()
         wt, mst = time_steps(lt, args.steps)
         lt.close()
         extras["trt_f32"] = {"GLUPS": round(cells * args.steps / wt / 1e9, 3),
-                             "hbm_frac": hbm_frac(step_bytes_per_cell("f32", n, True), cells, mst / args.steps),
+                             "hbm_frac": hbm_frac(step_bytes_per_cell("f32", n, kt), cells, mst / args.steps),
+                             "steps_per_launch": kt,
                              "kernel_us": round(mst / args.steps * 1e3, 2), "tau_plus": tp, "tau_minus": tm}
         log(f"[bench] trt: {extras['trt_f32']}")
         # Regularized (lbm.rs:596-666) and KBC (lbm.rs:458-590, nu of tau 0.8) on the same workload
@@ -409,10 +416,12 @@ def main():
         for key, coll in (("regularized_f32", _cc.CS_COLL_REGULARIZED), ("kbc_f32", _cc.CS_COLL_KBC)):
             lo = tgv_lattice(ctx, n, "f32", args.tau, args.u0, collision=coll,
                              ks_viscosity=(args.tau - 0.5) / 3.0)
+            ko = lo.temporal_steps()
             wo, mso = time_steps(lo, args.steps)
             lo.close()
             extras[key] = {"GLUPS": round(cells * args.steps / wo / 1e9, 3),
-                           "hbm_frac": hbm_frac(step_bytes_per_cell("f32", n, True), cells, mso / args.steps),
+                           "hbm_frac": hbm_frac(step_bytes_per_cell("f32", n, ko), cells, mso / args.steps),
+                           "steps_per_launch": ko,
                            "kernel_us": round(mso / args.steps * 1e3, 2)}
             log(f"[bench] {key}: {extras[key]}")
         # render pass (render.rs) of the 4096^2 state, each display mode; bytes moved per
@@ -495,7 +504,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "bytes_per_cell": round(bpc, 3), "kernel_us": round(kernel_ms * 1e3, 2),
-                     "kernel": "k_step_tb (8 steps per launch)" if temporal else "k_step / k_step_vec",
+                     "kernel": f"k_step_tb ({k_launch} steps per launch)" if temporal else "k_step / k_step_vec",
                      "per": "launch" if temporal else "step"},
         "cpu_baseline": cpu,
         "extras": extras,

@@ -127,6 +127,7 @@ SIGNATURES = {
     "cs_lattice_record": [_P, _I, _I, _I, _I, _I, FRAME_FN, _P, C.POINTER(cs_record_stats)],
     "cs_lattice_step": [_P, _I],
     "cs_lattice_step_timed": [_P, _I, _FP],
+    "cs_lattice_temporal_steps": [_P, _IP],
     "cs_lattice_stream": [_P],
     "cs_lattice_bounce_back": [_P],
     "cs_lattice_collide": [_P],

@@ -119,6 +119,11 @@ struct cs_lattice;
 namespace cs {
 // n fused steps on the lattice's stream (cs_lattice_step without the checks).
 void do_steps(cs_lattice* l, int n);
+// Temporally blocked steps (lattice_tb.hip): steps per launch this lattice
+// takes (0: one-step kernels; cpt_override: the cs_set_cells_per_thread
+// value), and one launch of k steps (advances cur, steps and time).
+int tb_steps(const cs_lattice* l, int cpt_override);
+void launch_tb(cs_lattice* l, int k);
 // The render passes of the current populations into d_out (device, h*w pixels
 // of `channels` bytes: 3 = RGB, 4 = RGBA with alpha 255), enqueued on the
 // lattice's stream; allocates the lattice's render scratch on first use.

@@ -14,7 +14,6 @@
 
 #include "cs_internal.hpp"
 #include "lbm_device.hpp"
-#include "lbm_tb.hpp"
 #include "lattice_kernels.hpp"
 
 namespace cs {
@@ -621,80 +620,6 @@ static void replay_steps(cs_lattice* l, int k)
     l->steps += GRAPH_STEPS;
 }
 
-// ---- temporally blocked steps (K steps per launch) ------------------------
-// Lattices without geometry in 32-bit storage take k_step_tb: K = 8 steps per
-// launch (4096^2 FP32 Taylor-Green, microseconds per step: one-step vector
-// kernel 203; K = 2 197, K = 4 137, K = 6 125, K = 8 116).  Beyond K = 8 the
-// halo's recomputation outgrows the saved traffic.  cs_set_temporal_steps
-// switches it off or fixes K; the remainder of a call's steps (n mod K) runs
-// one step per launch.  A launch-shape override from cs_set_cells_per_thread
-// selects the one-step kernels.
-static std::atomic<int> g_tb_steps{CS_TEMPORAL_AUTO};
-
-template <class St, int EDGE, int K, int COLL>
-static void launch_tb_k(cs_lattice* l)
-{
-    constexpr int IN = TB_EDGE - 2 * K;
-    StepArgs<typename St::T> a{};
-    a.fin = l->buf[l->cur];
-    a.fout = l->buf[l->cur ^ 1];
-    a.node = nullptr;
-    a.w = l->w;
-    a.h = l->h;
-    a.pitch = l->pitch;
-    a.cpitch = l->cpitch;
-    a.plane = l->plane;
-    a.tiles_x = (l->w + IN - 1) / IN;
-    a.k = consts_of<St>(l);
-    const int tiles_y = (l->h + IN - 1) / IN;
-    hipLaunchKernelGGL((k_step_tb<St, EDGE, K, COLL>), dim3((unsigned)(a.tiles_x * tiles_y)), dim3(TB_THREADS), 0,
-                       l->stream, a);
-    l->cur ^= 1;
-    l->steps += K;
-    for (int s = 0; s < K; ++s) l->time += l->params.delta_t;  // as step by step (lbm.rs:713)
-}
-
-template <class St, int EDGE, int K>
-static void launch_tb_coll(cs_lattice* l)
-{
-    if constexpr (std::is_same_v<St, StF32>) {
-        switch (l->params.collision) {
-        case CS_COLL_TRT: launch_tb_k<St, EDGE, K, CS_COLL_TRT>(l); return;
-        case CS_COLL_REGULARIZED: launch_tb_k<St, EDGE, K, CS_COLL_REGULARIZED>(l); return;
-        case CS_COLL_KBC: launch_tb_k<St, EDGE, K, CS_COLL_KBC>(l); return;
-        default: break;
-        }
-    }
-    launch_tb_k<St, EDGE, K, CS_COLL_BGK>(l);
-}
-
-// Steps per temporally blocked launch for this lattice (0: one-step kernels).
-static int tb_steps(const cs_lattice* l)
-{
-    const int k = g_tb_steps.load(std::memory_order_relaxed);
-    // a launch-shape override (cs_set_cells_per_thread) asks for a one-step kernel
-    if (k == 0 || l->has_flags || g_cpt_override.load(std::memory_order_relaxed) != 0) return 0;
-    if (l->storage != CS_F32 && l->storage != CS_F32DEV) return 0;
-    if (pop_elems(l) * 4 >= ((int64_t)1 << 32)) return 0;
-    return k == CS_TEMPORAL_AUTO ? 8 : k;
-}
-
-static void launch_tb(cs_lattice* l, int k)
-{
-    auto go = [&]<class St, int EDGE>() {
-        if (k == 4) launch_tb_coll<St, EDGE, 4>(l);
-        else launch_tb_coll<St, EDGE, 8>(l);
-    };
-    const bool periodic = l->params.edge == CS_EDGE_PERIODIC;
-    if (l->storage == CS_F32) {
-        if (periodic) go.template operator()<StF32, CS_EDGE_PERIODIC>();
-        else go.template operator()<StF32, CS_EDGE_ZERO>();
-    } else {
-        if (periodic) go.template operator()<StF32Dev, CS_EDGE_PERIODIC>();
-        else go.template operator()<StF32Dev, CS_EDGE_ZERO>();
-    }
-}
-
 // n steps from the current state: K-step launches while n >= K, then one
 // step per launch.
 static void advance(cs_lattice* l, int n, int k)
@@ -706,7 +631,7 @@ static void advance(cs_lattice* l, int n, int k)
 
 void do_steps(cs_lattice* l, int n)
 {
-    const int k = tb_steps(l);
+    const int k = tb_steps(l, g_cpt_override.load(std::memory_order_relaxed));
     if (n >= GRAPH_STEPS && graph_eligible(l))
         for (; n >= GRAPH_STEPS; n -= GRAPH_STEPS) replay_steps(l, k);
     advance(l, n, k);
@@ -1183,6 +1108,16 @@ extern "C" int cs_lattice_step_timed(cs_lattice* l, int n, float* ms)
     });
 }
 
+extern "C" int cs_lattice_temporal_steps(const cs_lattice* l, int* k)
+{
+    return guarded([&] {
+        check_lat(l);
+        CS_REQUIRE(k, CS_ERR_ARG, "null output");
+        const int t = tb_steps(l, g_cpt_override.load(std::memory_order_relaxed));
+        *k = t > 0 ? t : 1;
+    });
+}
+
 static void require_unfused(const cs_lattice* l)
 {
     CS_REQUIRE(l->storage == CS_F32 || l->storage == CS_F64, CS_ERR_UNSUPPORTED,
@@ -1574,15 +1509,6 @@ extern "C" int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_pe
 extern "C" int cs_set_step_graphs(int enable)
 {
     return guarded([&] { g_step_graphs.store(enable != 0, std::memory_order_relaxed); });
-}
-
-extern "C" int cs_set_temporal_steps(int k)
-{
-    return guarded([&] {
-        CS_REQUIRE(k == CS_TEMPORAL_AUTO || k == 0 || k == 4 || k == 8, CS_ERR_ARG,
-                   "temporal steps must be CS_TEMPORAL_AUTO, 0, 4 or 8");
-        g_tb_steps.store(k, std::memory_order_relaxed);
-    });
 }
 
 extern "C" int cs_set_cells_per_thread(int cpt)

@@ -17,7 +17,15 @@
 // number of times, so lattices smaller than a region work too).  The
 // per-cell arithmetic is the one-step kernels' (same pull order, same
 // collision functions), so K steps here equal K launches of k_step bit for
-// bit.  No geometry (every cell fluid), 32-bit storage.
+// bit.  32-bit storage.
+//
+// Geometry (MODE_REF, MODE_HALF): a cell's node word (its flag, which of its
+// pull sources are solid, which of those are a moving lid) does not change
+// between steps, so each lane keeps its cells' words in registers and the
+// pull follows gather(): the reference's on-node swap at solid cells, or
+// half-way links to the cell's own opposite population (+ the lid term) and
+// Zou-He at inlet/outlet cells.  Solid cells of the half-way mode are carried
+// unchanged, i.e. never written.
 #pragma once
 
 #include "lbm_device.hpp"
@@ -40,6 +48,42 @@ __device__ __forceinline__ void collide_cell(const Consts<typename St::T>& k, ty
     else bgk_collide(k, B);
 }
 
+// Pulls of region cell (rr, lane) for the step, in gather()'s order: the
+// normal source of each direction, on solid cells of the reference mode the
+// opposite direction's source (lbm.rs:745), in the half-way mode the cell's
+// own opposite population (+ lid term) over a solid link, then Zou-He.
+template <class St, int MODE>
+__device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][TB_EDGE][TB_EDGE], int rr, int lane,
+                                          uint32_t info, const Consts<typename St::T>& k, typename St::T (&B)[9])
+{
+    using T = typename St::T;
+    const int type = MODE == MODE_FLUID ? (int)CS_FLUID : (int)(info >> NODE_TYPE_SHIFT);
+    const bool solid = type == CS_WALL || type == CS_LID;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        T v = L[i][rr + CX[i]][(lane - CY[i]) & (TB_EDGE - 1)];
+        if (MODE == MODE_REF) {
+            const int j = OPP[i];
+            if (solid) v = L[j][rr + CX[j]][(lane - CY[j]) & (TB_EDGE - 1)];
+        } else if (MODE == MODE_HALF) {
+            if (!solid && ((info >> i) & 1u)) {
+                v = L[OPP[i]][rr][lane];
+                if ((info >> (9 + i)) & 1u) v = v + k.lid(i);
+            }
+        }
+        B[i] = v;
+    }
+    if (MODE == MODE_HALF) {
+        if constexpr (St::DEV) {
+            if (type == CS_INLET) zou_he_inlet_dev<T>(B, k.rho0, k.in_ux, k.in_uy);
+            else if (type == CS_OUTLET) zou_he_outlet_dev<T>(B, k.rho0, k.out_rho);
+        } else {
+            if (type == CS_INLET) zou_he_inlet<T>(B, k.in_ux, k.in_uy);
+            else if (type == CS_OUTLET) zou_he_outlet<T>(B, k.out_rho);
+        }
+    }
+}
+
 // Region of tile t: rows ty*(64-2K) - K .., columns tx*(64-2K) - K ..
 // (a.tiles_x = ceil(w / (64-2K))); blocks of TB_THREADS threads, one block
 // per CU (the region takes 144 KiB of LDS).  A wave's LDS accesses are 64
@@ -47,7 +91,7 @@ __device__ __forceinline__ void collide_cell(const Consts<typename St::T>& k, ty
 // within the row and belongs to a lane that discards its result), so they
 // are free of bank conflicts, and each lane's nine pull addresses per row
 // slot stay the same from step to step.
-template <class St, int EDGE, int K, int COLL>
+template <class St, int EDGE, int MODE, int K, int COLL>
 __global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T> a)
 {
     using S = typename St::S;
@@ -76,7 +120,8 @@ __global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T>
         cin = gc >= 0 && gc < a.w;
         gc = cin ? gc : 0;
     }
-    bool inside[TB_SLOTS];
+    bool upd[TB_SLOTS];       // the cell is inside the lattice and not carried
+    uint32_t info[TB_SLOTS];  // node words (geometry modes)
 #pragma unroll
     for (int j = 0; j < TB_SLOTS; ++j) {
         const int rr = wave + 16 * j;
@@ -89,13 +134,17 @@ __global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T>
             rin = gr >= 0 && gr < a.h;
             gr = rin ? gr : 0;
         }
-        inside[j] = cin && rin;
+        const bool inside = cin && rin;
+        info[j] = 0;
+        if (MODE != MODE_FLUID) info[j] = inside ? a.node[(uint32_t)gr * (uint32_t)a.cpitch + (uint32_t)gc] : 0u;
+        const int type = (int)(info[j] >> NODE_TYPE_SHIFT);
+        upd[j] = inside && !(MODE == MODE_HALF && (type == CS_WALL || type == CS_LID));
         const uint32_t o = (uint32_t)gr * (uint32_t)a.pitch + (uint32_t)gc;
         T v[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) v[i] = St::wid(*at<uint32_t>(fin, (uint32_t)i * P + o));
 #pragma unroll
-        for (int i = 0; i < 9; ++i) L[i][rr][lane] = inside[j] ? v[i] : (St::DEV ? -a.k.base(i) : T(0));
+        for (int i = 0; i < 9; ++i) L[i][rr][lane] = inside ? v[i] : (St::DEV ? -a.k.base(i) : T(0));
     }
 
 #pragma unroll 1
@@ -106,8 +155,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T>
         for (int j = 0; j < TB_SLOTS; ++j) {
             const int rr = wave + 16 * j;
             if (rr >= s && rr < TB_EDGE - s) {  // uniform over the wave
-#pragma unroll
-                for (int i = 0; i < 9; ++i) B[j][i] = L[i][rr + CX[i]][(lane - CY[i]) & (TB_EDGE - 1)];
+                tb_gather<St, MODE>(L, rr, lane, info[j], a.k, B[j]);
                 collide_cell<St, COLL>(a.k, B[j]);
             }
         }
@@ -116,7 +164,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T>
 #pragma unroll
         for (int j = 0; j < TB_SLOTS; ++j) {
             const int rr = wave + 16 * j;
-            if (rr >= s && rr < TB_EDGE - s && cols && inside[j]) {
+            if (rr >= s && rr < TB_EDGE - s && cols && upd[j]) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) L[i][rr][lane] = B[j][i];
             }

@@ -214,6 +214,12 @@ class Lattice:
         call("cs_lattice_step_timed", self.handle, int(n), C.byref(ms))
         return ms.value
 
+    def temporal_steps(self) -> int:
+        """Steps per launch cs_lattice_step takes now (1: one-step kernels)."""
+        k = C.c_int()
+        call("cs_lattice_temporal_steps", self.handle, C.byref(k))
+        return k.value
+
     def stream(self) -> None:
         call("cs_lattice_stream", self.handle)
 

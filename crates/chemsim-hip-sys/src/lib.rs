@@ -170,6 +170,7 @@ extern "C" {
     // the step
     pub fn cs_lattice_step(lat: *mut cs_lattice, n: c_int) -> c_int;
     pub fn cs_lattice_step_timed(lat: *mut cs_lattice, n: c_int, ms: *mut f32) -> c_int;
+    pub fn cs_lattice_temporal_steps(lat: *const cs_lattice, k: *mut c_int) -> c_int;
     pub fn cs_lattice_stream(lat: *mut cs_lattice) -> c_int;
     pub fn cs_lattice_bounce_back(lat: *mut cs_lattice) -> c_int;
     pub fn cs_lattice_collide(lat: *mut cs_lattice) -> c_int;

@@ -162,11 +162,16 @@ int cs_lattice_init_nonequilibrium(cs_lattice* lat, const float* rho, const floa
 
 /* ---- the step ------------------------------------------------------------ */
 /* n x State::step (lbm.rs:694-714): stream, bounce-back and BGK collide fused
- * into one pull kernel per step.  Asynchronous. */
+ * into one pull kernel per step, or (cs_lattice_temporal_steps) k steps per
+ * launch with the remainder one step at a time.  Asynchronous. */
 int cs_lattice_step(cs_lattice* lat, int n);
 /* As cs_lattice_step, bracketed by HIP events on the lattice's stream;
  * synchronises and returns the elapsed milliseconds. */
 int cs_lattice_step_timed(cs_lattice* lat, int n, float* ms);
+/* Steps per launch the lattice's cs_lattice_step takes now: k > 1 for the
+ * temporally blocked kernel (cs_set_temporal_steps), 1 for the one-step
+ * kernels.  No reference counterpart. */
+int cs_lattice_temporal_steps(const cs_lattice* lat, int* k);
 /* The three phases alone (State::stream :716-729, State::bounce_back
  * :741-751, State::collide :731-739), REFERENCE mode, F32/F64 storage.
  * stream + bounce_back + collide equals one cs_lattice_step. */
@@ -370,11 +375,12 @@ int cs_set_cells_per_thread(int cells_per_thread);
  * with stochastic rounding always launches step by step.  Process-wide; for
  * tests and tuning.  No reference counterpart. */
 int cs_set_step_graphs(int enable);
-/* Steps per launch of the temporally blocked step kernel, which lattices
- * without geometry in FP32 or FP32-deviation storage take: CS_TEMPORAL_AUTO
- * (the default, 8 steps), 4 or 8 steps, or 0 for one step per launch.  Results
- * are identical either way.  A launch-shape override of
- * cs_set_cells_per_thread selects the one-step kernels instead.
+/* Steps per launch of the temporally blocked step kernel: CS_TEMPORAL_AUTO
+ * (the default: 8 steps for FP32 and FP32-deviation lattices of at least
+ * 192 regions of 48 x 48 cells -- about 600^2 -- except with the KBC
+ * operator), 4 or 8 (every FP32 and FP32-deviation lattice), or 0 for one
+ * step per launch.  Results are identical either way.  A launch-shape
+ * override of cs_set_cells_per_thread selects the one-step kernels instead.
  * Process-wide; for tests and tuning.  No reference counterpart. */
 enum { CS_TEMPORAL_AUTO = -1 };
 int cs_set_temporal_steps(int k);

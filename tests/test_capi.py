@@ -52,3 +52,5 @@ def test_abi_version_and_error_path():
         assert capi.lib.cs_set_temporal_steps(k) == capi.CS_ERR_ARG
     for k in (0, 4, 8, capi.CS_TEMPORAL_AUTO):
         assert capi.lib.cs_set_temporal_steps(k) == capi.CS_OK
+    k = C.c_int()
+    assert capi.lib.cs_lattice_temporal_steps(None, C.byref(k)) == capi.CS_ERR_ARG
