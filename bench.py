@@ -239,8 +239,8 @@ def cavity_128_run(ctx, steps=10000):
 def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
     """512 independent lid-driven cavities (sides U{64..512}, tau U[0.55,1],
     lid U[0.02,0.1], seed 0), FP32, half-way walls: the autotuned batch launch
-    for `steps` steps, the other launch kind (persistent / tile list) for
-    `other_steps`, and one launch per lattice per step."""
+    for `steps` steps, the other launch kinds (persistent, tile list,
+    temporal) for `other_steps`, and one launch per lattice per step."""
     from chemsim_amd import Params, capi
     from chemsim_amd import workloads as W
     from chemsim_amd.batch import Batch, tune
@@ -252,24 +252,31 @@ def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
         b.init_equilibrium(i, *W.rest_state(s.side, s.side))
     cache = os.environ.get("CHEMSIM_TUNING_CACHE",
                            os.path.join(os.path.expanduser("~"), ".cache", "chemsim_amd", "batch_tuning.json"))
-    rec = tune(ctx, b, cache, trial_steps=20)
+    rec = tune(ctx, b, cache, trial_steps=24)
     ms = b.step(steps)
     best = b.tuning
-    other = dict(best)
-    if best.get("launch", 0) == capi.CS_LAUNCH_TILES:
-        other.update(launch=capi.CS_LAUNCH_PERSISTENT, block_threads=1024, workgroups_per_cu=2, cells_per_thread=1)
-    else:
-        other.update(launch=capi.CS_LAUNCH_TILES)
-    b.tuning = other
-    ms_other = b.step(other_steps)
+    kinds = {capi.CS_LAUNCH_PERSISTENT: "persistent", capi.CS_LAUNCH_TILES: "tiles",
+             capi.CS_LAUNCH_TEMPORAL: "temporal"}
+    others = {}
+    for kind in kinds:
+        if kind == best.get("launch", 0):
+            continue
+        t = dict(best, launch=kind)
+        if kind == capi.CS_LAUNCH_PERSISTENT:
+            t.update(block_threads=1024, workgroups_per_cu=2, cells_per_thread=1)
+        b.tuning = t
+        others["GLUPS_" + kinds[kind]] = round(b.cells * other_steps / b.step(other_steps) / 1e6, 3)
     b.tuning = best
     ms_pl = b.step_per_lattice(per_lattice_steps)
     cells = b.cells
-    kinds = {capi.CS_LAUNCH_PERSISTENT: "persistent", capi.CS_LAUNCH_TILES: "tiles"}
+    # bytes the launch kind must move per cell update: 72 for one-step launches; for the
+    # temporal kind every 64 x 64 region read and every cell written once per 8 steps
+    bpc = 72.0
+    if best.get("launch", 0) == capi.CS_LAUNCH_TEMPORAL:
+        bpc = sum(tb_bytes_per_step(s.side, s.side) for s in specs) / cells
     out = {"lattices": len(b), "cells": cells, "steps": steps,
-           "GLUPS_batch": round(cells * steps / ms / 1e6, 3), "hbm_frac": hbm_frac(72, cells, ms / steps),
-           "launch": kinds[best.get("launch", 0)],
-           "GLUPS_" + kinds[other["launch"]]: round(cells * other_steps / ms_other / 1e6, 3),
+           "GLUPS_batch": round(cells * steps / ms / 1e6, 3), "hbm_frac": hbm_frac(bpc, cells, ms / steps),
+           "bytes_per_cell_update": round(bpc, 3), "launch": kinds[best.get("launch", 0)], **others,
            "GLUPS_per_lattice_launches": round(cells * per_lattice_steps / ms_pl / 1e6, 3),
            "seconds": round(ms / 1e3, 3), "tuning": rec["tuning"]}
     b.close()

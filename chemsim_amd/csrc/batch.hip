@@ -9,7 +9,11 @@
 //    inter-workgroup synchronisation;
 //  - tiles (k_batch_tiles): one launch per step over the flattened list of
 //    4 x 64 tiles of every lattice, so a large lattice is spread over the
-//    whole GPU.
+//    whole GPU;
+//  - temporal (k_batch_tb, 32-bit storage): one launch per 8 steps over the
+//    flattened list of 64 x 64 regions of every lattice, each block running
+//    the 8 steps out of LDS (lbm_tb.hpp); the remainder of a call's steps
+//    (steps mod 8) runs as tile launches.
 // The per-cell code is the single-lattice gather/finish either way, so the
 // results equal per-lattice launches bit for bit.
 #include <hip/hip_runtime.h>
@@ -26,6 +30,7 @@
 #include "cs_internal.hpp"
 #include "lattice_kernels.hpp"
 #include "lbm_device.hpp"
+#include "lbm_tb.hpp"
 
 namespace cs {
 
@@ -215,14 +220,16 @@ __device__ __forceinline__ int xcd_chunk_tile(int b, int n)
 
 // One step of every lattice in a tile list: each block updates one tile
 // (TR rows x 64 columns, one cell per thread) of its lattice with the
-// single-lattice gather/finish.  `s` is the step's index within the call.
-// Each wave's 64 columns start on a 64-column boundary, so its row segment of
-// a plane is whole aligned cache lines.
+// single-lattice gather/finish.  `s` is the step's index within the call,
+// `flip` the number of buffer flips of the call before it (s, unless
+// temporally blocked launches came first).  Each wave's 64 columns start on
+// a 64-column boundary, so its row segment of a plane is whole aligned cache
+// lines.
 template <class St, int EDGE, int MODE, bool SR, int TR>
 __global__ __launch_bounds__(64 * TR) void k_batch_tiles(const BatchLat<typename St::T>* __restrict__ lats,
                                                      const TileRef* __restrict__ tiles, void* __restrict__ arena,
                                                      const uint32_t* __restrict__ nodes,
-                                                     const LatState* __restrict__ state, int s)
+                                                     const LatState* __restrict__ state, int flip, int s)
 {
     using S = typename St::S;
     using T = typename St::T;
@@ -235,7 +242,7 @@ __global__ __launch_bounds__(64 * TR) void k_batch_tiles(const BatchLat<typename
     const int r = ty * TR + (int)(threadIdx.x >> 6);
     if (r >= L.h) return;
     const int c = tx * 64 + (int)(threadIdx.x & 63);
-    const int cur = (state[li].cur + s) & 1;
+    const int cur = (state[li].cur + flip) & 1;
     const uint32_t key = sr_key(L.seed, state[li].steps + (uint64_t)s);
     const S* fin = (const S*)arena + L.off + (int64_t)cur * L.h * L.pitch;
     S* fout = (S*)arena + L.off + (int64_t)(cur ^ 1) * L.h * L.pitch;
@@ -246,6 +253,45 @@ __global__ __launch_bounds__(64 * TR) void k_batch_tiles(const BatchLat<typename
     if (c < L.w)
         finish<St, MODE, SR, uint32_t>(fin, fout, L.pitch, L.plane, L.k, r, c, type, B, key, (uint32_t)(r * L.w + c));
 }
+
+// K steps of every lattice in a region list: each block runs tb_region on one
+// 64 x 64 region of its lattice (region t of its ceil(w / (64-2K)) x
+// ceil(h / (64-2K))).  `flip` is the number of buffer flips of the call
+// before this launch.  The list is grouped by lattice, so regions that share
+// halo cells are neighbours in it and xcd_tile keeps them on one XCD.
+template <class St, int EDGE, int MODE, int K>
+__global__ __launch_bounds__(TB_THREADS) void k_batch_tb(const BatchLat<typename St::T>* __restrict__ lats,
+                                                      const TileRef* __restrict__ regions, void* __restrict__ arena,
+                                                      const uint32_t* __restrict__ nodes,
+                                                      const LatState* __restrict__ state, int flip)
+{
+    using S = typename St::S;
+    using T = typename St::T;
+    constexpr int IN = TB_EDGE - 2 * K;
+    __shared__ T L[9][TB_EDGE][TB_EDGE];
+    const TileRef tr = regions[xcd_tile(blockIdx.x, gridDim.x)];
+    const int li = __builtin_amdgcn_readfirstlane(tr.lat);
+    const int t = __builtin_amdgcn_readfirstlane(tr.tile);
+    const BatchLat<T> D = lats[li];
+    const int tiles_x = (D.w + IN - 1) / IN;
+    const int ty = t / tiles_x;
+    const int cur = (state[li].cur + flip) & 1;
+    StepArgs<T> a{};
+    a.fin = (const S*)arena + D.off + (int64_t)cur * D.h * D.pitch;
+    a.fout = (S*)arena + D.off + (int64_t)(cur ^ 1) * D.h * D.pitch;
+    a.node = nodes + D.node_off;
+    a.w = D.w;
+    a.h = D.h;
+    a.pitch = D.pitch;
+    a.cpitch = D.cpitch;
+    a.plane = D.plane;
+    a.tiles_x = tiles_x;
+    a.k = D.k;
+    tb_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, t - ty * tiles_x, L);
+}
+
+// Steps per launch of the temporal launch kind.
+constexpr int BATCH_TB_STEPS = 8;
 
 // Mode group of a lattice: periodic, mode (fluid/ref/half), stochastic rounding.
 inline int mode_group(int mode_bits)
@@ -279,6 +325,8 @@ struct cs_batch {
     int* d_queue = nullptr;
     TileRef* d_tiles = nullptr;
     int64_t max_tiles = 0;
+    TileRef* d_regions = nullptr;     // region list of the temporal launch kind
+    int64_t max_regions = 0;
     LatState* d_state = nullptr;
     bool lats_dirty = true;
     std::vector<LatState> state;      // per lattice: current buffer, steps taken
@@ -288,6 +336,7 @@ struct cs_batch {
     // call's upload.
     std::vector<int> host_order;
     std::vector<TileRef> host_tiles;
+    std::vector<TileRef> host_regions;
     std::vector<unsigned char> host_lats;  // BatchLat<T> descriptors
     std::vector<LatState> host_state;     // `state` as of the launch
     cs_batch_tuning tuning{256, 2, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_PERSISTENT};
@@ -438,11 +487,11 @@ static void upload_state(cs_batch* b)
                           b->stream));
 }
 
-static void advance_state(cs_batch* b, int steps)
+static void advance_state(cs_batch* b, int steps, int flips)
 {
     for (int i = 0; i < b->n; ++i) {
         if (!b->active[(size_t)i]) continue;
-        b->state[(size_t)i].cur = (b->state[(size_t)i].cur + steps) & 1;
+        b->state[(size_t)i].cur = (b->state[(size_t)i].cur + flips) & 1;
         b->state[(size_t)i].steps += (uint64_t)steps;
     }
 }
@@ -479,7 +528,7 @@ static std::vector<std::array<int, 3>> upload_tiles(cs_batch* b, const std::vect
 }
 
 template <class St>
-static void launch_tiles_group(cs_batch* b, int group, int first, int count, int s)
+static void launch_tiles_group(cs_batch* b, int group, int first, int count, int flip, int s)
 {
     using T = typename St::T;
     const auto* lats = (const BatchLat<T>*)b->d_lats;
@@ -488,13 +537,13 @@ static void launch_tiles_group(cs_batch* b, int group, int first, int count, int
     auto go = [&]<int EDGE, int MODE, bool SR>() {
         if (tr == 16)
             hipLaunchKernelGGL((k_batch_tiles<St, EDGE, MODE, SR, 16>), dim3((unsigned)count), dim3(1024), 0, b->stream,
-                               lats, tiles, b->arena, b->nodes, b->d_state, s);
+                               lats, tiles, b->arena, b->nodes, b->d_state, flip, s);
         else if (tr == 8)
             hipLaunchKernelGGL((k_batch_tiles<St, EDGE, MODE, SR, 8>), dim3((unsigned)count), dim3(512), 0, b->stream,
-                               lats, tiles, b->arena, b->nodes, b->d_state, s);
+                               lats, tiles, b->arena, b->nodes, b->d_state, flip, s);
         else
             hipLaunchKernelGGL((k_batch_tiles<St, EDGE, MODE, SR, 4>), dim3((unsigned)count), dim3(256), 0, b->stream,
-                               lats, tiles, b->arena, b->nodes, b->d_state, s);
+                               lats, tiles, b->arena, b->nodes, b->d_state, flip, s);
     };
     auto go_sr = [&]<int EDGE, int MODE>() {
         if constexpr (is_16bit<St>()) {
@@ -515,20 +564,87 @@ static void launch_tiles_group(cs_batch* b, int group, int first, int count, int
     else go_mode.template operator()<CS_EDGE_ZERO>();
 }
 
+// Region list (64 x 64 regions, interior 64 - 2 * BATCH_TB_STEPS) of the
+// active lattices, grouped like upload_order, into d_regions; returns
+// (group, first region, region count) triples.
+static std::vector<std::array<int, 3>> upload_regions(cs_batch* b, const std::vector<std::array<int, 3>>& groups)
+{
+    constexpr int IN = TB_EDGE - 2 * BATCH_TB_STEPS;
+    std::vector<TileRef>& regions = b->host_regions;
+    regions.clear();
+    std::vector<std::array<int, 3>> out;
+    for (const auto& g : groups) {
+        const int first = (int)regions.size();
+        for (int k = g[1]; k < g[1] + g[2]; ++k) {
+            const int li = b->host_order[(size_t)k];
+            const int nr = ((b->w[(size_t)li] + IN - 1) / IN) * ((b->h[(size_t)li] + IN - 1) / IN);
+            for (int t = 0; t < nr; ++t) regions.push_back(TileRef{li, t});
+        }
+        out.push_back({g[0], first, (int)regions.size() - first});
+    }
+    CS_REQUIRE((int64_t)regions.size() <= b->max_regions, CS_ERR_STATE, "region list larger than allocated");
+    if (!regions.empty())
+        CS_HIP(hipMemcpyAsync(b->d_regions, regions.data(), sizeof(TileRef) * regions.size(), hipMemcpyHostToDevice,
+                              b->stream));
+    return out;
+}
+
+template <class St>
+static void launch_tb_group(cs_batch* b, int group, int first, int count, int flip)
+{
+    if constexpr (sizeof(typename St::S) != 4) {
+        throw Error{CS_ERR_UNSUPPORTED, "the temporal launch kind needs 32-bit storage"};
+    } else {
+        using T = typename St::T;
+        const auto* lats = (const BatchLat<T>*)b->d_lats;
+        const TileRef* regions = b->d_regions + first;
+        auto go = [&]<int EDGE, int MODE>() {
+            hipLaunchKernelGGL((k_batch_tb<St, EDGE, MODE, BATCH_TB_STEPS>), dim3((unsigned)count), dim3(TB_THREADS), 0,
+                               b->stream, lats, regions, b->arena, b->nodes, b->d_state, flip);
+        };
+        auto go_mode = [&]<int EDGE>() {
+            const int m = (group % 6) / 2;
+            if (m == MODE_FLUID) go.template operator()<EDGE, MODE_FLUID>();
+            else if (m == MODE_HALF) go.template operator()<EDGE, MODE_HALF>();
+            else go.template operator()<EDGE, MODE_REF>();
+        };
+        if (group >= 6) go_mode.template operator()<CS_EDGE_PERIODIC>();
+        else go_mode.template operator()<CS_EDGE_ZERO>();
+    }
+}
+
+static bool batch_tb_storage(int storage) { return storage == CS_F32 || storage == CS_F32DEV; }
+
 static void launch_batch(cs_batch* b, int steps)
 {
     upload_descriptors(b);
     upload_state(b);
     const auto groups = upload_order(b);
     CS_REQUIRE(groups.size() <= 16, CS_ERR_STATE, "too many boundary-mode groups");
-    if (b->tuning.launch == CS_LAUNCH_TILES) {
-        // one launch per step and mode group over the flattened tile list
-        const auto tg = upload_tiles(b, groups);
-        for (int s = 0; s < steps; ++s)
-            for (const auto& g : tg)
-                if (g[2] > 0)
-                    dispatch_batch_storage(b->storage,
-                                           [&]<class St>() { launch_tiles_group<St>(b, g[0], g[1], g[2], s); });
+    int flips = steps;
+    if (b->tuning.launch == CS_LAUNCH_TILES || b->tuning.launch == CS_LAUNCH_TEMPORAL) {
+        int s = 0;
+        flips = 0;
+        if (b->tuning.launch == CS_LAUNCH_TEMPORAL && steps >= BATCH_TB_STEPS) {
+            CS_REQUIRE(batch_tb_storage(b->storage), CS_ERR_UNSUPPORTED,
+                       "the temporal launch kind needs FP32 or FP32-deviation storage");
+            // one launch per BATCH_TB_STEPS steps and mode group over the region list
+            const auto rg = upload_regions(b, groups);
+            for (; steps - s >= BATCH_TB_STEPS; s += BATCH_TB_STEPS, ++flips)
+                for (const auto& g : rg)
+                    if (g[2] > 0)
+                        dispatch_batch_storage(b->storage,
+                                               [&]<class St>() { launch_tb_group<St>(b, g[0], g[1], g[2], flips); });
+        }
+        if (s < steps) {
+            // one launch per step and mode group over the flattened tile list
+            const auto tg = upload_tiles(b, groups);
+            for (; s < steps; ++s, ++flips)
+                for (const auto& g : tg)
+                    if (g[2] > 0)
+                        dispatch_batch_storage(
+                            b->storage, [&]<class St>() { launch_tiles_group<St>(b, g[0], g[1], g[2], flips, s); });
+        }
     } else {
         CS_HIP(hipMemsetAsync(b->d_queue, 0, sizeof(int) * 16, b->stream));
         for (size_t gi = 0; gi < groups.size(); ++gi) {
@@ -541,7 +657,7 @@ static void launch_batch(cs_batch* b, int steps)
         }
     }
     CS_HIP(hipGetLastError());
-    advance_state(b, steps);
+    advance_state(b, steps, flips);
 }
 
 // One k_step launch per lattice per step (single-lattice code path).
@@ -598,7 +714,7 @@ static void launch_per_lattice(cs_batch* b, int steps)
         }
     });
     CS_HIP(hipGetLastError());
-    advance_state(b, steps);
+    advance_state(b, steps, steps);
 }
 
 static void* lattice_buf(cs_batch* b, int i)
@@ -659,7 +775,11 @@ extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const i
         }
         b->arena_elems = acc;
         b->node_elems = nacc;
-        for (int i = 0; i < count; ++i) b->max_tiles += (int64_t)((w[i] + 63) / 64) * ((h[i] + 3) / 4);
+        for (int i = 0; i < count; ++i) {
+            constexpr int IN = TB_EDGE - 2 * BATCH_TB_STEPS;
+            b->max_tiles += (int64_t)((w[i] + 63) / 64) * ((h[i] + 3) / 4);
+            b->max_regions += (int64_t)((w[i] + IN - 1) / IN) * ((h[i] + IN - 1) / IN);
+        }
         b->state.assign((size_t)count, LatState{0, 0, 0});
         cs_params p{};
         p.tau = 1.0;
@@ -682,6 +802,7 @@ extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const i
             CS_HIP(hipMalloc(&b->d_order, sizeof(int) * (size_t)count));
             CS_HIP(hipMalloc(&b->d_queue, sizeof(int) * 16));
             CS_HIP(hipMalloc(&b->d_tiles, sizeof(TileRef) * (size_t)b->max_tiles));
+            CS_HIP(hipMalloc(&b->d_regions, sizeof(TileRef) * (size_t)b->max_regions));
             CS_HIP(hipMalloc(&b->d_state, sizeof(LatState) * (size_t)count));
             CS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
             CS_HIP(hipEventCreate(&b->ev0));
@@ -700,7 +821,7 @@ extern "C" int cs_batch_destroy(cs_batch* b)
         if (!b) return;
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         for (void* p : {b->arena, (void*)b->nodes, b->d_lats, (void*)b->d_order, (void*)b->d_queue, (void*)b->snap,
-                        (void*)b->d_res, (void*)b->d_tiles, (void*)b->d_state})
+                        (void*)b->d_res, (void*)b->d_tiles, (void*)b->d_regions, (void*)b->d_state})
             if (p) (void)hipFree(p);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -904,7 +1025,10 @@ extern "C" int cs_batch_set_tuning(cs_batch* b, const cs_batch_tuning* t)
                    "workgroups_per_cu must be 1..16");
         CS_REQUIRE(t->cells_per_thread == 1 || t->cells_per_thread == 2, CS_ERR_ARG, "cells_per_thread must be 1 or 2");
         CS_REQUIRE(t->order >= CS_ORDER_LARGEST_FIRST && t->order <= CS_ORDER_SMALLEST_FIRST, CS_ERR_ARG, "bad order");
-        CS_REQUIRE(t->launch == CS_LAUNCH_PERSISTENT || t->launch == CS_LAUNCH_TILES, CS_ERR_ARG, "bad launch kind");
+        CS_REQUIRE(t->launch == CS_LAUNCH_PERSISTENT || t->launch == CS_LAUNCH_TILES ||
+                       t->launch == CS_LAUNCH_TEMPORAL, CS_ERR_ARG, "bad launch kind");
+        CS_REQUIRE(t->launch != CS_LAUNCH_TEMPORAL || batch_tb_storage(b->storage), CS_ERR_UNSUPPORTED,
+                   "the temporal launch kind needs FP32 or FP32-deviation storage");
         b->tuning = *t;
     });
 }
@@ -971,6 +1095,9 @@ extern "C" int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* 
         try {
             for (int blk : {256, 512, 1024})
                 trial(cs_batch_tuning{blk, 1, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_TILES});
+            if (batch_tb_storage(b->storage))
+                for (int blk : {256, 512, 1024})  // block_threads: the remainder steps' tiles
+                    trial(cs_batch_tuning{blk, 1, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_TEMPORAL});
             for (int blk : {256, 512, 1024})
                 for (int wpc : {1, 2, 4, 8})
                     for (int cpt : {1, 2})

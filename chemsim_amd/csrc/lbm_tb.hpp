@@ -84,32 +84,27 @@ __device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][TB_EDGE][
     }
 }
 
-// Region of tile t: rows ty*(64-2K) - K .., columns tx*(64-2K) - K ..
-// (a.tiles_x = ceil(w / (64-2K))); blocks of TB_THREADS threads, one block
-// per CU (the region takes 144 KiB of LDS).  A wave's LDS accesses are 64
-// consecutive words of one region row (a pull from column -1 or 64 wraps
-// within the row and belongs to a lane that discards its result), so they
-// are free of bank conflicts, and each lane's nine pull addresses per row
+// K steps on the region of tile (ty, tx) of the lattice `a` describes: rows
+// ty*(64-2K) - K .., columns tx*(64-2K) - K ..; blocks of TB_THREADS threads,
+// one block per CU (the region takes 144 KiB of LDS).  A wave's LDS accesses
+// are 64 consecutive words of one region row (a pull from column -1 or 64
+// wraps within the row and belongs to a lane that discards its result), so
+// they are free of bank conflicts, and each lane's nine pull addresses per row
 // slot stay the same from step to step.
 template <class St, int EDGE, int MODE, int K, int COLL>
-__global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T> a)
+__device__ __forceinline__ void tb_region(const StepArgs<typename St::T>& a, int ty, int tx,
+                                          typename St::T (&L)[9][TB_EDGE][TB_EDGE])
 {
     using S = typename St::S;
     using T = typename St::T;
     static_assert(sizeof(S) == 4 && sizeof(T) == 4, "32-bit storage and arithmetic");
     static_assert(K >= 1 && 2 * K < TB_EDGE, "halo");
     constexpr int IN = TB_EDGE - 2 * K;  // interior side
-    __shared__ T L[9][TB_EDGE][TB_EDGE];
-
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int ty = tile / a.tiles_x;
-    const int tx = tile - ty * a.tiles_x;
     const int r0 = ty * IN - K, c0 = tx * IN - K;
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
     const S* __restrict__ fin = (const S*)a.fin;
     S* __restrict__ fout = (S*)a.fout;
     const uint32_t P = (uint32_t)a.plane;
-
     // this lane's lattice column, and whether it lies inside the lattice
     int gc = c0 + lane;
     bool cin = true;
@@ -185,6 +180,18 @@ __global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T>
             for (int i = 0; i < 9; ++i) St::st(at<uint32_t>(fout, (uint32_t)i * P + o), 0, L[i][rr][lane]);
         }
     }
+}
+
+
+// One lattice: tile t of the lattice's ceil(w / (64-2K)) x ceil(h / (64-2K))
+// regions (a.tiles_x = ceil(w / (64-2K))).
+template <class St, int EDGE, int MODE, int K, int COLL>
+__global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T> a)
+{
+    __shared__ typename St::T L[9][TB_EDGE][TB_EDGE];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = tile / a.tiles_x;
+    tb_region<St, EDGE, MODE, K, COLL>(a, ty, tile - ty * a.tiles_x, L);
 }
 
 }  // namespace cs

@@ -124,6 +124,7 @@ pub const CS_ORDER_GIVEN: i32 = 1;
 pub const CS_ORDER_SMALLEST_FIRST: i32 = 2;
 pub const CS_LAUNCH_PERSISTENT: i32 = 0;
 pub const CS_LAUNCH_TILES: i32 = 1;
+pub const CS_LAUNCH_TEMPORAL: i32 = 2;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default, PartialEq)]

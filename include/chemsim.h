@@ -277,8 +277,12 @@ enum cs_batch_order { CS_ORDER_LARGEST_FIRST = 0, CS_ORDER_GIVEN = 1, CS_ORDER_S
 /* CS_LAUNCH_PERSISTENT: workgroups take whole lattices from a queue and run
  * all steps of a call on them (block_threads, workgroups_per_cu,
  * cells_per_thread, order apply).  CS_LAUNCH_TILES: one launch per step over
- * the (block_threads / 64) x 64 tiles of every lattice. */
-enum cs_batch_launch { CS_LAUNCH_PERSISTENT = 0, CS_LAUNCH_TILES = 1 };
+ * the (block_threads / 64) x 64 tiles of every lattice.  CS_LAUNCH_TEMPORAL
+ * (FP32 and FP32-deviation storage): one launch per 8 steps over the 64 x 64
+ * regions of every lattice, each run 8 steps out of LDS (the temporally
+ * blocked kernel); the remaining steps of a call (steps mod 8) as tile
+ * launches (block_threads applies). */
+enum cs_batch_launch { CS_LAUNCH_PERSISTENT = 0, CS_LAUNCH_TILES = 1, CS_LAUNCH_TEMPORAL = 2 };
 typedef struct cs_batch_tuning {
     int32_t block_threads;     /* 256, 512 or 1024 */
     int32_t workgroups_per_cu; /* persistent grid = CUs x this (capped at the lattice count) */

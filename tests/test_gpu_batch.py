@@ -51,6 +51,32 @@ def build_mixed(ctx, storage="f32"):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f32dev"])
+def test_batch_temporal_matches_oracle(ctx, storage):
+    """The temporal launch kind (8 steps per launch out of LDS, then tile
+    launches for the remainder) over every boundary mode."""
+    b, specs = build_mixed(ctx, storage)
+    with b:
+        b.tuning = dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0,
+                        launch=capi.CS_LAUNCH_TEMPORAL)
+        b.step(19)
+        b.step(8)
+        for i, (w, h, op, flags, init) in enumerate(specs):
+            if storage == "f32":
+                ref = O.run(op, O.equilibrium(op, *init), 27, flags)
+            else:
+                ref = O.from_dev(op, O.run_devf32(op, O.equilibrium_dev(op, *init), 27, flags))
+            np.testing.assert_array_equal(b.download(i), ref, err_msg=f"lattice {i} ({w}x{h})")
+
+
+def test_batch_temporal_needs_32bit_storage(ctx):
+    b, _ = build_mixed(ctx, "f16")
+    with b:
+        with pytest.raises(RuntimeError, match="32-bit|FP32"):
+            b.tuning = dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0,
+                            launch=capi.CS_LAUNCH_TEMPORAL)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32dev"])
 def test_batch_matches_oracle(ctx, storage):
     b, specs = build_mixed(ctx, storage)
     with b:
@@ -72,13 +98,18 @@ def test_batch_equals_per_lattice_launches(ctx, storage):
                dict(block_threads=512, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=capi.CS_LAUNCH_TILES),
                dict(block_threads=1024, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=capi.CS_LAUNCH_TILES),
                dict(block_threads=512, workgroups_per_cu=8, cells_per_thread=2, order=2)]
+    steps = [3] * len(tunings)
+    if storage == "f32":  # 8-step launches plus a remainder, then the others continue
+        tunings.insert(3, dict(block_threads=512, workgroups_per_cu=1, cells_per_thread=1, order=0,
+                               launch=capi.CS_LAUNCH_TEMPORAL))
+        steps.insert(3, 11)
     a, _ = build_mixed(ctx, storage)
     p, _ = build_mixed(ctx, storage)
     with a, p:
-        p.step_per_lattice(3 * len(tunings))
-        for t in tunings:
+        p.step_per_lattice(sum(steps))
+        for t, n in zip(tunings, steps):
             a.tuning = t
-            a.step(3)
+            a.step(n)
         for i in range(len(a)):
             np.testing.assert_array_equal(a.download_raw(i), p.download_raw(i), err_msg=f"lattice {i}")
 
@@ -153,7 +184,7 @@ def test_inactive_lattices_keep_their_state(ctx):
 
 
 @pytest.mark.parametrize("launch,threads", [(capi.CS_LAUNCH_TILES, 256), (capi.CS_LAUNCH_TILES, 1024),
-                                            (capi.CS_LAUNCH_PERSISTENT, 256)])
+                                            (capi.CS_LAUNCH_PERSISTENT, 256), (capi.CS_LAUNCH_TEMPORAL, 256)])
 def test_batch_tiny_lattices_match_oracle(ctx, launch, threads):
     """1x1, one-cell-wide and small ragged lattices beside a normal one, in
     both launch kinds: every lattice equals the oracle's run of it alone."""
@@ -174,6 +205,6 @@ def test_batch_tiny_lattices_match_oracle(ctx, launch, threads):
             uy = (0.02 * rng.standard_normal((h, w))).astype(np.float32)
             b.init_equilibrium(i, rho, ux, uy)
             specs.append((op, flags, O.equilibrium(op, rho, ux, uy)))
-        b.step(5)
+        b.step(13)
         for i, (op, flags, f0) in enumerate(specs):
-            np.testing.assert_array_equal(b.download(i), O.run(op, f0, 5, flags), err_msg=f"lattice {i} {shapes[i]}")
+            np.testing.assert_array_equal(b.download(i), O.run(op, f0, 13, flags), err_msg=f"lattice {i} {shapes[i]}")

@@ -86,11 +86,12 @@ def test_512_cavity_batch_full_size(ctx):
     cells = sum(s.side * s.side for s in specs)
     print(f"[fullsize] 512 cavities, {cells} cells", flush=True)
     assert cells > 50_000_000
-    steps = 3
+    steps = 11  # the temporal kind: one 8-step launch, then 3 tile launches
     persistent = dict(block_threads=256, workgroups_per_cu=2, cells_per_thread=1, order=capi.CS_ORDER_LARGEST_FIRST,
                       launch=capi.CS_LAUNCH_PERSISTENT)
     tiles = dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=capi.CS_ORDER_LARGEST_FIRST,
                  launch=capi.CS_LAUNCH_TILES)
+    temporal = dict(tiles, launch=capi.CS_LAUNCH_TEMPORAL)
     with cavity_batch(ctx, specs, persistent) as a, cavity_batch(ctx, specs, tiles) as b:
         a.step(steps)
         b.step(steps)
@@ -100,6 +101,10 @@ def test_512_cavity_batch_full_size(ctx):
                 ra = a.download_raw(i)
                 np.testing.assert_array_equal(ra, b.download_raw(i), err_msg=f"tiles, lattice {i}")
                 np.testing.assert_array_equal(ra, c.download_raw(i), err_msg=f"per-lattice, lattice {i}")
+        with cavity_batch(ctx, specs, temporal) as t:
+            t.step(steps)
+            for i in range(len(specs)):
+                np.testing.assert_array_equal(a.download_raw(i), t.download_raw(i), err_msg=f"temporal, lattice {i}")
         order = np.argsort([s.side for s in specs], kind="stable")
         for i in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
             s = specs[i]
