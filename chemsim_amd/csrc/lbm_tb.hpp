@@ -48,10 +48,13 @@ __device__ __forceinline__ void collide_cell(const Consts<typename St::T>& k, ty
     else bgk_collide(k, B);
 }
 
-// Pulls of region cell (rr, lane) for the step, in gather()'s order: the
-// normal source of each direction, on solid cells of the reference mode the
-// opposite direction's source (lbm.rs:745), in the half-way mode the cell's
-// own opposite population (+ lid term) over a solid link, then Zou-He.
+// Pulls of region cell (rr, lane) for the step, with gather()'s values: the
+// normal source of each direction; on solid cells of the reference mode the
+// opposite direction's normal pull (lbm.rs:745), i.e. the pulled pairs swap;
+// in the half-way mode the cell's own opposite population (+ lid term) over a
+// solid link, then Zou-He.  Half-way links are read only in waves that hold
+// one (most waves of a lattice hold none), so the common case is nine LDS
+// reads per cell.
 template <class St, int MODE>
 __device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][TB_EDGE][TB_EDGE], int rr, int lane,
                                           uint32_t info, const Consts<typename St::T>& k, typename St::T (&B)[9])
@@ -60,20 +63,28 @@ __device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][TB_EDGE][
     const int type = MODE == MODE_FLUID ? (int)CS_FLUID : (int)(info >> NODE_TYPE_SHIFT);
     const bool solid = type == CS_WALL || type == CS_LID;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        T v = L[i][rr + CX[i]][(lane - CY[i]) & (TB_EDGE - 1)];
-        if (MODE == MODE_REF) {
-            const int j = OPP[i];
-            if (solid) v = L[j][rr + CX[j]][(lane - CY[j]) & (TB_EDGE - 1)];
-        } else if (MODE == MODE_HALF) {
-            if (!solid && ((info >> i) & 1u)) {
-                v = L[OPP[i]][rr][lane];
-                if ((info >> (9 + i)) & 1u) v = v + k.lid(i);
+    for (int i = 0; i < 9; ++i) B[i] = L[i][rr + CX[i]][(lane - CY[i]) & (TB_EDGE - 1)];
+    if (MODE == MODE_REF) {
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            if (i < OPP[i]) {
+                const T a = B[i], b = B[OPP[i]];
+                B[i] = solid ? b : a;
+                B[OPP[i]] = solid ? a : b;
             }
         }
-        B[i] = v;
-    }
-    if (MODE == MODE_HALF) {
+    } else if (MODE == MODE_HALF) {
+        const bool link = !solid && (info & 0x1FEu) != 0;
+        if (__any(link)) {  // uniform over the wave
+#pragma unroll
+            for (int i = 1; i < 9; ++i) {
+                if (link && ((info >> i) & 1u)) {
+                    T v = L[OPP[i]][rr][lane];
+                    if ((info >> (9 + i)) & 1u) v = v + k.lid(i);
+                    B[i] = v;
+                }
+            }
+        }
         if constexpr (St::DEV) {
             if (type == CS_INLET) zou_he_inlet_dev<T>(B, k.rho0, k.in_ux, k.in_uy);
             else if (type == CS_OUTLET) zou_he_outlet_dev<T>(B, k.rho0, k.out_rho);

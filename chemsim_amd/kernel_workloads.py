@@ -16,15 +16,19 @@ Workloads:
   dev16_8192  Taylor-Green 8192^2, BF16 and FP16 deviations with stochastic
               rounding; 36 B/cell
   cavity4096  lid-driven cavity 4096^2 FP32, half-way walls, lid 0.1, from
-              rest: the geometry kernel (node word per cell); 72 B/cell of
-              populations + 4 B/cell of node words
+              rest, with geometry (node word per cell): the temporally
+              blocked kernel (8 steps per launch; regions read once, cells
+              written once per launch + 4 B/cell of node words) and the
+              one-step geometry kernel (72 B/cell + 4 B/cell of node words)
   cavity128   lid-driven cavity 128^2 FP32 (16,384 cells), launches one by one
               (step graphs off): the launch-bound case
   render      render pass of the 4096^2 Taylor-Green state in the four display
               modes (render.rs:7-178): field pass (36 B/cell read, 4 or 8 B/cell
               written), statistics, pixel pass (4 or 8 B/cell read, 3 B/pixel)
   batch512    512 lid-driven cavities, sides U{64..512} (seed 0), tile-list
-              and persistent launches; 72 B per cell update
+              and persistent launches (72 B per cell update) and the temporal
+              launch kind (8 steps per launch; every 64 x 64 region read and
+              every cell written once per launch)
 """
 from __future__ import annotations
 
@@ -62,7 +66,7 @@ def _tgv(ctx, n, storage, rounding=capi.CS_ROUND_STOCHASTIC, **kw):
 
 
 def _timed(lat, steps):
-    lat.step(2)
+    lat.step(max(2, lat.temporal_steps()))  # warm-up: every kernel of the timed steps launched once
     ms = lat.step_timed(steps)
     return ms / steps * 1e3
 
@@ -108,13 +112,34 @@ def _cavity(ctx, n):
     return lat
 
 
+def tb_bytes_per_launch(w: int, h: int, k: int = 8) -> int:
+    """Bytes one launch of the temporally blocked kernel (k steps) must move
+    for a w x h FP32 lattice: every 64 x 64 region read once, every cell
+    written once, 36 B per cell."""
+    inner = 64 - 2 * k
+    return (-(-w // inner) * -(-h // inner) * 64 * 64 + w * h) * 36
+
+
 def cavity4096(ctx, steps):
     cells = N4 * N4
-    _calibrate(ctx)
-    with _cavity(ctx, N4) as lat:
-        us = _timed(lat, steps)
-    _emit({"workload": "cavity4096", "kernel": "k_step", "cells": cells, "alg_bytes_per_launch": 72 * cells,
-           "node_bytes_per_launch": 4 * cells, "event_us_per_step": round(us, 2)})
+    steps8 = max(8, steps // 8 * 8)
+    for k in (capi.CS_TEMPORAL_AUTO, 0):
+        capi.call("cs_set_temporal_steps", k)
+        try:
+            _calibrate(ctx)
+            with _cavity(ctx, N4) as lat:
+                tb = lat.temporal_steps()
+                us = _timed(lat, steps8)
+        finally:
+            capi.call("cs_set_temporal_steps", capi.CS_TEMPORAL_AUTO)
+        if tb > 1:
+            _emit({"workload": "cavity4096/temporal", "kernel": "k_step_tb", "cells": cells, "steps_per_launch": tb,
+                   "alg_bytes_per_launch": tb_bytes_per_launch(N4, N4, tb), "node_bytes_per_launch": 4 * cells,
+                   "event_us_per_step": round(us, 2)})
+        else:
+            _emit({"workload": "cavity4096/one_step", "kernel": "k_step_vec", "cells": cells,
+                   "alg_bytes_per_launch": 72 * cells, "node_bytes_per_launch": 4 * cells,
+                   "event_us_per_step": round(us, 2)})
 
 
 def cavity128(ctx, steps):
@@ -156,16 +181,22 @@ def batch512(ctx, steps):
             b.set_flags(i, W.cavity_flags(s.side, s.side))
             b.init_equilibrium(i, *W.rest_state(s.side, s.side))
         cells = b.cells
+        tb_bytes = sum(tb_bytes_per_launch(s.side, s.side) for s in specs)
         for kind, tuning in (("tiles", dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0,
                                             launch=capi.CS_LAUNCH_TILES)),
                              ("persistent", dict(block_threads=1024, workgroups_per_cu=2, cells_per_thread=1,
-                                                 order=0, launch=capi.CS_LAUNCH_PERSISTENT))):
+                                                 order=0, launch=capi.CS_LAUNCH_PERSISTENT)),
+                             ("temporal", dict(block_threads=512, workgroups_per_cu=1, cells_per_thread=1, order=0,
+                                               launch=capi.CS_LAUNCH_TEMPORAL))):
             b.tuning = tuning
-            b.step(2)
-            ms = b.step(steps)
-            _emit({"workload": f"batch512/{kind}", "kernel": "k_batch_tiles" if kind == "tiles" else "k_batch",
-                   "lattices": len(b), "cells": cells, "alg_bytes_per_launch": 72 * cells,
-                   "event_us_per_step": round(ms / steps * 1e3, 2)})
+            n = max(8, steps // 8 * 8) if kind == "temporal" else steps
+            b.step(8 if kind == "temporal" else 2)
+            ms = b.step(n)
+            kernel = {"tiles": "k_batch_tiles", "persistent": "k_batch", "temporal": "k_batch_tb"}[kind]
+            _emit({"workload": f"batch512/{kind}", "kernel": kernel, "lattices": len(b), "cells": cells,
+                   "alg_bytes_per_launch": tb_bytes if kind == "temporal" else 72 * cells,
+                   "steps_per_launch": 8 if kind == "temporal" else 1,
+                   "event_us_per_step": round(ms / n * 1e3, 2)})
 
 
 WORKLOADS = {"collisions": collisions, "f64": f64, "dev16_8192": dev16_8192, "cavity4096": cavity4096,
