@@ -511,7 +511,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "bytes_per_cell": round(bpc, 3), "kernel_us": round(kernel_ms * 1e3, 2),
-                     "kernel": f"k_step_tb ({k_launch} steps per launch)" if temporal else "k_step / k_step_vec",
+                     "kernel": f"temporally blocked, {k_launch} steps per launch" if temporal else "k_step / k_step_vec",
                      "per": "launch" if temporal else "step"},
         "cpu_baseline": cpu,
         "extras": extras,

@@ -260,23 +260,22 @@ __global__ __launch_bounds__(64 * TR) void k_batch_tiles(const BatchLat<typename
 // before this launch.  The list is grouped by lattice, so regions that share
 // halo cells are neighbours in it and xcd_tile keeps them on one XCD.
 template <class St, int EDGE, int MODE, int K>
-__global__ __launch_bounds__(TB_THREADS) void k_batch_tb(const BatchLat<typename St::T>* __restrict__ lats,
-                                                      const TileRef* __restrict__ regions, void* __restrict__ arena,
-                                                      const uint32_t* __restrict__ nodes,
-                                                      const LatState* __restrict__ state, int flip)
+__device__ __forceinline__ void batch_tb_region(const BatchLat<typename St::T>* __restrict__ lats,
+                                                const TileRef* __restrict__ regions, void* __restrict__ arena,
+                                                const uint32_t* __restrict__ nodes,
+                                                const LatState* __restrict__ state, int flip,
+                                                StepArgs<typename St::T>& a, int& ty, int& tx)
 {
     using S = typename St::S;
     using T = typename St::T;
     constexpr int IN = TB_EDGE - 2 * K;
-    __shared__ T L[9][TB_EDGE][TB_EDGE];
     const TileRef tr = regions[xcd_tile(blockIdx.x, gridDim.x)];
     const int li = __builtin_amdgcn_readfirstlane(tr.lat);
     const int t = __builtin_amdgcn_readfirstlane(tr.tile);
     const BatchLat<T> D = lats[li];
     const int tiles_x = (D.w + IN - 1) / IN;
-    const int ty = t / tiles_x;
+    ty = t / tiles_x;
     const int cur = (state[li].cur + flip) & 1;
-    StepArgs<T> a{};
     a.fin = (const S*)arena + D.off + (int64_t)cur * D.h * D.pitch;
     a.fout = (S*)arena + D.off + (int64_t)(cur ^ 1) * D.h * D.pitch;
     a.node = nodes + D.node_off;
@@ -287,7 +286,36 @@ __global__ __launch_bounds__(TB_THREADS) void k_batch_tb(const BatchLat<typename
     a.plane = D.plane;
     a.tiles_x = tiles_x;
     a.k = D.k;
-    tb_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, t - ty * tiles_x, L);
+    tx = t - ty * tiles_x;
+}
+
+// The half-way mode runs the region in LDS (tb_region), the others in
+// registers (tbr_region), as single lattices do.
+template <class St, int EDGE, int MODE, int K>
+__global__ __launch_bounds__(TB_THREADS) void k_batch_tb(const BatchLat<typename St::T>* __restrict__ lats,
+                                                      const TileRef* __restrict__ regions, void* __restrict__ arena,
+                                                      const uint32_t* __restrict__ nodes,
+                                                      const LatState* __restrict__ state, int flip)
+{
+    __shared__ typename St::T L[9][TB_EDGE][TB_EDGE];
+    StepArgs<typename St::T> a{};
+    int ty, tx;
+    batch_tb_region<St, EDGE, MODE, K>(lats, regions, arena, nodes, state, flip, a, ty, tx);
+    tb_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, L);
+}
+
+template <class St, int EDGE, int MODE, int K>
+__global__ __launch_bounds__(TBR_THREADS, 4) void k_batch_tbr(const BatchLat<typename St::T>* __restrict__ lats,
+                                                           const TileRef* __restrict__ regions,
+                                                           void* __restrict__ arena,
+                                                           const uint32_t* __restrict__ nodes,
+                                                           const LatState* __restrict__ state, int flip)
+{
+    __shared__ typename St::T X[2][TBR_WAVES][2][3][TB_EDGE];
+    StepArgs<typename St::T> a{};
+    int ty, tx;
+    batch_tb_region<St, EDGE, MODE, K>(lats, regions, arena, nodes, state, flip, a, ty, tx);
+    tbr_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, X);
 }
 
 // Steps per launch of the temporal launch kind.
@@ -599,8 +627,12 @@ static void launch_tb_group(cs_batch* b, int group, int first, int count, int fl
         const auto* lats = (const BatchLat<T>*)b->d_lats;
         const TileRef* regions = b->d_regions + first;
         auto go = [&]<int EDGE, int MODE>() {
-            hipLaunchKernelGGL((k_batch_tb<St, EDGE, MODE, BATCH_TB_STEPS>), dim3((unsigned)count), dim3(TB_THREADS), 0,
-                               b->stream, lats, regions, b->arena, b->nodes, b->d_state, flip);
+            if constexpr (MODE == MODE_HALF)
+                hipLaunchKernelGGL((k_batch_tb<St, EDGE, MODE, BATCH_TB_STEPS>), dim3((unsigned)count),
+                                   dim3(TB_THREADS), 0, b->stream, lats, regions, b->arena, b->nodes, b->d_state, flip);
+            else
+                hipLaunchKernelGGL((k_batch_tbr<St, EDGE, MODE, BATCH_TB_STEPS>), dim3((unsigned)count),
+                                   dim3(TBR_THREADS), 0, b->stream, lats, regions, b->arena, b->nodes, b->d_state, flip);
         };
         auto go_mode = [&]<int EDGE>() {
             const int m = (group % 6) / 2;

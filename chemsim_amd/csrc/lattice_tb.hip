@@ -3,19 +3,24 @@
 // template dispatch over storage, edge, boundary mode and collision operator.
 // Kept apart from lattice.hip so its instantiations compile in parallel.
 //
-// FP32 and FP32-deviation lattices take k_step_tb with K = 8 steps per launch
-// (4096^2 FP32 Taylor-Green, microseconds per step: one-step vector kernel
-// 203; K = 2 197, K = 4 137, K = 6 125, K = 8 116 -- beyond 8 the halo's
-// recomputation outgrows the saved traffic), with or without geometry, when
-// the lattice has at least TB_MIN_REGIONS regions: a block updates one 64 x 64
-// region at one CU's VALU rate (about 3.9 us per step), so a lattice of fewer
-// regions leaves CUs idle and the one-step kernels, spread over every CU and
-// replayed from HIP graphs, are as fast or faster (512^2, 121 regions: 3.9
-// against 3.9 us per step; 128^2 cavity, 9 regions: 5.0 against 3.1).  KBC,
-// whose one-step kernel is already VALU-bound (18 divisions per cell), keeps
-// it: 4096^2, 342 against 267 us per step.  cs_set_temporal_steps(4 or 8)
-// forces the kernel on every FP32 lattice, (0) switches it off; a launch-shape
-// override from cs_set_cells_per_thread selects the one-step kernels.
+// FP32 and FP32-deviation lattices take the temporally blocked kernels with
+// K = 8 steps per launch (4096^2 FP32 Taylor-Green with the region in LDS,
+// microseconds per step: one-step vector kernel 203; K = 2 197, K = 4 137,
+// K = 6 125, K = 8 116 -- beyond 8 the halo's recomputation outgrows the
+// saved traffic): k_step_tbr (the region in registers, 88 us per step)
+// without geometry and in the reference boundary mode, k_step_tb (the region
+// in LDS) in the half-way mode, whose link and Zou-He code does not fit the
+// register variant's budget (it spills: 297 against 130 us per step on a
+// 4096^2 cavity).  They are taken when the lattice has at least
+// TB_MIN_REGIONS regions: a block updates one 64 x 64 region at one CU's
+// VALU rate (about 3-4 us per step), so a lattice of fewer regions leaves CUs
+// idle and the one-step kernels, spread over every CU and replayed from HIP
+// graphs, are as fast or faster (512^2, 121 regions: 3.9 against 3.9 us per
+// step; 128^2 cavity, 9 regions: 5.0 against 3.1).  KBC, whose one-step
+// kernel is already VALU-bound (18 divisions per cell), keeps it: 4096^2, 342
+// against 267 us per step.  cs_set_temporal_steps(4 or 8) forces the kernels
+// on every FP32 lattice, (0) switches them off; a launch-shape override from
+// cs_set_cells_per_thread selects the one-step kernels.
 #include <atomic>
 #include <type_traits>
 
@@ -51,8 +56,12 @@ void launch_k(cs_lattice* l)
     a.tiles_x = (l->w + IN - 1) / IN;
     a.k = consts32<St>(l);
     const int tiles_y = (l->h + IN - 1) / IN;
-    hipLaunchKernelGGL((k_step_tb<St, EDGE, MODE, K, COLL>), dim3((unsigned)(a.tiles_x * tiles_y)),
-                       dim3(TB_THREADS), 0, l->stream, a);
+    if constexpr (MODE == MODE_HALF)
+        hipLaunchKernelGGL((k_step_tb<St, EDGE, MODE, K, COLL>), dim3((unsigned)(a.tiles_x * tiles_y)),
+                           dim3(TB_THREADS), 0, l->stream, a);
+    else
+        hipLaunchKernelGGL((k_step_tbr<St, EDGE, MODE, K, COLL>), dim3((unsigned)(a.tiles_x * tiles_y)),
+                           dim3(TBR_THREADS), 0, l->stream, a);
     l->cur ^= 1;
     l->steps += K;
     for (int s = 0; s < K; ++s) l->time += l->params.delta_t;  // as step by step (lbm.rs:713)
