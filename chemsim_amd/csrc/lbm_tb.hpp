@@ -52,18 +52,21 @@ __device__ __forceinline__ void collide_cell(const Consts<typename St::T>& k, ty
 // normal source of each direction; on solid cells of the reference mode the
 // opposite direction's normal pull (lbm.rs:745), i.e. the pulled pairs swap;
 // in the half-way mode the cell's own opposite population (+ lid term) over a
-// solid link, then Zou-He.  Half-way links are read only in waves that hold
-// one (most waves of a lattice hold none), so the common case is nine LDS
-// reads per cell.
+// solid link, then Zou-He.  `special` (uniform over the wave) says whether any
+// cell of the wave's row is solid (reference mode) or has a link or an
+// inlet/outlet flag (half-way mode); rows without one -- most rows of a
+// lattice -- take the plain pulls, nine LDS reads per cell.
 template <class St, int MODE>
 __device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][TB_EDGE][TB_EDGE], int rr, int lane,
-                                          uint32_t info, const Consts<typename St::T>& k, typename St::T (&B)[9])
+                                          uint32_t info, bool special, const Consts<typename St::T>& k,
+                                          typename St::T (&B)[9])
 {
     using T = typename St::T;
     const int type = MODE == MODE_FLUID ? (int)CS_FLUID : (int)(info >> NODE_TYPE_SHIFT);
     const bool solid = type == CS_WALL || type == CS_LID;
 #pragma unroll
     for (int i = 0; i < 9; ++i) B[i] = L[i][rr + CX[i]][(lane - CY[i]) & (TB_EDGE - 1)];
+    if (!special) return;  // no cell of the wave's row needs more than the plain pulls
     if (MODE == MODE_REF) {
 #pragma unroll
         for (int i = 1; i < 9; ++i) {
@@ -127,6 +130,7 @@ __device__ __forceinline__ void tb_region(const StepArgs<typename St::T>& a, int
         gc = cin ? gc : 0;
     }
     bool upd[TB_SLOTS];       // the cell is inside the lattice and not carried
+    bool special[TB_SLOTS];   // the wave's row needs the geometry pulls (uniform over the wave)
     uint32_t info[TB_SLOTS];  // node words (geometry modes)
 #pragma unroll
     for (int j = 0; j < TB_SLOTS; ++j) {
@@ -145,6 +149,9 @@ __device__ __forceinline__ void tb_region(const StepArgs<typename St::T>& a, int
         if (MODE != MODE_FLUID) info[j] = inside ? a.node[(uint32_t)gr * (uint32_t)a.cpitch + (uint32_t)gc] : 0u;
         const int type = (int)(info[j] >> NODE_TYPE_SHIFT);
         upd[j] = inside && !(MODE == MODE_HALF && (type == CS_WALL || type == CS_LID));
+        const bool needs = MODE == MODE_REF ? (type == CS_WALL || type == CS_LID)
+                                            : ((info[j] & 0x1FEu) != 0 || type == CS_INLET || type == CS_OUTLET);
+        special[j] = MODE != MODE_FLUID && __any(needs);
         const uint32_t o = (uint32_t)gr * (uint32_t)a.pitch + (uint32_t)gc;
         T v[9];
 #pragma unroll
@@ -161,7 +168,7 @@ __device__ __forceinline__ void tb_region(const StepArgs<typename St::T>& a, int
         for (int j = 0; j < TB_SLOTS; ++j) {
             const int rr = wave + 16 * j;
             if (rr >= s && rr < TB_EDGE - s) {  // uniform over the wave
-                tb_gather<St, MODE>(L, rr, lane, info[j], a.k, B[j]);
+                tb_gather<St, MODE>(L, rr, lane, info[j], special[j], a.k, B[j]);
                 collide_cell<St, COLL>(a.k, B[j]);
             }
         }
