@@ -77,9 +77,13 @@ __global__ void k_pool(const uint16_t* __restrict__ x, uint16_t* __restrict__ y,
     y[i] = bf16_rne(0.25f * s);
 }
 
-// Pooling backward: dx += dy / 4 on each of the four inputs (accumulates
-// into an f32 gradient buffer of the pooled tensor's input).
-__global__ void k_pool_back(const uint16_t* __restrict__ dy, float* __restrict__ dx, int N, int H, int W, int C)
+// Pooling backward, fused with the next layer's ReLU mask: the gradient of
+// the pooled tensor's input is the skip gradient accumulated so far plus
+// dy / 4 from the pooled output; dz = bf16(that * (y > 0)) (relu = 0: no
+// mask), y the input tensor (the conv output that was pooled).
+__global__ void k_pool_back(const uint16_t* __restrict__ dy, const float* __restrict__ skip,
+                            const uint16_t* __restrict__ y, int relu, uint16_t* __restrict__ dz, int N, int H, int W,
+                            int C)
 {
     const int64_t n_in = (int64_t)N * H * W * C;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,17 +92,23 @@ __global__ void k_pool_back(const uint16_t* __restrict__ dy, float* __restrict__
     int64_t t = i / C;
     const int x = (int)(t % W);
     t /= W;
-    const int y = (int)(t % H);
+    const int yy = (int)(t % H);
     const int64_t n = t / H;
     const int Ho = H / 2, Wo = W / 2;
-    dx[i] += 0.25f * bf16_f32(dy[((n * Ho + y / 2) * Wo + x / 2) * C + c]);
+    float g = skip[i] + 0.25f * bf16_f32(dy[((n * Ho + yy / 2) * Wo + x / 2) * C + c]);
+    if (relu && !(bf16_f32(y[i]) > 0.f)) g = 0.f;
+    dz[i] = bf16_rne(g);
 }
 
 // Backward of a virtual concatenation (up-sampled coarse ++ skip) from the
-// gradient dy of the concatenation: dx (f32, coarse) = sum over the 4
-// children of the first C1 channels; ds (f32) += the last C2 channels.
-__global__ void k_up_concat_back(const uint16_t* __restrict__ dy, float* __restrict__ dx, float* __restrict__ ds,
-                                 int N, int H, int W, int C1, int C2)
+// gradient dy of the concatenation: the coarse tensor's gradient is the sum
+// over its 4 children of the first C1 channels, written as
+// dz = bf16(sum * (y > 0)) with the coarse tensor y (the ReLU output that was
+// up-sampled; relu = 0: no mask); the skip tensor's gradient (f32) is the
+// last C2 channels, the first contribution to it.
+__global__ void k_up_concat_back(const uint16_t* __restrict__ dy, const uint16_t* __restrict__ y, int relu,
+                                 uint16_t* __restrict__ dz, float* __restrict__ ds, int N, int H, int W, int C1,
+                                 int C2)
 {
     const int Ct = C1 + C2;
     const int Hc = H / 2, Wc = W / 2;
@@ -112,35 +122,24 @@ __global__ void k_up_concat_back(const uint16_t* __restrict__ dy, float* __restr
         t /= Wc;
         const int yc = (int)(t % Hc);
         const int64_t n = t / Hc;
-        float s = 0.f;
+        float g = 0.f;
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b)
-                s += bf16_f32(dy[((n * H + 2 * yc + a) * W + 2 * xc + b) * Ct + c]);
-        dx[i] = s;
+                g += bf16_f32(dy[((n * H + 2 * yc + a) * W + 2 * xc + b) * Ct + c]);
+        if (relu && !(bf16_f32(y[i]) > 0.f)) g = 0.f;
+        dz[i] = bf16_rne(g);
     }
     if (i < n_s) {
         const int c = (int)(i % C2);
         const int64_t p = i / C2;
-        ds[i] += bf16_f32(dy[p * Ct + C1 + c]);
+        ds[i] = 0.f + bf16_f32(dy[p * Ct + C1 + c]);
     }
-}
-
-// dz = dy * (y > 0) for a ReLU output y; dy is f32, dz bf16.  Also used with
-// relu = 0 to convert f32 gradients to bf16.
-__global__ void k_relu_back(const float* __restrict__ dy, const uint16_t* __restrict__ y, uint16_t* __restrict__ dz,
-                            int64_t n, int relu)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float g = dy[i];
-    if (relu && !(bf16_f32(y[i]) > 0.f)) g = 0.f;
-    dz[i] = bf16_rne(g);
 }
 
 // Bias gradient, first stage: block b sums the rows [b*rows, (b+1)*rows) of
 // dz [P][C] per channel into part[b][C] (8 <= C, C divides 256; fixed order).
 // A thread reads 8 channels (16 bytes) of a pixel per iteration.  The second
-// stage is k_colsum over the blocks.
+// stage is part of k_reduce_grads.
 __global__ void k_bias_partial(const uint16_t* __restrict__ dz, float* __restrict__ part, int64_t P, int C,
                                int64_t rows)
 {
@@ -169,47 +168,52 @@ __global__ void k_bias_partial(const uint16_t* __restrict__ dz, float* __restric
     }
 }
 
-// Column sums of part[rows][C] in a fixed order: one block per column.
-__global__ void k_colsum(const float* __restrict__ part, float* __restrict__ out, int rows, int C)
-{
-    __shared__ float sh[256];
-    const int c = blockIdx.x, t = threadIdx.x;
-    float s = 0.f;
-    for (int r = t; r < rows; r += 256) s += part[(int64_t)r * C + c];
-    sh[t] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (t < k) sh[t] += sh[t + k];
-        __syncthreads();
-    }
-    if (t == 0) out[c] = sh[0];
-}
-
-// Sum of split-K partials in a fixed order: a block takes 64 outputs, 16
-// thread groups each sum every 16th split (coalesced rows of 64 floats), and
-// one thread per output adds the 16 group sums in group order.
-__global__ __launch_bounds__(1024) void k_reduce_splits(const float* __restrict__ part, float* __restrict__ out,
-                                                        int64_t n, int splits)
+// Weight and bias gradients from their partials, in a fixed order.  Blocks
+// below nbw: the split-K partials of the weights -- a block takes 64
+// outputs, 16 thread groups each sum every 16th split (coalesced rows of 64
+// floats), and one thread per output adds the 16 group sums in group order.
+// The next C blocks: the bias partials bpart[rows][C], one block per column,
+// 256 threads striding the rows and a tree over them.
+__global__ __launch_bounds__(1024) void k_reduce_grads(const float* __restrict__ part, float* __restrict__ out,
+                                                       int64_t n, int splits, int nbw, const float* __restrict__ bpart,
+                                                       float* __restrict__ bout, int rows, int C)
 {
     __shared__ float sh[16][64];
+    if ((int)blockIdx.x >= nbw) {
+        float* sb = &sh[0][0];
+        const int c = (int)blockIdx.x - nbw, t = threadIdx.x;
+        if (t < 256) {
+            float v = 0.f;
+            for (int r = t; r < rows; r += 256) v += bpart[(int64_t)r * C + c];
+            sb[t] = v;
+        }
+        __syncthreads();
+        for (int k = 128; k > 0; k >>= 1) {
+            if (t < k) sb[t] += sb[t + k];
+            __syncthreads();
+        }
+        if (t == 0) bout[c] = sb[0];
+        return;
+    }
     const int o = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int64_t i = (int64_t)blockIdx.x * 64 + o;
-    float s = 0.f;
+    float v = 0.f;
     if (i < n)
-        for (int z = g; z < splits; z += 16) s += part[(int64_t)z * n + i];
-    sh[g][o] = s;
+        for (int z = g; z < splits; z += 16) v += part[(int64_t)z * n + i];
+    sh[g][o] = v;
     __syncthreads();
     if (g == 0 && i < n) {
-        float v = 0.f;
-        for (int k = 0; k < 16; ++k) v += sh[k][o];
-        out[i] = v;
+        float acc = 0.f;
+        for (int k = 0; k < 16; ++k) acc += sh[k][o];
+        out[i] = acc;
     }
 }
 
 // Weighted MSE over the first `used` channels of the f32 head output y
-// ([P][8]): loss partials per block and dy = 2 w (y - t) / norm.
+// ([P][8]): loss partials per block and the head's output gradient
+// dz = bf16(2 w (y - t) / norm) (the head has no ReLU).
 __global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ w,
-                      float* __restrict__ dy, float* __restrict__ partial, int64_t P, int used, float inv_norm)
+                      uint16_t* __restrict__ dz, float* __restrict__ partial, int64_t P, int used, float inv_norm)
 {
     __shared__ float sh[256];
     float s = 0.f;
@@ -222,7 +226,7 @@ __global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, 
                 s += wt * d * d;
                 g = 2.f * wt * d * inv_norm;
             }
-            dy[p * 8 + c] = g;
+            dz[p * 8 + c] = bf16_rne(g);
         }
     }
     sh[threadIdx.x] = s;
@@ -311,7 +315,8 @@ struct cs_cnn {
     // scratch
     float* part = nullptr;
     int64_t part_elems = 0;
-    float* dy32 = nullptr;            // two f32 gradient buffers of act_max elements each
+    float* bpart = nullptr;           // bias-gradient partials [<= 1024][cout]
+    float* stage32 = nullptr;         // f32 staging of host inputs (act_max elements)
     int64_t act_max = 0;
     float* skip32[3] = {nullptr, nullptr, nullptr};  // accumulated gradients of e1, e2, e3 outputs
     uint16_t* dz = nullptr;           // bf16 gradients of a conv's output and input (largest activation each)
@@ -463,33 +468,27 @@ void conv_forward(cs_cnn* net, int li, const Act& x, const Act& y, float* y32 = 
              (int)x.P(), c.cout, c.kpad, 1, &x, 1, c.dil);
 }
 
-// Backward of conv li.  The gradient of its output comes either as f32 (dy32,
-// then masked by the layer's ReLU into dz) or, with dy32 null, already masked
-// in bf16 in dz.  Writes dW, dB and, when dz_out is not null, the gradient of
-// its input in bf16: masked by the input's own ReLU when mask_in (the input is
-// the previous conv's ReLU output), plain otherwise (pool and concat outputs).
-void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float* dy32, uint16_t* dz,
-                   uint16_t* dz_out, bool mask_in)
+// Backward of conv li from the bf16 gradient of its output in dz (already
+// masked by the layer's ReLU).  Writes dW, dB and, when dz_out is not null,
+// the gradient of its input in bf16: masked by the input's own ReLU when
+// mask_in (the input is the previous conv's ReLU output), plain otherwise
+// (pool and concat outputs).  The bias partials go to bpart; one reduction
+// launch then sums them and the weight gradient's split-K partials.
+void conv_backward(cs_cnn* net, int li, const Act& x, uint16_t* dz, uint16_t* dz_out, bool mask_in)
 {
     Conv& c = net->convs[(size_t)li];
     const int64_t P = x.P();
     CS_REQUIRE(dz != dz_out, CS_ERR_STATE, "gradient buffers alias");
-    if (dy32) {
-        hipLaunchKernelGGL(k_relu_back, dim3(blocks_for(P * c.cout)), dim3(256), 0, net->stream, dy32, y.d, dz,
-                           P * c.cout, c.relu ? 1 : 0);
-        CS_HIP(hipGetLastError());
-    }
     CS_REQUIRE(c.cout >= 8 && c.cout <= 256 && 256 % c.cout == 0 && P % 8 == 0, CS_ERR_STATE,
                "unsupported layer shape");
+    const int64_t bblocks = std::min<int64_t>(1024, (P + 255) / 256);
     {
-        const int64_t blocks = std::min<int64_t>(1024, (P + 255) / 256);
-        const int64_t rows = (P + blocks - 1) / blocks;
-        hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)blocks), dim3(256), 0, net->stream, dz, net->part, P,
+        const int64_t rows = (P + bblocks - 1) / bblocks;
+        hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)bblocks), dim3(256), 0, net->stream, dz, net->bpart, P,
                            c.cout, rows);
-        hipLaunchKernelGGL(k_colsum, dim3((unsigned)c.cout), dim3(256), 0, net->stream, net->part, c.gb, (int)blocks,
-                           c.cout);
         CS_HIP(hipGetLastError());
     }
+    const int nbw = (int)((c.nw() + 63) / 64);
     // weight gradient: dW[co][k] = sum_p dz[p][co] x[p + tap offset][ci],
     // k = tap*cin + ci, split over p.  Dilation 1: the LDS-tile kernel;
     // otherwise the GEMM, whose loaders read both operands from their NHWC
@@ -507,8 +506,8 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
         else hipLaunchKernelGGL((k_wgrad_halo<64>), grid, dim3(256), 0, net->stream, wa);
         CS_HIP(hipGetLastError());
         net->gemm_flops += 2.0 * c.cout * c.kpad * (double)P;
-        hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream,
-                           net->part, c.gw, c.nw(), runs);
+        hipLaunchKernelGGL(k_reduce_grads, dim3((unsigned)(nbw + c.cout)), dim3(1024), 0, net->stream, net->part, c.gw,
+                           c.nw(), runs, nbw, net->bpart, c.gb, (int)bblocks, c.cout);
         CS_HIP(hipGetLastError());
     } else {
         // enough K ranges for ~1024 workgroups, each at least 1024 pixels long
@@ -519,8 +518,8 @@ void conv_backward(cs_cnn* net, int li, const Act& x, const Act& y, const float*
         while (splits > 1 && (int64_t)splits * c.cout * c.kpad > net->part_elems) --splits;
         gemm(net, PMAJOR_T, GATHER_T, EPI_F32, dz, c.cout, x.d, 0, net->part, c.kpad, nullptr, c.cout, c.kpad, (int)P,
              splits, &x, 1, c.dil);
-        hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((c.nw() + 63) / 64)), dim3(1024), 0, net->stream,
-                           net->part, c.gw, c.nw(), splits);
+        hipLaunchKernelGGL(k_reduce_grads, dim3((unsigned)(nbw + c.cout)), dim3(1024), 0, net->stream, net->part, c.gw,
+                           c.nw(), splits, nbw, net->bpart, c.gb, (int)bblocks, c.cout);
         CS_HIP(hipGetLastError());
     }
     if (!dz_out) return;
@@ -579,10 +578,12 @@ void forward(cs_cnn* net, int n)
     CS_HIP(hipGetLastError());
 }
 
-// Backward from the f32 gradient of the head output (net->dy32 holds it on entry).
-// Between consecutive convs the gradient stays in bf16 (two buffers used in
-// turn); where gradients add up (concat and pool skips) it is accumulated in
-// f32 and masked by the next conv's backward.
+// Backward from the bf16 gradient of the head output (net->dz holds it on
+// entry, written by k_mse).  Between consecutive convs the gradient stays in
+// bf16 (two buffers used in turn); the skip gradients of the encoder outputs
+// add up in f32 (the concat's share, then the pool's), and the kernels that
+// finish a gradient write it in bf16, masked by the ReLU of the conv whose
+// output it is.
 void backward(cs_cnn* net, int n)
 {
     auto& ci = net->conv_in;
@@ -590,53 +591,53 @@ void backward(cs_cnn* net, int n)
     hipStream_t s = net->stream;
     const int H = net->H, W = net->W;
     uint16_t* zb[2] = {net->dz, net->dz2};
-    int zc = 0;                     // zb[zc]: bf16 gradient of the current conv's output
-    const float* g32 = net->dy32;   // ... or, when not null, its f32 gradient
-    float* dcoarse = net->dy32 + net->act_max;
+    int zc = 0;  // zb[zc]: bf16 gradient of the current conv's output
     // input gradient kinds: 0 none, 1 masked by the previous conv's ReLU, 2 plain
     auto step = [&](int li, int kind) {
         const bool mask = kind == 1 && net->convs[(size_t)li - 1].relu;
-        conv_backward(net, li, ci[(size_t)li], co[(size_t)li], g32, zb[zc], kind ? zb[zc ^ 1] : nullptr, mask);
-        g32 = nullptr;
+        conv_backward(net, li, ci[(size_t)li], zb[zc], kind ? zb[zc ^ 1] : nullptr, mask);
         zc ^= 1;
     };
-    auto concat_back = [&](int C1, int C2, int Hc, int Wc, float* skip, int64_t nskip) {
-        // zb[zc]: bf16 gradient of the concat; coarse part -> dcoarse (f32),
-        // skip part accumulated into the skip gradient
-        CS_HIP(hipMemsetAsync(skip, 0, sizeof(float) * (size_t)nskip, s));
-        const int64_t nmax = std::max<int64_t>((int64_t)n * (Hc / 2) * (Wc / 2) * C1, nskip);
-        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, zb[zc], dcoarse, skip, n, Hc,
-                           Wc, C1, C2);
+    // zb[zc]: bf16 gradient of the concat of conv `li`'s up-sampled output and
+    // a skip tensor: conv li's masked output gradient -> zb[zc ^ 1], the skip
+    // part -> the skip accumulator
+    auto concat_back = [&](int li, int C2, int Hc, int Wc, float* skip) {
+        const int C1 = co[(size_t)li].C;
+        const int64_t nmax = std::max<int64_t>((int64_t)n * (Hc / 2) * (Wc / 2) * C1, (int64_t)n * Hc * Wc * C2);
+        hipLaunchKernelGGL(k_up_concat_back, dim3(blocks_for(nmax)), dim3(256), 0, s, zb[zc], co[(size_t)li].d,
+                           net->convs[(size_t)li].relu ? 1 : 0, zb[zc ^ 1], skip, n, Hc, Wc, C1, C2);
         CS_HIP(hipGetLastError());
-        g32 = dcoarse;
+        zc ^= 1;
     };
-    auto pool_back = [&](int Hc, int Wc, int C, float* skip) {
-        // zb[zc]: bf16 gradient of the pooled tensor, added to the skip gradient
-        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for((int64_t)n * Hc * Wc * C)), dim3(256), 0, s, zb[zc], skip, n,
-                           Hc, Wc, C);
+    // zb[zc]: bf16 gradient of conv `li`'s pooled output; with the skip
+    // accumulator, conv li's masked output gradient -> zb[zc ^ 1]
+    auto pool_back = [&](int li, int Hc, int Wc, float* skip) {
+        const int C = co[(size_t)li].C;
+        hipLaunchKernelGGL(k_pool_back, dim3(blocks_for((int64_t)n * Hc * Wc * C)), dim3(256), 0, s, zb[zc], skip,
+                           co[(size_t)li].d, net->convs[(size_t)li].relu ? 1 : 0, zb[zc ^ 1], n, Hc, Wc, C);
         CS_HIP(hipGetLastError());
-        g32 = skip;
+        zc ^= 1;
     };
     // head and d1
     step(14, 1);
     step(13, 1);
     step(12, 2);  // gradient of cat1 (channels co[11].C + co[1].C)
-    concat_back(co[11].C, co[1].C, H, W, net->skip32[0], co[1].n());
+    concat_back(11, co[1].C, H, W, net->skip32[0]);
     step(11, 1);
     step(10, 2);  // gradient of cat2
-    concat_back(co[9].C, co[3].C, H / 2, W / 2, net->skip32[1], co[3].n());
+    concat_back(9, co[3].C, H / 2, W / 2, net->skip32[1]);
     step(9, 1);
     step(8, 2);   // gradient of cat3
-    concat_back(co[7].C, co[5].C, H / 4, W / 4, net->skip32[2], co[5].n());
+    concat_back(7, co[5].C, H / 4, W / 4, net->skip32[2]);
     step(7, 1);
-    step(6, 2);   // gradient of p3 -> the e3 skip accumulator, total gradient of co[5]
-    pool_back(H / 4, W / 4, co[5].C, net->skip32[2]);
+    step(6, 2);   // gradient of p3 -> with the e3 skip gradient, the gradient of co[5]
+    pool_back(5, H / 4, W / 4, net->skip32[2]);
     step(5, 1);
     step(4, 2);   // gradient of p2
-    pool_back(H / 2, W / 2, co[3].C, net->skip32[1]);
+    pool_back(3, H / 2, W / 2, net->skip32[1]);
     step(3, 1);
     step(2, 2);   // gradient of p1
-    pool_back(H, W, co[1].C, net->skip32[0]);
+    pool_back(1, H, W, net->skip32[0]);
     step(1, 1);
     step(0, 0);
 }
@@ -655,7 +656,7 @@ void check_net(const cs_cnn* n) { CS_REQUIRE(n != nullptr, CS_ERR_ARG, "null net
 void upload_inputs(cs_cnn* net, int n, const float* input)
 {
     const int64_t cnt = (int64_t)n * net->H * net->W * 8;
-    float* tmp = (float*)net->dy32;  // large enough f32 staging
+    float* tmp = net->stage32;  // large enough f32 staging
     CS_HIP(hipMemcpyAsync(tmp, input, sizeof(float) * (size_t)cnt, hipMemcpyHostToDevice, net->stream));
     hipLaunchKernelGGL(k_f32_to_bf16, dim3(blocks_for(cnt)), dim3(256), 0, net->stream, tmp, net->x_in.d, cnt);
     CS_HIP(hipGetLastError());
@@ -667,7 +668,7 @@ float loss_and_grad(cs_cnn* net, int n, const float* targets_dev, const float* w
     // loss = sum w (y - t)^2 / (3 P)
     const int blocks = 256;
     hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, net->stream, net->head_out, targets_dev, weights_dev,
-                       net->dy32, net->loss_part, P, 3, 1.0f / (float)(P * 3));
+                       net->dz, net->loss_part, P, 3, 1.0f / (float)(P * 3));
     CS_HIP(hipGetLastError());
     std::vector<float> part(blocks);
     CS_HIP(hipMemcpyAsync(part.data(), net->loss_part, sizeof(float) * blocks, hipMemcpyDeviceToHost, net->stream));
@@ -783,7 +784,7 @@ extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int b
                                  net->cat2,        net->conv_out[10], net->cat1,       net->conv_out[12], net->conv_out[13]};
             for (int i = 0; i < 15; ++i) net->conv_in.push_back(ins[i]);
             net->head_out = dalloc<float>(net, (int64_t)n * H * W * 8);
-            // scratch: split-K partials of the weight gradient and the bias partials
+            // scratch: split-K partials of the weight gradient
             int64_t part = 0;
             for (int i = 0; i < 15; ++i) {
                 const int64_t P = net->conv_in[(size_t)i].P();
@@ -792,10 +793,11 @@ extern "C" int cs_cnn_create(cs_ctx* ctx, int height, int width, int base, int b
             }
             net->part = dalloc<float>(net, part);
             net->part_elems = part;
+            net->bpart = dalloc<float>(net, (int64_t)1024 * 256);
             const int64_t amax = (int64_t)n * H * W * 96;  // >= every activation (3c <= 96 for c <= 32)
             const int64_t amax2 = std::max<int64_t>(amax, (int64_t)n * H * W * 3 * c);
             net->act_max = amax2;
-            net->dy32 = dalloc<float>(net, 2 * amax2);
+            net->stage32 = dalloc<float>(net, amax2);
             net->dz = dalloc<uint16_t>(net, amax2);
             net->dz2 = dalloc<uint16_t>(net, amax2);
             net->skip32[0] = dalloc<float>(net, net->conv_out[1].n());
@@ -964,11 +966,11 @@ extern "C" int cs_cnn_set_dataset(cs_cnn* net, int count, const float* inputs, c
         CS_HIP(hipMalloc(&net->ds_in, sizeof(uint16_t) * (size_t)(count * hw * 8)));
         CS_HIP(hipMalloc(&net->ds_t, sizeof(float) * (size_t)(count * hw * 3)));
         CS_HIP(hipMalloc(&net->ds_w, sizeof(float) * (size_t)(count * hw)));
-        // inputs converted on the device, one sample at a time through dy32
+        // inputs converted on the device, one sample at a time through stage32
         for (int s = 0; s < count; ++s) {
-            CS_HIP(hipMemcpyAsync(net->dy32, inputs + (int64_t)s * hw * 8, sizeof(float) * (size_t)(hw * 8),
+            CS_HIP(hipMemcpyAsync(net->stage32, inputs + (int64_t)s * hw * 8, sizeof(float) * (size_t)(hw * 8),
                                   hipMemcpyHostToDevice, net->stream));
-            hipLaunchKernelGGL(k_f32_to_bf16, dim3(blocks_for(hw * 8)), dim3(256), 0, net->stream, net->dy32,
+            hipLaunchKernelGGL(k_f32_to_bf16, dim3(blocks_for(hw * 8)), dim3(256), 0, net->stream, net->stage32,
                                net->ds_in + (int64_t)s * hw * 8, hw * 8);
             CS_HIP(hipGetLastError());
             CS_HIP(hipStreamSynchronize(net->stream));
@@ -1023,7 +1025,7 @@ extern "C" int cs_cnn_time_step(cs_cnn* net, int iters, float* ms, double* flops
             forward(net, n);
             f = net->gemm_flops;
             hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, net->stream, net->head_out, net->targets, net->weights,
-                               net->dy32, net->loss_part, (int64_t)n * net->H * net->W, 3, 1.0f);
+                               net->dz, net->loss_part, (int64_t)n * net->H * net->W, 3, 1.0f);
             net->gemm_flops = 0;
             backward(net, n);
             f += net->gemm_flops;
