@@ -240,7 +240,8 @@ def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
     """512 independent lid-driven cavities (sides U{64..512}, tau U[0.55,1],
     lid U[0.02,0.1], seed 0), FP32, half-way walls: the autotuned batch launch
     for `steps` steps, the other launch kinds (persistent, tile list,
-    temporal) for `other_steps`, and one launch per lattice per step."""
+    temporal, persistent temporal) for `other_steps`, and one launch per
+    lattice per step."""
     from chemsim_amd import Params, capi
     from chemsim_amd import workloads as W
     from chemsim_amd.batch import Batch, tune
@@ -256,7 +257,7 @@ def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
     ms = b.step(steps)
     best = b.tuning
     kinds = {capi.CS_LAUNCH_PERSISTENT: "persistent", capi.CS_LAUNCH_TILES: "tiles",
-             capi.CS_LAUNCH_TEMPORAL: "temporal"}
+             capi.CS_LAUNCH_TEMPORAL: "temporal", capi.CS_LAUNCH_TEMPORAL_PERSISTENT: "temporal_persistent"}
     others = {}
     for kind in kinds:
         if kind == best.get("launch", 0):
@@ -272,7 +273,7 @@ def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
     # bytes the launch kind must move per cell update: 72 for one-step launches; for the
     # temporal kind every 64 x 64 region read and every cell written once per 8 steps
     bpc = 72.0
-    if best.get("launch", 0) == capi.CS_LAUNCH_TEMPORAL:
+    if best.get("launch", 0) in (capi.CS_LAUNCH_TEMPORAL, capi.CS_LAUNCH_TEMPORAL_PERSISTENT):
         bpc = sum(tb_bytes_per_step(s.side, s.side) for s in specs) / cells
     out = {"lattices": len(b), "cells": cells, "steps": steps,
            "GLUPS_batch": round(cells * steps / ms / 1e6, 3), "hbm_frac": hbm_frac(bpc, cells, ms / steps),

@@ -26,7 +26,7 @@ CS_BB_REFERENCE, CS_BB_HALFWAY = range(2)
 CS_FLUID, CS_WALL, CS_LID, CS_INLET, CS_OUTLET = range(5)
 CS_ROUND_NEAREST, CS_ROUND_STOCHASTIC = range(2)
 CS_ORDER_LARGEST_FIRST, CS_ORDER_GIVEN, CS_ORDER_SMALLEST_FIRST = range(3)
-CS_LAUNCH_PERSISTENT, CS_LAUNCH_TILES, CS_LAUNCH_TEMPORAL = range(3)
+CS_LAUNCH_PERSISTENT, CS_LAUNCH_TILES, CS_LAUNCH_TEMPORAL, CS_LAUNCH_TEMPORAL_PERSISTENT = range(4)
 CS_COLL_BGK, CS_COLL_TRT, CS_COLL_REGULARIZED, CS_COLL_KBC = range(4)
 CS_RENDER_DENSITY, CS_RENDER_SPEED, CS_RENDER_VELOCITY, CS_RENDER_MOMENTUM = range(4)
 

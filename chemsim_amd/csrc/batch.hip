@@ -10,10 +10,14 @@
 //  - tiles (k_batch_tiles): one launch per step over the flattened list of
 //    4 x 64 tiles of every lattice, so a large lattice is spread over the
 //    whole GPU;
-//  - temporal (k_batch_tb, 32-bit storage): one launch per 8 steps over the
-//    flattened list of 64 x 64 regions of every lattice, each block running
-//    the 8 steps out of LDS (lbm_tb.hpp); the remainder of a call's steps
-//    (steps mod 8) runs as tile launches.
+//  - temporal (k_batch_tb / k_batch_tbr, 32-bit storage): one launch per 8
+//    steps over the flattened list of 64 x 64 regions of every lattice, each
+//    block running the 8 steps on its region in LDS or registers (lbm_tb.hpp);
+//    the remainder of a call's steps (steps mod 8) runs as tile launches;
+//  - persistent temporal (k_batch_tb_grid / k_batch_tbr_grid): the same
+//    rounds of 8 steps in one cooperative launch, a co-resident grid whose
+//    blocks each take a contiguous share of the region list and meet at a
+//    grid-wide barrier after every round.
 // The per-cell code is the single-lattice gather/finish either way, so the
 // results equal per-lattice launches bit for bit.
 #include <hip/hip_runtime.h>
@@ -264,12 +268,12 @@ __device__ __forceinline__ void batch_tb_region(const BatchLat<typename St::T>* 
                                                 const TileRef* __restrict__ regions, void* __restrict__ arena,
                                                 const uint32_t* __restrict__ nodes,
                                                 const LatState* __restrict__ state, int flip,
-                                                StepArgs<typename St::T>& a, int& ty, int& tx)
+                                                StepArgs<typename St::T>& a, int& ty, int& tx, bool direct = false)
 {
     using S = typename St::S;
     using T = typename St::T;
     constexpr int IN = TB_EDGE - 2 * K;
-    const TileRef tr = regions[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileRef tr = direct ? regions[0] : regions[xcd_tile(blockIdx.x, gridDim.x)];
     const int li = __builtin_amdgcn_readfirstlane(tr.lat);
     const int t = __builtin_amdgcn_readfirstlane(tr.tile);
     const BatchLat<T> D = lats[li];
@@ -318,6 +322,95 @@ __global__ __launch_bounds__(TBR_THREADS, 4) void k_batch_tbr(const BatchLat<typ
     tbr_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, X);
 }
 
+// Grid-wide barrier of the persistent temporal kind: a monotonic arrival
+// counter (round r waits for (r + 1) x blocks arrivals) with agent-scope
+// release/acquire ordering, so every block's stores of a round are visible to
+// every block (on any XCD) after it.  The wait is bounded: past about a
+// second it raises `abort`, every block leaves at its next barrier and the
+// host reports the timeout -- the grid always drains.
+struct GridBarrier {
+    unsigned count;
+    unsigned abort;
+};
+
+__device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned target)
+{
+    __shared__ int s_ok;
+    __threadfence();  // this thread's stores of the round, at agent scope
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        unsigned it = 0;
+        while (__hip_atomic_load(&b->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++it > (1u << 24)) {
+                __hip_atomic_store(&b->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);  // every wave reads the other blocks' stores afresh
+    return s_ok != 0;
+}
+
+// Region range of block b of a persistent grid: contiguous shares, so the
+// neighbouring regions of a lattice mostly run in one block.
+__device__ __forceinline__ void grid_share(int nregions, int& r0, int& r1)
+{
+    r0 = (int)((int64_t)nregions * blockIdx.x / gridDim.x);
+    r1 = (int)((int64_t)nregions * (blockIdx.x + 1) / gridDim.x);
+}
+
+template <class St, int EDGE, int MODE, int K>
+__global__ __launch_bounds__(TB_THREADS) void k_batch_tb_grid(const BatchLat<typename St::T>* __restrict__ lats,
+                                                           const TileRef* __restrict__ regions,
+                                                           void* __restrict__ arena,
+                                                           const uint32_t* __restrict__ nodes,
+                                                           const LatState* __restrict__ state, int nregions,
+                                                           int rounds, GridBarrier* bar)
+{
+    __shared__ typename St::T L[9][TB_EDGE][TB_EDGE];
+    int r0, r1;
+    grid_share(nregions, r0, r1);
+    for (int round = 0; round < rounds; ++round) {
+        for (int r = r0; r < r1; ++r) {
+            StepArgs<typename St::T> a{};
+            int ty, tx;
+            batch_tb_region<St, EDGE, MODE, K>(lats, regions + r, arena, nodes, state, round, a, ty, tx, true);
+            tb_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, L);
+            __syncthreads();  // the region's interior stores have read the LDS before the next load
+        }
+        if (!grid_barrier(bar, (unsigned)((round + 1) * gridDim.x))) return;
+    }
+}
+
+template <class St, int EDGE, int MODE, int K>
+__global__ __launch_bounds__(TBR_THREADS, 4) void k_batch_tbr_grid(const BatchLat<typename St::T>* __restrict__ lats,
+                                                                const TileRef* __restrict__ regions,
+                                                                void* __restrict__ arena,
+                                                                const uint32_t* __restrict__ nodes,
+                                                                const LatState* __restrict__ state, int nregions,
+                                                                int rounds, GridBarrier* bar)
+{
+    __shared__ typename St::T X[2][TBR_WAVES][2][3][TB_EDGE];
+    int r0, r1;
+    grid_share(nregions, r0, r1);
+    for (int round = 0; round < rounds; ++round) {
+        for (int r = r0; r < r1; ++r) {
+            StepArgs<typename St::T> a{};
+            int ty, tx;
+            batch_tb_region<St, EDGE, MODE, K>(lats, regions + r, arena, nodes, state, round, a, ty, tx, true);
+            tbr_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, X);
+            __syncthreads();  // every wave has read the exchange buffers of the region's last step
+        }
+        if (!grid_barrier(bar, (unsigned)((round + 1) * gridDim.x))) return;
+    }
+}
+
 // Steps per launch of the temporal launch kind.
 constexpr int BATCH_TB_STEPS = 8;
 
@@ -353,7 +446,8 @@ struct cs_batch {
     int* d_queue = nullptr;
     TileRef* d_tiles = nullptr;
     int64_t max_tiles = 0;
-    TileRef* d_regions = nullptr;     // region list of the temporal launch kind
+    TileRef* d_regions = nullptr;     // region list of the temporal launch kinds
+    GridBarrier* d_bar = nullptr;     // grid barrier of the persistent temporal kind
     int64_t max_regions = 0;
     LatState* d_state = nullptr;
     bool lats_dirty = true;
@@ -647,6 +741,47 @@ static void launch_tb_group(cs_batch* b, int group, int first, int count, int fl
 
 static bool batch_tb_storage(int storage) { return storage == CS_F32 || storage == CS_F32DEV; }
 
+// All rounds of 8 steps of one mode group in one cooperative launch: the
+// grid is as many blocks as can be resident at once (the launch fails rather
+// than start a grid that could not meet at its barriers).
+template <class St>
+static void launch_tb_grid_group(cs_batch* b, int group, int first, int count, int rounds)
+{
+    if constexpr (sizeof(typename St::S) != 4) {
+        throw Error{CS_ERR_UNSUPPORTED, "the temporal launch kinds need 32-bit storage"};
+    } else {
+        using T = typename St::T;
+        const auto* lats = (const BatchLat<T>*)b->d_lats;
+        const TileRef* regions = b->d_regions + first;
+        void* arena = b->arena;
+        const uint32_t* nodes = b->nodes;
+        const LatState* state = b->d_state;
+        GridBarrier* bar = b->d_bar;
+        int nregions = count;
+        auto go = [&]<int EDGE, int MODE>() {
+            const void* fn = MODE == MODE_HALF ? (const void*)k_batch_tb_grid<St, EDGE, MODE, BATCH_TB_STEPS>
+                                               : (const void*)k_batch_tbr_grid<St, EDGE, MODE, BATCH_TB_STEPS>;
+            const int threads = MODE == MODE_HALF ? TB_THREADS : TBR_THREADS;
+            int per_cu = 0;
+            CS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0));
+            CS_REQUIRE(per_cu >= 1, CS_ERR_UNSUPPORTED, "the persistent temporal kernel does not fit a CU");
+            const int grid = std::min(count, per_cu * b->ctx->prop.multiProcessorCount);
+            void* args[] = {(void*)&lats, (void*)&regions, (void*)&arena, (void*)&nodes, (void*)&state,
+                            (void*)&nregions, (void*)&rounds, (void*)&bar};
+            CS_HIP(hipMemsetAsync(bar, 0, sizeof(GridBarrier), b->stream));
+            CS_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3((unsigned)threads), args, 0, b->stream));
+        };
+        auto go_mode = [&]<int EDGE>() {
+            const int m = (group % 6) / 2;
+            if (m == MODE_FLUID) go.template operator()<EDGE, MODE_FLUID>();
+            else if (m == MODE_HALF) go.template operator()<EDGE, MODE_HALF>();
+            else go.template operator()<EDGE, MODE_REF>();
+        };
+        if (group >= 6) go_mode.template operator()<CS_EDGE_PERIODIC>();
+        else go_mode.template operator()<CS_EDGE_ZERO>();
+    }
+}
+
 static void launch_batch(cs_batch* b, int steps)
 {
     upload_descriptors(b);
@@ -654,19 +789,31 @@ static void launch_batch(cs_batch* b, int steps)
     const auto groups = upload_order(b);
     CS_REQUIRE(groups.size() <= 16, CS_ERR_STATE, "too many boundary-mode groups");
     int flips = steps;
-    if (b->tuning.launch == CS_LAUNCH_TILES || b->tuning.launch == CS_LAUNCH_TEMPORAL) {
+    const int kind = b->tuning.launch;
+    if (kind == CS_LAUNCH_TILES || kind == CS_LAUNCH_TEMPORAL || kind == CS_LAUNCH_TEMPORAL_PERSISTENT) {
         int s = 0;
         flips = 0;
-        if (b->tuning.launch == CS_LAUNCH_TEMPORAL && steps >= BATCH_TB_STEPS) {
+        if (kind != CS_LAUNCH_TILES && steps >= BATCH_TB_STEPS) {
             CS_REQUIRE(batch_tb_storage(b->storage), CS_ERR_UNSUPPORTED,
-                       "the temporal launch kind needs FP32 or FP32-deviation storage");
-            // one launch per BATCH_TB_STEPS steps and mode group over the region list
+                       "the temporal launch kinds need FP32 or FP32-deviation storage");
             const auto rg = upload_regions(b, groups);
-            for (; steps - s >= BATCH_TB_STEPS; s += BATCH_TB_STEPS, ++flips)
-                for (const auto& g : rg)
-                    if (g[2] > 0)
-                        dispatch_batch_storage(b->storage,
-                                               [&]<class St>() { launch_tb_group<St>(b, g[0], g[1], g[2], flips); });
+            const int rounds = steps / BATCH_TB_STEPS;
+            if (kind == CS_LAUNCH_TEMPORAL_PERSISTENT && rg.size() == 1) {
+                // every round of every lattice in one launch (one mode group)
+                const auto& g = rg[0];
+                if (g[2] > 0)
+                    dispatch_batch_storage(b->storage,
+                                           [&]<class St>() { launch_tb_grid_group<St>(b, g[0], g[1], g[2], rounds); });
+                s += rounds * BATCH_TB_STEPS;
+                flips += rounds;
+            } else {
+                // one launch per BATCH_TB_STEPS steps and mode group over the region list
+                for (; steps - s >= BATCH_TB_STEPS; s += BATCH_TB_STEPS, ++flips)
+                    for (const auto& g : rg)
+                        if (g[2] > 0)
+                            dispatch_batch_storage(
+                                b->storage, [&]<class St>() { launch_tb_group<St>(b, g[0], g[1], g[2], flips); });
+            }
         }
         if (s < steps) {
             // one launch per step and mode group over the flattened tile list
@@ -690,6 +837,12 @@ static void launch_batch(cs_batch* b, int steps)
     }
     CS_HIP(hipGetLastError());
     advance_state(b, steps, flips);
+    if (kind == CS_LAUNCH_TEMPORAL_PERSISTENT) {
+        GridBarrier hb{};
+        CS_HIP(hipMemcpyAsync(&hb, b->d_bar, sizeof hb, hipMemcpyDeviceToHost, b->stream));
+        CS_HIP(hipStreamSynchronize(b->stream));
+        CS_REQUIRE(hb.abort == 0u, CS_ERR_STATE, "persistent temporal launch: grid barrier timed out");
+    }
 }
 
 // One k_step launch per lattice per step (single-lattice code path).
@@ -835,6 +988,8 @@ extern "C" int cs_batch_create(cs_ctx* ctx, int count, const int32_t* w, const i
             CS_HIP(hipMalloc(&b->d_queue, sizeof(int) * 16));
             CS_HIP(hipMalloc(&b->d_tiles, sizeof(TileRef) * (size_t)b->max_tiles));
             CS_HIP(hipMalloc(&b->d_regions, sizeof(TileRef) * (size_t)b->max_regions));
+            CS_HIP(hipMalloc(&b->d_bar, sizeof(GridBarrier)));
+            CS_HIP(hipMemset(b->d_bar, 0, sizeof(GridBarrier)));
             CS_HIP(hipMalloc(&b->d_state, sizeof(LatState) * (size_t)count));
             CS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
             CS_HIP(hipEventCreate(&b->ev0));
@@ -853,7 +1008,7 @@ extern "C" int cs_batch_destroy(cs_batch* b)
         if (!b) return;
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         for (void* p : {b->arena, (void*)b->nodes, b->d_lats, (void*)b->d_order, (void*)b->d_queue, (void*)b->snap,
-                        (void*)b->d_res, (void*)b->d_tiles, (void*)b->d_regions, (void*)b->d_state})
+                        (void*)b->d_res, (void*)b->d_tiles, (void*)b->d_regions, (void*)b->d_bar, (void*)b->d_state})
             if (p) (void)hipFree(p);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1057,10 +1212,11 @@ extern "C" int cs_batch_set_tuning(cs_batch* b, const cs_batch_tuning* t)
                    "workgroups_per_cu must be 1..16");
         CS_REQUIRE(t->cells_per_thread == 1 || t->cells_per_thread == 2, CS_ERR_ARG, "cells_per_thread must be 1 or 2");
         CS_REQUIRE(t->order >= CS_ORDER_LARGEST_FIRST && t->order <= CS_ORDER_SMALLEST_FIRST, CS_ERR_ARG, "bad order");
-        CS_REQUIRE(t->launch == CS_LAUNCH_PERSISTENT || t->launch == CS_LAUNCH_TILES ||
-                       t->launch == CS_LAUNCH_TEMPORAL, CS_ERR_ARG, "bad launch kind");
-        CS_REQUIRE(t->launch != CS_LAUNCH_TEMPORAL || batch_tb_storage(b->storage), CS_ERR_UNSUPPORTED,
-                   "the temporal launch kind needs FP32 or FP32-deviation storage");
+        CS_REQUIRE(t->launch >= CS_LAUNCH_PERSISTENT && t->launch <= CS_LAUNCH_TEMPORAL_PERSISTENT, CS_ERR_ARG,
+                   "bad launch kind");
+        CS_REQUIRE((t->launch != CS_LAUNCH_TEMPORAL && t->launch != CS_LAUNCH_TEMPORAL_PERSISTENT) ||
+                       batch_tb_storage(b->storage),
+                   CS_ERR_UNSUPPORTED, "the temporal launch kinds need FP32 or FP32-deviation storage");
         b->tuning = *t;
     });
 }
@@ -1129,7 +1285,8 @@ extern "C" int cs_batch_autotune(cs_batch* b, int trial_steps, cs_batch_tuning* 
                 trial(cs_batch_tuning{blk, 1, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_TILES});
             if (batch_tb_storage(b->storage))
                 for (int blk : {256, 512, 1024})  // block_threads: the remainder steps' tiles
-                    trial(cs_batch_tuning{blk, 1, 1, CS_ORDER_LARGEST_FIRST, CS_LAUNCH_TEMPORAL});
+                    for (int kind : {CS_LAUNCH_TEMPORAL, CS_LAUNCH_TEMPORAL_PERSISTENT})
+                        trial(cs_batch_tuning{blk, 1, 1, CS_ORDER_LARGEST_FIRST, kind});
             for (int blk : {256, 512, 1024})
                 for (int wpc : {1, 2, 4, 8})
                     for (int cpt : {1, 2})

@@ -187,15 +187,19 @@ def batch512(ctx, steps):
                              ("persistent", dict(block_threads=1024, workgroups_per_cu=2, cells_per_thread=1,
                                                  order=0, launch=capi.CS_LAUNCH_PERSISTENT)),
                              ("temporal", dict(block_threads=512, workgroups_per_cu=1, cells_per_thread=1, order=0,
-                                               launch=capi.CS_LAUNCH_TEMPORAL))):
+                                               launch=capi.CS_LAUNCH_TEMPORAL)),
+                             ("temporal_persistent", dict(block_threads=512, workgroups_per_cu=1, cells_per_thread=1,
+                                                          order=0, launch=capi.CS_LAUNCH_TEMPORAL_PERSISTENT))):
             b.tuning = tuning
-            n = max(8, steps // 8 * 8) if kind == "temporal" else steps
-            b.step(8 if kind == "temporal" else 2)
+            tb = kind.startswith("temporal")
+            n = max(8, steps // 8 * 8) if tb else steps
+            b.step(8 if tb else 2)
             ms = b.step(n)
-            kernel = {"tiles": "k_batch_tiles", "persistent": "k_batch", "temporal": "k_batch_tb"}[kind]
+            kernel = {"tiles": "k_batch_tiles", "persistent": "k_batch", "temporal": "k_batch_tb",
+                      "temporal_persistent": "k_batch_tb_grid"}[kind]
             _emit({"workload": f"batch512/{kind}", "kernel": kernel, "lattices": len(b), "cells": cells,
-                   "alg_bytes_per_launch": tb_bytes if kind == "temporal" else 72 * cells,
-                   "steps_per_launch": 8 if kind == "temporal" else 1,
+                   "alg_bytes_per_launch": tb_bytes if tb else 72 * cells,
+                   "steps_per_launch": n if kind == "temporal_persistent" else (8 if tb else 1),
                    "event_us_per_step": round(ms / n * 1e3, 2)})
 
 

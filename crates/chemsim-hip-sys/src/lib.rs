@@ -125,6 +125,7 @@ pub const CS_ORDER_SMALLEST_FIRST: i32 = 2;
 pub const CS_LAUNCH_PERSISTENT: i32 = 0;
 pub const CS_LAUNCH_TILES: i32 = 1;
 pub const CS_LAUNCH_TEMPORAL: i32 = 2;
+pub const CS_LAUNCH_TEMPORAL_PERSISTENT: i32 = 3;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default, PartialEq)]

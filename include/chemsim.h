@@ -281,8 +281,16 @@ enum cs_batch_order { CS_ORDER_LARGEST_FIRST = 0, CS_ORDER_GIVEN = 1, CS_ORDER_S
  * (FP32 and FP32-deviation storage): one launch per 8 steps over the 64 x 64
  * regions of every lattice, each run 8 steps out of LDS (the temporally
  * blocked kernel); the remaining steps of a call (steps mod 8) as tile
- * launches (block_threads applies). */
-enum cs_batch_launch { CS_LAUNCH_PERSISTENT = 0, CS_LAUNCH_TILES = 1, CS_LAUNCH_TEMPORAL = 2 };
+ * launches (block_threads applies).  CS_LAUNCH_TEMPORAL_PERSISTENT: the same
+ * rounds of 8 steps in one cooperative launch whose resident blocks meet at a
+ * grid-wide barrier after every round (a batch whose lattices fall into more
+ * than one boundary-mode group runs as CS_LAUNCH_TEMPORAL). */
+enum cs_batch_launch {
+    CS_LAUNCH_PERSISTENT = 0,
+    CS_LAUNCH_TILES = 1,
+    CS_LAUNCH_TEMPORAL = 2,
+    CS_LAUNCH_TEMPORAL_PERSISTENT = 3
+};
 typedef struct cs_batch_tuning {
     int32_t block_threads;     /* 256, 512 or 1024 */
     int32_t workgroups_per_cu; /* persistent grid = CUs x this (capped at the lattice count) */

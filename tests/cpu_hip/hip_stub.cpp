@@ -15,6 +15,7 @@
 // heap corruption.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -316,6 +317,21 @@ hipError_t hipLaunchKernel(const void*, dim3 grid, dim3 block, void**, size_t sh
     if (grid.x == 0 || grid.y == 0 || grid.z == 0 || block.x * block.y * block.z == 0 ||
         block.x * block.y * block.z > 1024 || shared > 160 * 1024)
         return hipErrorInvalidConfiguration;
+    return hipSuccess;
+}
+// Cooperative launches (a resident grid) check the same configuration; one
+// 1024-thread block per CU of the model device's 256 fits.
+hipError_t hipLaunchCooperativeKernel(const void* f, dim3 grid, dim3 block, void** args, unsigned int shared,
+                                      hipStream_t s)
+{
+    if (grid.x * grid.y * grid.z > 256u * (1024u / std::max(1u, block.x * block.y * block.z)))
+        return hipErrorCooperativeLaunchTooLarge;
+    return hipLaunchKernel(f, grid, block, args, shared, s);
+}
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int block, size_t)
+{
+    if (!n || block <= 0 || block > 1024) return hipErrorInvalidValue;
+    *n = 1024 / block;
     return hipSuccess;
 }
 void** __hipRegisterFatBinary(const void*)

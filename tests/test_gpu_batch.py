@@ -138,11 +138,16 @@ def test_autotune_keeps_state(ctx):
             np.testing.assert_array_equal(a.download(i), p.download(i))
 
 
-def test_cavity_batch_small(ctx):
-    """A reduced version of the 512-cavity workload: 24 cavities, sides 64..160."""
+@pytest.mark.parametrize("launch", [None, capi.CS_LAUNCH_TEMPORAL, capi.CS_LAUNCH_TEMPORAL_PERSISTENT])
+def test_cavity_batch_small(ctx, launch):
+    """A reduced version of the 512-cavity workload: 24 cavities, sides 64..160
+    (one boundary-mode group), with the default launch and the temporal kinds
+    (6 rounds of 8 steps in one persistent launch, then 2 tile launches)."""
     specs = W.batch_cavities(seed=3, count=24, lo=64, hi=160)
     b = Batch(ctx, [(s.side, s.side) for s in specs], "f32")
     with b:
+        if launch is not None:
+            b.tuning = dict(block_threads=256, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=launch)
         for i, s in enumerate(specs):
             b.set_params(i, Params(tau=s.tau, boundary=capi.CS_BB_HALFWAY, lid_uy=s.u_lid))
             b.set_flags(i, W.cavity_flags(s.side, s.side))
@@ -184,7 +189,8 @@ def test_inactive_lattices_keep_their_state(ctx):
 
 
 @pytest.mark.parametrize("launch,threads", [(capi.CS_LAUNCH_TILES, 256), (capi.CS_LAUNCH_TILES, 1024),
-                                            (capi.CS_LAUNCH_PERSISTENT, 256), (capi.CS_LAUNCH_TEMPORAL, 256)])
+                                            (capi.CS_LAUNCH_PERSISTENT, 256), (capi.CS_LAUNCH_TEMPORAL, 256),
+                                            (capi.CS_LAUNCH_TEMPORAL_PERSISTENT, 256)])
 def test_batch_tiny_lattices_match_oracle(ctx, launch, threads):
     """1x1, one-cell-wide and small ragged lattices beside a normal one, in
     both launch kinds: every lattice equals the oracle's run of it alone."""
@@ -208,3 +214,30 @@ def test_batch_tiny_lattices_match_oracle(ctx, launch, threads):
         b.step(13)
         for i, (op, flags, f0) in enumerate(specs):
             np.testing.assert_array_equal(b.download(i), O.run(op, f0, 13, flags), err_msg=f"lattice {i} {shapes[i]}")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32dev"])
+def test_persistent_temporal_fluid_batch(ctx, storage):
+    """One boundary-mode group without geometry (periodic, the register
+    variant's grid kernel): 12 lattices of uneven sizes, 19 steps (2 rounds in
+    one persistent launch, then 3 tile launches) and 16 more, against one
+    launch per lattice per step, bit for bit."""
+    shapes = [(64, 64), (100, 37), (33, 80), (128, 128), (72, 40), (40, 72), (96, 24), (17, 19), (150, 64),
+              (64, 150), (1, 1), (300, 7)]
+    def make():
+        b = Batch(ctx, shapes, storage)
+        for i, (w, h) in enumerate(shapes):
+            b.set_params(i, Params(tau=0.6 + 0.03 * i, edge=capi.CS_EDGE_PERIODIC))
+            r = np.random.default_rng(100 + i)
+            b.init_equilibrium(i, (1 + 0.01 * r.standard_normal((h, w))).astype(np.float32),
+                               (0.02 * r.standard_normal((h, w))).astype(np.float32),
+                               (0.02 * r.standard_normal((h, w))).astype(np.float32))
+        return b
+    with make() as a, make() as p:
+        a.tuning = dict(block_threads=512, workgroups_per_cu=1, cells_per_thread=1, order=0,
+                        launch=capi.CS_LAUNCH_TEMPORAL_PERSISTENT)
+        a.step(19)
+        a.step(16)
+        p.step_per_lattice(35)
+        for i in range(len(shapes)):
+            np.testing.assert_array_equal(a.download_raw(i), p.download_raw(i), err_msg=f"lattice {i} {shapes[i]}")

@@ -101,10 +101,13 @@ def test_512_cavity_batch_full_size(ctx):
                 ra = a.download_raw(i)
                 np.testing.assert_array_equal(ra, b.download_raw(i), err_msg=f"tiles, lattice {i}")
                 np.testing.assert_array_equal(ra, c.download_raw(i), err_msg=f"per-lattice, lattice {i}")
-        with cavity_batch(ctx, specs, temporal) as t:
-            t.step(steps)
-            for i in range(len(specs)):
-                np.testing.assert_array_equal(a.download_raw(i), t.download_raw(i), err_msg=f"temporal, lattice {i}")
+        for kind in (capi.CS_LAUNCH_TEMPORAL, capi.CS_LAUNCH_TEMPORAL_PERSISTENT):
+            print(f"[fullsize] 512 cavities, launch kind {kind}", flush=True)
+            with cavity_batch(ctx, specs, dict(temporal, launch=kind)) as t:
+                t.step(steps)
+                for i in range(len(specs)):
+                    np.testing.assert_array_equal(a.download_raw(i), t.download_raw(i),
+                                                  err_msg=f"launch kind {kind}, lattice {i}")
         order = np.argsort([s.side for s in specs], kind="stable")
         for i in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
             s = specs[i]
