@@ -613,14 +613,16 @@ union VecN {
 // Neighbour-lane exchange of a row vector's edge element by DPP wave shifts
 // (one VALU move; no LDS crossbar).  lane_prev: each lane receives lane - 1's
 // value, lane_next: lane + 1's; lane 0 / lane 63 receive 0 (the callers
-// replace those with a direct load).
+// replace those with a direct load).  The bound control bit writes that 0
+// itself (also for a source lane switched off in EXEC), so no register has to
+// be cleared before each shift.
 __device__ __forceinline__ uint32_t dpp_prev(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, true);
 }
 __device__ __forceinline__ uint32_t dpp_next(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130 /* wave_shl:1 */, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130 /* wave_shl:1 */, 0xF, 0xF, true);
 }
 template <class S, bool NEXT>
 __device__ __forceinline__ S lane_shift(S v)
