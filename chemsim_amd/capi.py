@@ -182,6 +182,7 @@ SIGNATURES = {
     "cs_cnn_time_step": [_P, _I, _FP, _DP],
     "cs_gemm_bf16": [_P, _I, _I, _I, _P, _P, _P, _FP],
     "cs_bench_copy": [_P, _I64, _I, _FP],
+    "cs_check_reciprocal": [_P, _P],
     "cs_set_cells_per_thread": [_I],
     "cs_set_step_graphs": [_I],
     "cs_set_temporal_steps": [_I],

@@ -1502,6 +1502,38 @@ extern "C" int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_pe
     });
 }
 
+// Every 32-bit pattern b: recip_rn(b) against the IEEE quotient one / b (one
+// = 1 from the host, so nothing folds); two NaNs count as equal.  65536 blocks
+// x 256 threads x 256 inputs = 2^32.
+__global__ void k_check_reciprocal(unsigned long long* __restrict__ bad, float one)
+{
+    const uint64_t base = ((uint64_t)blockIdx.x * 256u + threadIdx.x) * 256u;
+    unsigned int n = 0;
+    for (uint32_t k = 0; k < 256u; ++k) {
+        const float b = __builtin_bit_cast(float, (uint32_t)(base + k));
+        const float ref = one / b, got = recip_rn(b);
+        const bool same = __builtin_bit_cast(uint32_t, ref) == __builtin_bit_cast(uint32_t, got) ||
+                          (ref != ref && got != got);
+        n += same ? 0u : 1u;
+    }
+    if (n) atomicAdd(bad, (unsigned long long)n);
+}
+
+extern "C" int cs_check_reciprocal(cs_ctx* ctx, uint64_t* mismatches)
+{
+    return guarded([&] {
+        CS_REQUIRE(ctx && mismatches, CS_ERR_ARG, "bad argument");
+        CS_HIP(hipSetDevice(ctx->device));
+        DevBuf d(sizeof(unsigned long long));
+        CS_HIP(hipMemset(d.p, 0, sizeof(unsigned long long)));
+        hipLaunchKernelGGL(k_check_reciprocal, dim3(65536), dim3(256), 0, nullptr, (unsigned long long*)d.p, 1.0f);
+        CS_HIP(hipGetLastError());
+        unsigned long long n = 0;
+        CS_HIP(hipMemcpy(&n, d.p, sizeof n, hipMemcpyDeviceToHost));
+        *mismatches = (uint64_t)n;
+    });
+}
+
 // Launch-shape override: cells per thread of the scalar step kernel (1, 2 or
 // 4), CS_STEP_VECTOR for the vector kernel wherever rows split into vectors;
 // 0 restores the choice by lattice size.

@@ -79,6 +79,12 @@ class Context:
         call("cs_bench_copy", self.handle, int(nbytes), int(iters), C.byref(ms))
         return ms.value
 
+    def check_reciprocal(self) -> int:
+        """Inputs (of all 2^32) where the step kernels' reciprocal differs from IEEE 1/b."""
+        n = C.c_uint64()
+        call("cs_check_reciprocal", self.handle, C.byref(n))
+        return n.value
+
     def close(self):
         if self.handle:
             capi.lib.cs_ctx_destroy(self.handle)

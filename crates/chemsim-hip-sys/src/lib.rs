@@ -259,6 +259,7 @@ extern "C" {
 
     // measurement helpers
     pub fn cs_bench_copy(ctx: *mut cs_ctx, bytes: i64, iters: c_int, ms_per_copy: *mut f32) -> c_int;
+    pub fn cs_check_reciprocal(ctx: *mut cs_ctx, mismatches: *mut u64) -> c_int;
     pub fn cs_set_cells_per_thread(cells_per_thread: c_int) -> c_int;
     pub fn cs_set_step_graphs(enable: c_int) -> c_int;
     pub fn cs_set_temporal_steps(k: c_int) -> c_int;

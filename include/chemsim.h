@@ -368,6 +368,12 @@ int cs_gemm_bf16(cs_ctx* ctx, int M, int N, int K, const uint16_t* A, const uint
  * the average milliseconds per copy (HIP events).  Calibrates bandwidth and
  * the HBM byte counters. */
 int cs_bench_copy(cs_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy);
+/* No counterpart in the reference.  The FP32 step kernels compute 1 / rho
+ * with the hardware reciprocal and one FMA Newton-Raphson step where that
+ * equals the IEEE quotient; this runs every one of the 2^32 single-precision
+ * inputs through that reciprocal on the context's GPU and returns how many
+ * differ from IEEE 1.0f / b (0 expected; two NaNs count as equal). */
+int cs_check_reciprocal(cs_ctx* ctx, uint64_t* mismatches);
 /* Launch shape of the step kernel.  Both kernels give the same values: the
  * scalar kernel (one row per wave, 1, 2 or 4 cells per lane) and the vector
  * kernel (a lane owns consecutive cells of a row, one 8- or 16-byte access per

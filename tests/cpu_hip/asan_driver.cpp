@@ -419,6 +419,8 @@ int main()
         OK(cs_trt_new(0.25f, 10.0f, 1.0f, 1.0f, &tp, &tm));
         float ms = 0;
         OK(cs_bench_copy(ctx, 1 << 20, 2, &ms));
+        uint64_t mismatches = 1;
+        OK(cs_check_reciprocal(ctx, &mismatches));
     }
     try {
         host_layer();

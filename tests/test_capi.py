@@ -54,3 +54,5 @@ def test_abi_version_and_error_path():
         assert capi.lib.cs_set_temporal_steps(k) == capi.CS_OK
     k = C.c_int()
     assert capi.lib.cs_lattice_temporal_steps(None, C.byref(k)) == capi.CS_ERR_ARG
+    n = C.c_uint64()
+    assert capi.lib.cs_check_reciprocal(None, C.byref(n)) == capi.CS_ERR_ARG

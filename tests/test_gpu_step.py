@@ -113,6 +113,14 @@ def test_unfused_phases(ctx, storage):
         np.testing.assert_array_equal(lat.download(dt), O.step(op, f, flags))
 
 
+def test_reciprocal_equals_ieee_for_every_input(ctx):
+    """The kernels' 1 / rho (hardware reciprocal + one FMA Newton step on
+    normal |b| < 2^126, IEEE division elsewhere) against IEEE 1.0f / b over
+    all 2^32 bit patterns on this GPU: the exhaustive check that the FP32 and
+    deviation steps keep the reference's quotient."""
+    assert ctx.check_reciprocal() == 0
+
+
 def test_cavity_128_halfway_lid_10000_steps(ctx):
     """Lid-driven cavity 128x128, tau 0.6, lid 0.1 on row 0, half-way walls,
     f = f_eq(1, 0), 10,000 steps: f, rho and u equal the oracle's."""

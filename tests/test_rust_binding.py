@@ -14,7 +14,8 @@ HEADER = os.path.join(ROOT, "include", "chemsim.h")
 LIB_RS = os.path.join(ROOT, "crates", "chemsim-hip-sys", "src", "lib.rs")
 
 C_TO_RUST = {
-    "int": "c_int", "int32_t": "i32", "int64_t": "i64", "uint32_t": "u32", "uint16_t": "u16", "uint8_t": "u8",
+    "int": "c_int", "int32_t": "i32", "int64_t": "i64", "uint64_t": "u64", "uint32_t": "u32", "uint16_t": "u16",
+    "uint8_t": "u8",
     "size_t": "usize", "float": "f32", "double": "f64", "void": "c_void", "char": "c_char",
 }
 
