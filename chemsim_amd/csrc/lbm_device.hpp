@@ -220,6 +220,29 @@ __device__ __forceinline__ float recip_rn<float>(float b)
 }
 
 // ------------------------------------------------------------ cell update --
+// c_i . u for the nine directions, {0, ux, uy, -ux, -uy, ux + uy, -ux + uy,
+// -ux - uy, ux - uy} as the reference sums them.  The last two are written as
+// the negations of the two diagonals before them: rounding to nearest is
+// symmetric, so (-ux) - uy == -(ux + uy) and ux - uy == -((-ux) + uy) exactly
+// (an exact zero may change sign, which 1 + vc A and vc vc, the only uses,
+// absorb), and the compiler then shares vc A, vc vc and (vc vc) B between
+// the opposite diagonals: 8 fewer operations per cell update.
+template <class T>
+__device__ __forceinline__ void lattice_dots(T ux, T uy, T (&vc)[9])
+{
+#pragma clang fp contract(off)
+    const T d1 = ux + uy, d2 = -ux + uy;
+    vc[0] = T(0);
+    vc[1] = ux;
+    vc[2] = uy;
+    vc[3] = -ux;
+    vc[4] = -uy;
+    vc[5] = d1;
+    vc[6] = d2;
+    vc[7] = -d1;
+    vc[8] = -d2;
+}
+
 // Moments, equilibrium and BGK (lbm.rs:117-149, 43-71, 350-364) with the
 // reference's operation order; terms that multiply by a zero lattice
 // component are dropped, which leaves every value unchanged.
@@ -233,7 +256,8 @@ __device__ __forceinline__ void bgk_collide(const Consts<T>& k, T (&B)[9])
     const T inv = recip_rn(rho);
     const T ux = inv * mx, uy = inv * my;
     const T v2 = ux * ux + uy * uy;
-    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+    T vc[9];
+    lattice_dots(ux, uy, vc);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const T s = ((T(1) + vc[i] * k.A) + (vc[i] * vc[i]) * k.B) + v2 * k.C;
@@ -258,7 +282,8 @@ __device__ __forceinline__ void trt_collide(const Consts<T>& k, T (&B)[9])
     const T inv = recip_rn(rho);
     const T ux = inv * mx, uy = inv * my;
     const T v2 = ux * ux + uy * uy;
-    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+    T vc[9];
+    lattice_dots(ux, uy, vc);
     T feq[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -292,7 +317,8 @@ __device__ __forceinline__ void cell_equilibrium(const Consts<T>& k, const T (&B
     ux = inv * mx;
     uy = inv * my;
     const T v2 = ux * ux + uy * uy;
-    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+    T vc[9];
+    lattice_dots(ux, uy, vc);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const T s = ((T(1) + vc[i] * k.A) + (vc[i] * vc[i]) * k.B) + v2 * k.C;
@@ -396,7 +422,8 @@ __device__ __forceinline__ void bgk_collide_dev(const Consts<T>& k, T (&B)[9])
     const T inv = recip_rn(rho);
     const T ux = inv * mx, uy = inv * my;
     const T v2 = ux * ux + uy * uy;
-    const T vc[9] = {T(0), ux, uy, -ux, -uy, ux + uy, -ux + uy, -ux - uy, ux - uy};
+    T vc[9];
+    lattice_dots(ux, uy, vc);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const T s = (vc[i] * k.A + (vc[i] * vc[i]) * k.B) + v2 * k.C;
