@@ -119,11 +119,13 @@ struct cs_lattice;
 namespace cs {
 // n fused steps on the lattice's stream (cs_lattice_step without the checks).
 void do_steps(cs_lattice* l, int n);
-// Temporally blocked steps (lattice_tb.hip): steps per launch this lattice
-// takes (0: one-step kernels; cpt_override: the cs_set_cells_per_thread
-// value), and one launch of k steps (advances cur, steps and time).
+// Temporally blocked steps (lattice_tb.hip): the plan this lattice takes --
+// steps per launch K in the low byte (tb_plan_steps), the region side above
+// it (0: one-step kernels; cpt_override: the cs_set_cells_per_thread value)
+// -- and one launch of that plan (advances cur, steps and time by K).
 int tb_steps(const cs_lattice* l, int cpt_override);
-void launch_tb(cs_lattice* l, int k);
+inline int tb_plan_steps(int plan) { return plan & 0xFF; }
+void launch_tb(cs_lattice* l, int plan);
 // The render passes of the current populations into d_out (device, h*w pixels
 // of `channels` bytes: 3 = RGB, 4 = RGBA with alpha 255), enqueued on the
 // lattice's stream; allocates the lattice's render scratch on first use.
@@ -171,5 +173,5 @@ struct cs_lattice {
     // shape they were captured with (cleared when any of them changes).
     hipGraphExec_t step_graph[2] = {nullptr, nullptr};
     int graph_cpt = INT_MIN;    // cells-per-thread override at capture (INT_MIN: none captured)
-    int graph_tb = 0;           // steps per temporally blocked launch at capture
+    int graph_tb = 0;           // temporally blocked plan (tb_steps) at capture
 };

@@ -579,7 +579,8 @@ static void step_once(cs_lattice* l)
 static void advance(cs_lattice* l, int n, int k);
 
 // GRAPH_STEPS steps from the current parity, captured (first time) and
-// replayed; k: steps per temporally blocked launch (0: one per launch).
+// replayed; k: the temporally blocked plan (0: one step per launch), part of
+// the graph's key with the launch-shape override.
 static void replay_steps(cs_lattice* l, int k)
 {
     const int ov = g_cpt_override.load(std::memory_order_relaxed);
@@ -621,11 +622,12 @@ static void replay_steps(cs_lattice* l, int k)
 }
 
 // n steps from the current state: K-step launches while n >= K, then one
-// step per launch.
-static void advance(cs_lattice* l, int n, int k)
+// step per launch (plan: tb_steps's, 0 for one step per launch).
+static void advance(cs_lattice* l, int n, int plan)
 {
+    const int k = tb_plan_steps(plan);
     if (k > 0)
-        for (; n >= k; n -= k) launch_tb(l, k);
+        for (; n >= k; n -= k) launch_tb(l, plan);
     for (int s = 0; s < n; ++s) step_once(l);
 }
 
@@ -1113,7 +1115,7 @@ extern "C" int cs_lattice_temporal_steps(const cs_lattice* l, int* k)
     return guarded([&] {
         check_lat(l);
         CS_REQUIRE(k, CS_ERR_ARG, "null output");
-        const int t = tb_steps(l, g_cpt_override.load(std::memory_order_relaxed));
+        const int t = tb_plan_steps(tb_steps(l, g_cpt_override.load(std::memory_order_relaxed)));
         *k = t > 0 ? t : 1;
     });
 }

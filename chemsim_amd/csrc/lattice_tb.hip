@@ -7,20 +7,27 @@
 // K = 8 steps per launch (4096^2 FP32 Taylor-Green with the region in LDS,
 // microseconds per step: one-step vector kernel 203; K = 2 197, K = 4 137,
 // K = 6 125, K = 8 116 -- beyond 8 the halo's recomputation outgrows the
-// saved traffic): k_step_tbr (the region in registers, 88 us per step)
-// without geometry and in the reference boundary mode, k_step_tb (the region
-// in LDS) in the half-way mode, whose link and Zou-He code does not fit the
-// register variant's budget (it spills: 297 against 130 us per step on a
-// 4096^2 cavity).  They are taken when the lattice has at least
-// TB_MIN_REGIONS regions: a block updates one 64 x 64 region at one CU's
-// VALU rate (about 3-4 us per step), so a lattice of fewer regions leaves CUs
-// idle and the one-step kernels, spread over every CU and replayed from HIP
-// graphs, are as fast or faster (512^2, 121 regions: 3.9 against 3.9 us per
-// step; 128^2 cavity, 9 regions: 5.0 against 3.1).  KBC, whose one-step
-// kernel is already VALU-bound (18 divisions per cell), keeps it: 4096^2, 342
-// against 267 us per step.  cs_set_temporal_steps(4 or 8) forces the kernels
-// on every FP32 lattice, (0) switches them off; a launch-shape override from
-// cs_set_cells_per_thread selects the one-step kernels.
+// saved traffic; with the region in registers K = 6 measured slower than 8
+// too).  Lattices of more than TB_SMALL_MAX_REGIONS 32 x 32 regions (about
+// 512^2) take 64 x 64 regions: k_step_tbr (the region in registers, 73 us
+// per step at 4096^2) without geometry and in the reference boundary mode,
+// k_step_tb (the region in LDS) in the half-way mode, whose link and Zou-He
+// code does not fit the register variant's budget (it spills: 297 against
+// 130 us per step on a 4096^2 cavity).  Smaller lattices take k_step_tb on
+// 32 x 32 regions (16 x 16 interior): a block updates its region at one CU's
+// rate, so 64 x 64 regions would leave most CUs idle on them (128^2 cavity,
+// 9 regions: 5.0 us per step), while 32 x 32 regions cost a CU a quarter of
+// the cells per step and spread over four times as many CUs.  Measured, us
+// per step (cavity with half-way walls / periodic Taylor-Green): 128^2 1.64 /
+// 1.15 against 3.02 / 2.42 for the one-step kernels replayed from HIP
+// graphs; 512^2 4.00 / 2.81 against 4.68 / 3.48 with 64 x 64 regions and
+// 4.91 / 3.81 one-step; 576^2 (1296 small regions) 4.76 / 3.93 against
+// 4.85 / 3.56 with 64 x 64 regions, where those take over.  KBC, whose
+// one-step kernel is already VALU-bound (18 divisions per cell), keeps it:
+// 4096^2, 342 against 267 us per step.  cs_set_temporal_steps(4 or 8)
+// forces the 64 x 64 kernels on every FP32 lattice, (0) switches temporal
+// blocking off; a launch-shape override from cs_set_cells_per_thread selects
+// the one-step kernels.
 #include <atomic>
 #include <type_traits>
 
@@ -32,7 +39,7 @@ using namespace cs;
 namespace {
 
 std::atomic<int> g_tb_steps{CS_TEMPORAL_AUTO};
-constexpr int64_t TB_MIN_REGIONS = 192;  // 2/3 of a region per CU of MI355X's 256
+constexpr int64_t TB_SMALL_MAX_REGIONS = 1024;  // 32 x 32 regions: four per CU of MI355X's 256
 
 template <class St>
 const Consts<float>& consts32(const cs_lattice* l)
@@ -40,10 +47,10 @@ const Consts<float>& consts32(const cs_lattice* l)
     return l->k32;
 }
 
-template <class St, int EDGE, int MODE, int K, int COLL>
+template <class St, int EDGE, int MODE, int K, int COLL, int RE>
 void launch_k(cs_lattice* l)
 {
-    constexpr int IN = TB_EDGE - 2 * K;
+    constexpr int IN = RE - 2 * K;
     StepArgs<float> a{};
     a.fin = l->buf[l->cur];
     a.fout = l->buf[l->cur ^ 1];
@@ -56,8 +63,8 @@ void launch_k(cs_lattice* l)
     a.tiles_x = (l->w + IN - 1) / IN;
     a.k = consts32<St>(l);
     const int tiles_y = (l->h + IN - 1) / IN;
-    if constexpr (MODE == MODE_HALF)
-        hipLaunchKernelGGL((k_step_tb<St, EDGE, MODE, K, COLL>), dim3((unsigned)(a.tiles_x * tiles_y)),
+    if constexpr (MODE == MODE_HALF || RE == TB_EDGE_SMALL)
+        hipLaunchKernelGGL((k_step_tb<St, EDGE, MODE, K, COLL, RE>), dim3((unsigned)(a.tiles_x * tiles_y)),
                            dim3(TB_THREADS), 0, l->stream, a);
     else
         hipLaunchKernelGGL((k_step_tbr<St, EDGE, MODE, K, COLL>), dim3((unsigned)(a.tiles_x * tiles_y)),
@@ -67,41 +74,49 @@ void launch_k(cs_lattice* l)
     for (int s = 0; s < K; ++s) l->time += l->params.delta_t;  // as step by step (lbm.rs:713)
 }
 
-template <class St, int EDGE, int MODE, int K>
+template <class St, int EDGE, int MODE, int K, int RE>
 void launch_coll(cs_lattice* l)
 {
     if constexpr (std::is_same_v<St, StF32> && K == 8) {
         switch (l->params.collision) {
-        case CS_COLL_TRT: launch_k<St, EDGE, MODE, K, CS_COLL_TRT>(l); return;
-        case CS_COLL_REGULARIZED: launch_k<St, EDGE, MODE, K, CS_COLL_REGULARIZED>(l); return;
-        case CS_COLL_KBC: launch_k<St, EDGE, MODE, K, CS_COLL_KBC>(l); return;
+        case CS_COLL_TRT: launch_k<St, EDGE, MODE, K, CS_COLL_TRT, RE>(l); return;
+        case CS_COLL_REGULARIZED: launch_k<St, EDGE, MODE, K, CS_COLL_REGULARIZED, RE>(l); return;
+        case CS_COLL_KBC:
+            if constexpr (RE == TB_EDGE) {
+                launch_k<St, EDGE, MODE, K, CS_COLL_KBC, RE>(l);
+                return;
+            }
+            break;
         default: break;
         }
     }
-    launch_k<St, EDGE, MODE, K, CS_COLL_BGK>(l);
+    launch_k<St, EDGE, MODE, K, CS_COLL_BGK, RE>(l);
 }
 
 template <class St, int EDGE, int MODE>
-void launch_steps(cs_lattice* l, int k)
+void launch_steps(cs_lattice* l, int plan)
 {
-    if (k == 4) launch_coll<St, EDGE, MODE, 4>(l);
-    else launch_coll<St, EDGE, MODE, 8>(l);
+    if ((plan >> 8) == TB_EDGE_SMALL) launch_coll<St, EDGE, MODE, 8, TB_EDGE_SMALL>(l);
+    else if (tb_plan_steps(plan) == 4) launch_coll<St, EDGE, MODE, 4, TB_EDGE>(l);
+    else launch_coll<St, EDGE, MODE, 8, TB_EDGE>(l);
 }
 
 template <class St, int EDGE>
-void launch_mode(cs_lattice* l, int k)
+void launch_mode(cs_lattice* l, int plan)
 {
-    if (!l->has_flags) launch_steps<St, EDGE, MODE_FLUID>(l, k);
-    else if (l->params.boundary == CS_BB_HALFWAY) launch_steps<St, EDGE, MODE_HALF>(l, k);
-    else launch_steps<St, EDGE, MODE_REF>(l, k);
+    if (!l->has_flags) launch_steps<St, EDGE, MODE_FLUID>(l, plan);
+    else if (l->params.boundary == CS_BB_HALFWAY) launch_steps<St, EDGE, MODE_HALF>(l, plan);
+    else launch_steps<St, EDGE, MODE_REF>(l, plan);
 }
 
 template <class St>
-void launch_edge(cs_lattice* l, int k)
+void launch_edge(cs_lattice* l, int plan)
 {
-    if (l->params.edge == CS_EDGE_PERIODIC) launch_mode<St, CS_EDGE_PERIODIC>(l, k);
-    else launch_mode<St, CS_EDGE_ZERO>(l, k);
+    if (l->params.edge == CS_EDGE_PERIODIC) launch_mode<St, CS_EDGE_PERIODIC>(l, plan);
+    else launch_mode<St, CS_EDGE_ZERO>(l, plan);
 }
+
+constexpr int tb_plan(int k, int edge) { return k | (edge << 8); }
 
 }  // namespace
 
@@ -114,20 +129,20 @@ int tb_steps(const cs_lattice* l, int cpt_override)
     if (l->storage != CS_F32 && l->storage != CS_F32DEV) return 0;
     if ((int64_t)l->h * l->pitch * 4 >= ((int64_t)1 << 32)) return 0;  // 32-bit offsets
     if (k == CS_TEMPORAL_AUTO) {
-        constexpr int IN = TB_EDGE - 2 * 8;
-        const int64_t regions = (int64_t)((l->w + IN - 1) / IN) * ((l->h + IN - 1) / IN);
-        if (regions < TB_MIN_REGIONS || l->params.collision == CS_COLL_KBC) return 0;
-        return 8;
+        if (l->params.collision == CS_COLL_KBC) return 0;
+        constexpr int INS = TB_EDGE_SMALL - 2 * 8;
+        const int64_t small = (int64_t)((l->w + INS - 1) / INS) * ((l->h + INS - 1) / INS);
+        return tb_plan(8, small <= TB_SMALL_MAX_REGIONS ? TB_EDGE_SMALL : TB_EDGE);
     }
     // K = 4 is kept for BGK; the other operators run 8 steps per launch
-    if (k == 4 && l->params.collision != CS_COLL_BGK) return 8;
-    return k;
+    if (k == 4 && l->params.collision != CS_COLL_BGK) return tb_plan(8, TB_EDGE);
+    return tb_plan(k, TB_EDGE);
 }
 
-void launch_tb(cs_lattice* l, int k)
+void launch_tb(cs_lattice* l, int plan)
 {
-    if (l->storage == CS_F32) launch_edge<StF32>(l, k);
-    else launch_edge<StF32Dev>(l, k);
+    if (l->storage == CS_F32) launch_edge<StF32>(l, plan);
+    else launch_edge<StF32Dev>(l, plan);
 }
 
 }  // namespace cs

@@ -35,6 +35,10 @@ namespace cs {
 constexpr int TB_EDGE = 64;        // region side (cells); lane l owns region column l
 constexpr int TB_THREADS = 1024;   // 16 waves; wave w owns region rows w, w + 16, w + 32, w + 48
 constexpr int TB_SLOTS = TB_EDGE * TB_EDGE / TB_THREADS;
+// Small lattices: 32 x 32 regions, one cell per thread (a wave holds two
+// region rows), so a lattice of a few dozen regions still spreads over as
+// many CUs and each step costs a CU 1024 cells instead of 4096.
+constexpr int TB_EDGE_SMALL = 32;
 
 // Collision of one fluid cell: the operator finish() applies.
 template <class St, int COLL>
@@ -56,8 +60,8 @@ __device__ __forceinline__ void collide_cell(const Consts<typename St::T>& k, ty
 // cell of the wave's row is solid (reference mode) or has a link or an
 // inlet/outlet flag (half-way mode); rows without one -- most rows of a
 // lattice -- take the plain pulls, nine LDS reads per cell.
-template <class St, int MODE>
-__device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][TB_EDGE][TB_EDGE], int rr, int lane,
+template <class St, int MODE, int RE>
+__device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][RE][RE], int rr, int lane,
                                           uint32_t info, bool special, const Consts<typename St::T>& k,
                                           typename St::T (&B)[9])
 {
@@ -65,7 +69,7 @@ __device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][TB_EDGE][
     const int type = MODE == MODE_FLUID ? (int)CS_FLUID : (int)(info >> NODE_TYPE_SHIFT);
     const bool solid = type == CS_WALL || type == CS_LID;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) B[i] = L[i][rr + CX[i]][(lane - CY[i]) & (TB_EDGE - 1)];
+    for (int i = 0; i < 9; ++i) B[i] = L[i][rr + CX[i]][(lane - CY[i]) & (RE - 1)];
     if (!special) return;  // no cell of the wave's row needs more than the plain pulls
     if (MODE == MODE_REF) {
 #pragma unroll
@@ -98,24 +102,30 @@ __device__ __forceinline__ void tb_gather(const typename St::T (&L)[9][TB_EDGE][
     }
 }
 
-// K steps on the region of tile (ty, tx) of the lattice `a` describes: rows
-// ty*(64-2K) - K .., columns tx*(64-2K) - K ..; blocks of TB_THREADS threads,
-// one block per CU (the region takes 144 KiB of LDS).  A wave's LDS accesses
-// are 64 consecutive words of one region row (a pull from column -1 or 64
-// wraps within the row and belongs to a lane that discards its result), so
+// K steps on the region of tile (ty, tx) of the lattice `a` describes, a
+// region of RE x RE cells (RE = 64 or 32): rows ty*(RE-2K) - K .., columns
+// tx*(RE-2K) - K ..; blocks of TB_THREADS threads (RE = 64: one block per CU,
+// the region takes 144 KiB of LDS; RE = 32: 36 KiB).  Thread t owns region
+// column t % RE of rows (t / RE) + (1024 / RE) j.  A wave's LDS accesses are
+// 64 consecutive words (one row, or two at RE = 32; a pull from column -1 or
+// RE wraps within its row and belongs to a lane that discards its result), so
 // they are free of bank conflicts, and each lane's nine pull addresses per row
-// slot stay the same from step to step.
-template <class St, int EDGE, int MODE, int K, int COLL>
+// slot stay the same from step to step.  At RE = 64 a row is one wave, so the
+// row tests below are uniform over the wave; at RE = 32 they mask lanes.
+template <class St, int EDGE, int MODE, int K, int COLL, int RE>
 __device__ __forceinline__ void tb_region(const StepArgs<typename St::T>& a, int ty, int tx,
-                                          typename St::T (&L)[9][TB_EDGE][TB_EDGE])
+                                          typename St::T (&L)[9][RE][RE])
 {
     using S = typename St::S;
     using T = typename St::T;
     static_assert(sizeof(S) == 4 && sizeof(T) == 4, "32-bit storage and arithmetic");
-    static_assert(K >= 1 && 2 * K < TB_EDGE, "halo");
-    constexpr int IN = TB_EDGE - 2 * K;  // interior side
+    static_assert(RE == 64 || RE == 32, "region side");
+    static_assert(K >= 1 && 2 * K < RE, "halo");
+    constexpr int IN = RE - 2 * K;                   // interior side
+    constexpr int SLOTS = RE * RE / TB_THREADS;      // rows per thread
+    constexpr int RSTRIDE = TB_THREADS / RE;         // region rows between a thread's slots
     const int r0 = ty * IN - K, c0 = tx * IN - K;
-    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int lane = (int)(threadIdx.x & (RE - 1)), row0 = (int)(threadIdx.x / RE);
     const S* __restrict__ fin = (const S*)a.fin;
     S* __restrict__ fout = (S*)a.fout;
     const uint32_t P = (uint32_t)a.plane;
@@ -129,12 +139,12 @@ __device__ __forceinline__ void tb_region(const StepArgs<typename St::T>& a, int
         cin = gc >= 0 && gc < a.w;
         gc = cin ? gc : 0;
     }
-    bool upd[TB_SLOTS];       // the cell is inside the lattice and not carried
-    bool special[TB_SLOTS];   // the wave's row needs the geometry pulls (uniform over the wave)
-    uint32_t info[TB_SLOTS];  // node words (geometry modes)
+    bool upd[SLOTS];       // the cell is inside the lattice and not carried
+    bool special[SLOTS];   // the wave's row(s) need the geometry pulls (uniform over the wave)
+    uint32_t info[SLOTS];  // node words (geometry modes)
 #pragma unroll
-    for (int j = 0; j < TB_SLOTS; ++j) {
-        const int rr = wave + 16 * j;
+    for (int j = 0; j < SLOTS; ++j) {
+        const int rr = row0 + RSTRIDE * j;
         int gr = r0 + rr;
         bool rin = true;
         if (EDGE == CS_EDGE_PERIODIC) {
@@ -163,21 +173,21 @@ __device__ __forceinline__ void tb_region(const StepArgs<typename St::T>& a, int
 #pragma unroll 1
     for (int s = 1; s <= K; ++s) {
         __syncthreads();  // step s - 1 fully in LDS
-        T B[TB_SLOTS][9];
+        T B[SLOTS][9];
 #pragma unroll
-        for (int j = 0; j < TB_SLOTS; ++j) {
-            const int rr = wave + 16 * j;
-            if (rr >= s && rr < TB_EDGE - s) {  // uniform over the wave
-                tb_gather<St, MODE>(L, rr, lane, info[j], special[j], a.k, B[j]);
+        for (int j = 0; j < SLOTS; ++j) {
+            const int rr = row0 + RSTRIDE * j;
+            if (rr >= s && rr < RE - s) {  // uniform over the wave at RE = 64
+                tb_gather<St, MODE, RE>(L, rr, lane, info[j], special[j], a.k, B[j]);
                 collide_cell<St, COLL>(a.k, B[j]);
             }
         }
         __syncthreads();  // every read of step s done before any write
-        const bool cols = lane >= s && lane < TB_EDGE - s;
+        const bool cols = lane >= s && lane < RE - s;
 #pragma unroll
-        for (int j = 0; j < TB_SLOTS; ++j) {
-            const int rr = wave + 16 * j;
-            if (rr >= s && rr < TB_EDGE - s && cols && upd[j]) {
+        for (int j = 0; j < SLOTS; ++j) {
+            const int rr = row0 + RSTRIDE * j;
+            if (rr >= s && rr < RE - s && cols && upd[j]) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) L[i][rr][lane] = B[j][i];
             }
@@ -187,12 +197,12 @@ __device__ __forceinline__ void tb_region(const StepArgs<typename St::T>& a, int
 
     // interior: rows ty*IN .. and columns tx*IN .. of the lattice (no wrap)
     const int oc = c0 + lane;
-    const bool ocol = lane >= K && lane < TB_EDGE - K && oc < a.w;
+    const bool ocol = lane >= K && lane < RE - K && oc < a.w;
 #pragma unroll
-    for (int j = 0; j < TB_SLOTS; ++j) {
-        const int rr = wave + 16 * j;
+    for (int j = 0; j < SLOTS; ++j) {
+        const int rr = row0 + RSTRIDE * j;
         const int orow = r0 + rr;
-        if (ocol && rr >= K && rr < TB_EDGE - K && orow < a.h) {
+        if (ocol && rr >= K && rr < RE - K && orow < a.h) {
             const uint32_t o = (uint32_t)orow * (uint32_t)a.pitch + (uint32_t)oc;
 #pragma unroll
             for (int i = 0; i < 9; ++i) St::st(at<uint32_t>(fout, (uint32_t)i * P + o), 0, L[i][rr][lane]);
@@ -381,12 +391,12 @@ __global__ __launch_bounds__(TBR_THREADS, 4) void k_step_tbr(StepArgs<typename S
     tbr_region<St, EDGE, MODE, K, COLL>(a, ty, tile - ty * a.tiles_x, X);
 }
 
-// One lattice: tile t of the lattice's ceil(w / (64-2K)) x ceil(h / (64-2K))
-// regions (a.tiles_x = ceil(w / (64-2K))).
-template <class St, int EDGE, int MODE, int K, int COLL>
+// One lattice: tile t of the lattice's ceil(w / (RE-2K)) x ceil(h / (RE-2K))
+// regions of RE x RE cells (a.tiles_x = ceil(w / (RE-2K))).
+template <class St, int EDGE, int MODE, int K, int COLL, int RE>
 __global__ __launch_bounds__(TB_THREADS) void k_step_tb(StepArgs<typename St::T> a)
 {
-    __shared__ typename St::T L[9][TB_EDGE][TB_EDGE];
+    __shared__ typename St::T L[9][RE][RE];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / a.tiles_x;
     tb_region<St, EDGE, MODE, K, COLL>(a, ty, tile - ty * a.tiles_x, L);

@@ -21,7 +21,8 @@ Workloads:
               written once per launch + 4 B/cell of node words) and the
               one-step geometry kernel (72 B/cell + 4 B/cell of node words)
   cavity128   lid-driven cavity 128^2 FP32 (16,384 cells), launches one by one
-              (step graphs off): the launch-bound case
+              (step graphs off): the temporally blocked kernel on 32 x 32
+              regions (8 steps per launch, 64 regions) and the one-step kernel
   render      render pass of the 4096^2 Taylor-Green state in the four display
               modes (render.rs:7-178): field pass (36 B/cell read, 4 or 8 B/cell
               written), statistics, pixel pass (4 or 8 B/cell read, 3 B/pixel)
@@ -112,12 +113,12 @@ def _cavity(ctx, n):
     return lat
 
 
-def tb_bytes_per_launch(w: int, h: int, k: int = 8) -> int:
+def tb_bytes_per_launch(w: int, h: int, k: int = 8, edge: int = 64) -> int:
     """Bytes one launch of the temporally blocked kernel (k steps) must move
-    for a w x h FP32 lattice: every 64 x 64 region read once, every cell
+    for a w x h FP32 lattice: every edge x edge region read once, every cell
     written once, 36 B per cell."""
-    inner = 64 - 2 * k
-    return (-(-w // inner) * -(-h // inner) * 64 * 64 + w * h) * 36
+    inner = edge - 2 * k
+    return (-(-w // inner) * -(-h // inner) * edge * edge + w * h) * 36
 
 
 def cavity4096(ctx, steps):
@@ -148,10 +149,18 @@ def cavity128(ctx, steps):
     try:
         _calibrate(ctx)
         with _cavity(ctx, n) as lat:
+            k = lat.temporal_steps()
+            us_tb = _timed(lat, max(k, steps // k * k))
+        capi.call("cs_set_temporal_steps", 0)
+        with _cavity(ctx, n) as lat:
             us = _timed(lat, steps)
     finally:
+        capi.call("cs_set_temporal_steps", capi.CS_TEMPORAL_AUTO)
         capi.call("cs_set_step_graphs", 1)
-    _emit({"workload": "cavity128", "kernel": "k_step", "cells": n * n, "alg_bytes_per_launch": 72 * n * n,
+    _emit({"workload": "cavity128/temporal", "kernel": "k_step_tb (32 x 32 regions)", "cells": n * n,
+           "alg_bytes_per_launch": tb_bytes_per_launch(n, n, k, 32), "steps_per_launch": k,
+           "event_us_per_step": round(us_tb, 2)})
+    _emit({"workload": "cavity128/one_step", "kernel": "k_step", "cells": n * n, "alg_bytes_per_launch": 72 * n * n,
            "event_us_per_step": round(us, 2)})
 
 

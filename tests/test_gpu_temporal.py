@@ -1,10 +1,11 @@
-"""The temporally blocked step (k_step_tb: K steps per launch out of LDS)
+"""The temporally blocked steps (K steps per launch: k_step_tbr with 64 x 64
+regions in registers, k_step_tb with 64 x 64 or 32 x 32 regions in LDS)
 against the CPU oracle, value for value, for FP32 and FP32 deviation
 storage: zero and periodic edges, every collision operator, geometry (the
 reference's on-node swap, half-way walls with a moving lid, Zou-He inlet and
-outlet), lattices smaller than one 64 x 64 region and ragged ones, step
-counts that leave a remainder for the one-step kernels, graph replay, and a
-full-size lattice against the one-step kernels."""
+outlet), lattices smaller than one region and ragged ones, step counts that
+leave a remainder for the one-step kernels, graph replay, and a full-size
+lattice against the one-step kernels."""
 import numpy as np
 import pytest
 
@@ -38,11 +39,14 @@ def gpu_params(op: O.Params) -> Params:
                   ks_viscosity=op.ks_viscosity)
 
 
-@pytest.fixture(params=[4, 8])
+@pytest.fixture(params=[4, 8, "small"])
 def steps_per_launch(request):
-    """Forces the temporally blocked kernel (the built-in choice leaves
-    lattices of fewer than 192 regions to the one-step kernels)."""
-    capi.call("cs_set_temporal_steps", request.param)
+    """4 or 8: forces the 64 x 64 temporally blocked kernels; "small": the
+    built-in choice, which takes 8 steps on 32 x 32 regions for lattices of
+    at most 1024 of them -- every lattice of these tests (KBC keeps the
+    one-step kernels)."""
+    k = capi.CS_TEMPORAL_AUTO if request.param == "small" else request.param
+    capi.call("cs_set_temporal_steps", k)
     try:
         yield request.param
     finally:
@@ -219,12 +223,12 @@ def test_cavity_graph_replay(ctx, steps_per_launch):
         np.testing.assert_array_equal(lat.download(), O.run(op, f0, 64 * 4 + 7, flags))
 
 
-def test_built_in_choice(ctx):
-    """The built-in choice: large FP32 lattices (here 1000 x 700, 315
-    regions) take the temporally blocked kernel and equal the one-step
-    kernels bit for bit; KBC and small lattices keep the one-step kernels
-    (all three agree with each other whatever is taken)."""
-    w, h = 1000, 700
+@pytest.mark.parametrize("shape", [(1000, 700), (512, 512), (520, 520)], ids=["1000x700", "512x512", "520x520"])
+def test_built_in_choice(ctx, shape):
+    """The built-in choice: 1000 x 700 and 520 x 520 (more than 1024 regions
+    of 32 x 32) take 64 x 64 regions in registers, 512 x 512 (1024) takes
+    32 x 32 regions in LDS; each equals the one-step kernels bit for bit."""
+    w, h = shape
     rng = np.random.default_rng(48)
     f = rand_state(rng, h, w)
     op = O.params(w, h, tau=0.8, edge=O.EDGE_PERIODIC)
@@ -235,6 +239,7 @@ def test_built_in_choice(ctx):
             with Lattice(ctx, w, h, "f32", gpu_params(op)) as lat:
                 lat.upload(f)
                 lat.step(24)
+                assert lat.temporal_steps() == (8 if k == capi.CS_TEMPORAL_AUTO else 1)
                 out[k] = lat.download()
     finally:
         capi.call("cs_set_temporal_steps", capi.CS_TEMPORAL_AUTO)
@@ -251,7 +256,7 @@ def test_temporal_steps_query(ctx):
         with Lattice(ctx, 1000, 700, "f32", big) as a, Lattice(ctx, 1000, 700, "f32", kbc) as b, \
                 Lattice(ctx, 128, 128, "f32", cav, W.cavity_flags(128, 128)) as c, \
                 Lattice(ctx, 1000, 700, "f64", big) as d, Lattice(ctx, 1000, 700, "f32dev", big) as e:
-            assert [x.temporal_steps() for x in (a, b, c, d, e)] == [8, 1, 1, 1, 8]
+            assert [x.temporal_steps() for x in (a, b, c, d, e)] == [8, 1, 8, 1, 8]
             capi.call("cs_set_temporal_steps", 4)
             assert [x.temporal_steps() for x in (a, b, c, d, e)] == [4, 8, 4, 1, 4]
             capi.call("cs_set_temporal_steps", 0)

@@ -16,5 +16,6 @@ def test_unknown_workload_is_an_error():
 def test_small_workloads_run(capsys):
     assert K.main(["cavity128", "--steps", "3"]) == 0
     recs = [json.loads(line) for line in capsys.readouterr().out.splitlines() if line.startswith("{")]
-    assert [r["workload"] for r in recs] == ["cavity128"]
-    assert recs[0]["cells"] == 128 * 128 and recs[0]["event_us_per_step"] > 0
+    assert [r["workload"] for r in recs] == ["cavity128/temporal", "cavity128/one_step"]
+    assert recs[0]["steps_per_launch"] == 8
+    assert all(r["cells"] == 128 * 128 and r["event_us_per_step"] > 0 for r in recs)
