@@ -393,12 +393,13 @@ int cs_set_cells_per_thread(int cells_per_thread);
  * with stochastic rounding always launches step by step.  Process-wide; for
  * tests and tuning.  No reference counterpart. */
 int cs_set_step_graphs(int enable);
-/* Steps per launch of the temporally blocked step kernel: CS_TEMPORAL_AUTO
- * (the default: 8 steps for FP32 and FP32-deviation lattices of at least
- * 192 regions of 48 x 48 cells -- about 600^2 -- except with the KBC
- * operator), 4 or 8 (every FP32 and FP32-deviation lattice), or 0 for one
- * step per launch.  Results are identical either way.  A launch-shape
- * override of cs_set_cells_per_thread selects the one-step kernels instead.
+/* Steps per launch of the temporally blocked step kernels: CS_TEMPORAL_AUTO
+ * (the default: 8 steps for every FP32 and FP32-deviation lattice except
+ * with the KBC operator -- on 32 x 32 regions for lattices of at most 1024
+ * of them, about 512^2, on 64 x 64 regions above), 4 or 8 (every FP32 and
+ * FP32-deviation lattice, on 64 x 64 regions), or 0 for one step per launch.
+ * Results are identical either way.  A launch-shape override of
+ * cs_set_cells_per_thread selects the one-step kernels instead.
  * Process-wide; for tests and tuning.  No reference counterpart. */
 enum { CS_TEMPORAL_AUTO = -1 };
 int cs_set_temporal_steps(int k);
