@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <functional>
 #include <numeric>
@@ -323,15 +324,23 @@ __global__ __launch_bounds__(TBR_THREADS, 4) void k_batch_tbr(const BatchLat<typ
 }
 
 // Grid-wide barrier of the persistent temporal kind: a monotonic arrival
-// counter (round r waits for (r + 1) x blocks arrivals) with agent-scope
+// counter (barrier r waits for (r + 1) x blocks arrivals) with agent-scope
 // release/acquire ordering, so every block's stores of a round are visible to
 // every block (on any XCD) after it.  The wait is bounded: past about a
 // second it raises `abort`, every block leaves at its next barrier and the
-// host reports the timeout -- the grid always drains.
+// host reports the timeout -- the grid always drains.  `next` hands out the
+// work: draw i is region i % nregions of round i / nregions.  The host clears
+// `next` and `count` before each launch and `abort` once per call.
 struct GridBarrier {
+    unsigned long long next;
     unsigned count;
     unsigned abort;
 };
+constexpr size_t GRID_BARRIER_PER_LAUNCH = offsetof(GridBarrier, abort);  // bytes cleared per launch
+// Rounds per persistent launch (32768 steps): the arrival count stays far
+// below 2^32 (4096 rounds x at most 1024 blocks); a longer call takes one
+// launch per 4096 rounds, which costs nothing measurable.
+constexpr int GRID_MAX_ROUNDS = 4096;
 
 __device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned target)
 {
@@ -357,34 +366,48 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned target)
     return s_ok != 0;
 }
 
-// Region range of block b of a persistent grid: contiguous shares, so the
-// neighbouring regions of a lattice mostly run in one block.
-__device__ __forceinline__ void grid_share(int nregions, int& r0, int& r1)
+// Next piece of work of a persistent grid: the block's draw from `next`
+// (uniform over the block) as (round, region), after the block has passed
+// every barrier below that round -- it has finished its regions of the
+// earlier rounds, and barrier r lets nobody into round r + 1 before every
+// block has drawn past round r, i.e. before all of round r is done.  A draw
+// past the last round still passes the remaining barriers (every block
+// arrives at each exactly once) and returns false, as does an aborted
+// barrier.  `round` is the block's current round (0 at the start).
+__device__ __forceinline__ bool grid_next(GridBarrier* bar, int nregions, int rounds, int& round, int& region)
 {
-    r0 = (int)((int64_t)nregions * blockIdx.x / gridDim.x);
-    r1 = (int)((int64_t)nregions * (blockIdx.x + 1) / gridDim.x);
+    __shared__ unsigned long long s_draw;
+    if (threadIdx.x == 0) s_draw = atomicAdd(&bar->next, 1ull);
+    __syncthreads();
+    const unsigned long long draw = s_draw;
+    __syncthreads();  // every thread has the draw before thread 0 takes the next one
+    const long long q = (long long)(draw / (unsigned long long)nregions);
+    const int upto = (int)(q < rounds ? q : rounds - 1);
+    for (; round < upto; ++round)
+        if (!grid_barrier(bar, (unsigned)(round + 1) * gridDim.x)) return false;
+    if (q >= rounds) return false;
+    region = (int)(draw - (unsigned long long)q * (unsigned long long)nregions);
+    return true;
 }
 
+// Rounds round0 .. round0 + rounds - 1 of the call (round r flips the
+// buffers r times after the state upload), regions drawn one at a time.
 template <class St, int EDGE, int MODE, int K>
 __global__ __launch_bounds__(TB_THREADS) void k_batch_tb_grid(const BatchLat<typename St::T>* __restrict__ lats,
                                                            const TileRef* __restrict__ regions,
                                                            void* __restrict__ arena,
                                                            const uint32_t* __restrict__ nodes,
                                                            const LatState* __restrict__ state, int nregions,
-                                                           int rounds, GridBarrier* bar)
+                                                           int round0, int rounds, GridBarrier* bar)
 {
     __shared__ typename St::T L[9][TB_EDGE][TB_EDGE];
-    int r0, r1;
-    grid_share(nregions, r0, r1);
-    for (int round = 0; round < rounds; ++round) {
-        for (int r = r0; r < r1; ++r) {
-            StepArgs<typename St::T> a{};
-            int ty, tx;
-            batch_tb_region<St, EDGE, MODE, K>(lats, regions + r, arena, nodes, state, round, a, ty, tx, true);
-            tb_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, L);
-            __syncthreads();  // the region's interior stores have read the LDS before the next load
-        }
-        if (!grid_barrier(bar, (unsigned)((round + 1) * gridDim.x))) return;
+    int round = 0, r = 0;
+    while (grid_next(bar, nregions, rounds, round, r)) {
+        StepArgs<typename St::T> a{};
+        int ty, tx;
+        batch_tb_region<St, EDGE, MODE, K>(lats, regions + r, arena, nodes, state, round0 + round, a, ty, tx, true);
+        tb_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, L);
+        __syncthreads();  // the region's interior stores have read the LDS before the next load
     }
 }
 
@@ -394,20 +417,16 @@ __global__ __launch_bounds__(TBR_THREADS, 4) void k_batch_tbr_grid(const BatchLa
                                                                 void* __restrict__ arena,
                                                                 const uint32_t* __restrict__ nodes,
                                                                 const LatState* __restrict__ state, int nregions,
-                                                                int rounds, GridBarrier* bar)
+                                                                int round0, int rounds, GridBarrier* bar)
 {
     __shared__ typename St::T X[2][TBR_WAVES][2][3][TB_EDGE];
-    int r0, r1;
-    grid_share(nregions, r0, r1);
-    for (int round = 0; round < rounds; ++round) {
-        for (int r = r0; r < r1; ++r) {
-            StepArgs<typename St::T> a{};
-            int ty, tx;
-            batch_tb_region<St, EDGE, MODE, K>(lats, regions + r, arena, nodes, state, round, a, ty, tx, true);
-            tbr_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, X);
-            __syncthreads();  // every wave has read the exchange buffers of the region's last step
-        }
-        if (!grid_barrier(bar, (unsigned)((round + 1) * gridDim.x))) return;
+    int round = 0, r = 0;
+    while (grid_next(bar, nregions, rounds, round, r)) {
+        StepArgs<typename St::T> a{};
+        int ty, tx;
+        batch_tb_region<St, EDGE, MODE, K>(lats, regions + r, arena, nodes, state, round0 + round, a, ty, tx, true);
+        tbr_region<St, EDGE, MODE, K, CS_COLL_BGK>(a, ty, tx, X);
+        __syncthreads();  // every wave has read the exchange buffers of the region's last step
     }
 }
 
@@ -745,7 +764,7 @@ static bool batch_tb_storage(int storage) { return storage == CS_F32 || storage 
 // grid is as many blocks as can be resident at once (the launch fails rather
 // than start a grid that could not meet at its barriers).
 template <class St>
-static void launch_tb_grid_group(cs_batch* b, int group, int first, int count, int rounds)
+static void launch_tb_grid_group(cs_batch* b, int group, int first, int count, int round0, int rounds)
 {
     if constexpr (sizeof(typename St::S) != 4) {
         throw Error{CS_ERR_UNSUPPORTED, "the temporal launch kinds need 32-bit storage"};
@@ -767,8 +786,8 @@ static void launch_tb_grid_group(cs_batch* b, int group, int first, int count, i
             CS_REQUIRE(per_cu >= 1, CS_ERR_UNSUPPORTED, "the persistent temporal kernel does not fit a CU");
             const int grid = std::min(count, per_cu * b->ctx->prop.multiProcessorCount);
             void* args[] = {(void*)&lats, (void*)&regions, (void*)&arena, (void*)&nodes, (void*)&state,
-                            (void*)&nregions, (void*)&rounds, (void*)&bar};
-            CS_HIP(hipMemsetAsync(bar, 0, sizeof(GridBarrier), b->stream));
+                            (void*)&nregions, (void*)&round0, (void*)&rounds, (void*)&bar};
+            CS_HIP(hipMemsetAsync(bar, 0, GRID_BARRIER_PER_LAUNCH, b->stream));
             CS_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3((unsigned)threads), args, 0, b->stream));
         };
         auto go_mode = [&]<int EDGE>() {
@@ -799,11 +818,15 @@ static void launch_batch(cs_batch* b, int steps)
             const auto rg = upload_regions(b, groups);
             const int rounds = steps / BATCH_TB_STEPS;
             if (kind == CS_LAUNCH_TEMPORAL_PERSISTENT && rg.size() == 1) {
-                // every round of every lattice in one launch (one mode group)
+                // every round of every lattice in one launch (one mode group;
+                // GRID_MAX_ROUNDS rounds per launch for very long calls)
                 const auto& g = rg[0];
-                if (g[2] > 0)
-                    dispatch_batch_storage(b->storage,
-                                           [&]<class St>() { launch_tb_grid_group<St>(b, g[0], g[1], g[2], rounds); });
+                CS_HIP(hipMemsetAsync(b->d_bar, 0, sizeof(GridBarrier), b->stream));
+                for (int r0 = 0; r0 < rounds && g[2] > 0; r0 += GRID_MAX_ROUNDS) {
+                    const int nr = std::min(GRID_MAX_ROUNDS, rounds - r0);
+                    dispatch_batch_storage(
+                        b->storage, [&]<class St>() { launch_tb_grid_group<St>(b, g[0], g[1], g[2], r0, nr); });
+                }
                 s += rounds * BATCH_TB_STEPS;
                 flips += rounds;
             } else {

@@ -216,6 +216,29 @@ def test_batch_tiny_lattices_match_oracle(ctx, launch, threads):
             np.testing.assert_array_equal(b.download(i), O.run(op, f0, 13, flags), err_msg=f"lattice {i} {shapes[i]}")
 
 
+def test_persistent_temporal_long_call(ctx):
+    """A call of more than 4096 rounds of 8 steps (32,811 steps): two
+    persistent launches, the second continuing the first's round count (the
+    buffer parity of each lattice), then 3 tile launches; against the
+    temporal kind's one launch per round, bit for bit."""
+    shapes = [(20, 20), (9, 31), (1, 1)]
+    def make(kind):
+        b = Batch(ctx, shapes, "f32")
+        for i, (w, h) in enumerate(shapes):
+            b.set_params(i, Params(tau=0.7 + 0.05 * i, edge=capi.CS_EDGE_PERIODIC))
+            r = np.random.default_rng(300 + i)
+            b.init_equilibrium(i, (1 + 0.01 * r.standard_normal((h, w))).astype(np.float32),
+                               (0.02 * r.standard_normal((h, w))).astype(np.float32),
+                               (0.02 * r.standard_normal((h, w))).astype(np.float32))
+        b.tuning = dict(block_threads=512, workgroups_per_cu=1, cells_per_thread=1, order=0, launch=kind)
+        return b
+    steps = 8 * 4101 + 3
+    with make(capi.CS_LAUNCH_TEMPORAL_PERSISTENT) as a, make(capi.CS_LAUNCH_TEMPORAL) as t:
+        a.step(steps)
+        t.step(steps)
+        for i in range(len(shapes)):
+            np.testing.assert_array_equal(a.download_raw(i), t.download_raw(i), err_msg=f"lattice {i} {shapes[i]}")
+
 @pytest.mark.parametrize("storage", ["f32", "f32dev"])
 def test_persistent_temporal_fluid_batch(ctx, storage):
     """One boundary-mode group without geometry (periodic, the register
