@@ -24,7 +24,7 @@
 // 4.91 / 3.81 one-step; 576^2 (1296 small regions) 4.76 / 3.93 against
 // 4.85 / 3.56 with 64 x 64 regions, where those take over.  KBC, whose
 // one-step kernel is already VALU-bound (18 divisions per cell), keeps it:
-// 4096^2, 342 against 267 us per step.  cs_set_temporal_steps(4 or 8)
+// 4096^2, 314 against 299 us per step.  cs_set_temporal_steps(4 or 8)
 // forces the 64 x 64 kernels on every FP32 lattice, (0) switches temporal
 // blocking off; a launch-shape override from cs_set_cells_per_thread selects
 // the one-step kernels.
