@@ -125,6 +125,9 @@ void do_steps(cs_lattice* l, int n);
 // -- and one launch of that plan (advances cur, steps and time by K).
 int tb_steps(const cs_lattice* l, int cpt_override);
 inline int tb_plan_steps(int plan) { return plan & 0xFF; }
+// The plan of 4 steps per launch on the same regions, for the remainder of
+// a run of 8-step launches (BGK lattices; 0 when there is none).
+int tb_remainder_plan(const cs_lattice* l, int plan);
 void launch_tb(cs_lattice* l, int plan);
 // The render passes of the current populations into d_out (device, h*w pixels
 // of `channels` bytes: 3 = RGB, 4 = RGBA with alpha 255), enqueued on the

@@ -621,13 +621,20 @@ static void replay_steps(cs_lattice* l, int k)
     l->steps += GRAPH_STEPS;
 }
 
-// n steps from the current state: K-step launches while n >= K, then one
-// step per launch (plan: tb_steps's, 0 for one step per launch).
+// n steps from the current state: K-step launches while n >= K, one 4-step
+// launch when at least 4 remain (BGK), then one step per launch (plan:
+// tb_steps's, 0 for one step per launch).
 static void advance(cs_lattice* l, int n, int plan)
 {
     const int k = tb_plan_steps(plan);
-    if (k > 0)
+    if (k > 0) {
         for (; n >= k; n -= k) launch_tb(l, plan);
+        const int plan4 = tb_remainder_plan(l, plan);
+        if (plan4 != 0 && n >= 4) {
+            launch_tb(l, plan4);
+            n -= 4;
+        }
+    }
     for (int s = 0; s < n; ++s) step_once(l);
 }
 

@@ -96,9 +96,15 @@ void launch_coll(cs_lattice* l)
 template <class St, int EDGE, int MODE>
 void launch_steps(cs_lattice* l, int plan)
 {
-    if ((plan >> 8) == TB_EDGE_SMALL) launch_coll<St, EDGE, MODE, 8, TB_EDGE_SMALL>(l);
-    else if (tb_plan_steps(plan) == 4) launch_coll<St, EDGE, MODE, 4, TB_EDGE>(l);
-    else launch_coll<St, EDGE, MODE, 8, TB_EDGE>(l);
+    const bool four = tb_plan_steps(plan) == 4;  // BGK only (launch_coll takes the operator at K = 8)
+    if ((plan >> 8) == TB_EDGE_SMALL) {
+        if (four) launch_coll<St, EDGE, MODE, 4, TB_EDGE_SMALL>(l);
+        else launch_coll<St, EDGE, MODE, 8, TB_EDGE_SMALL>(l);
+    } else if (four) {
+        launch_coll<St, EDGE, MODE, 4, TB_EDGE>(l);
+    } else {
+        launch_coll<St, EDGE, MODE, 8, TB_EDGE>(l);
+    }
 }
 
 template <class St, int EDGE>
@@ -137,6 +143,12 @@ int tb_steps(const cs_lattice* l, int cpt_override)
     // K = 4 is kept for BGK; the other operators run 8 steps per launch
     if (k == 4 && l->params.collision != CS_COLL_BGK) return tb_plan(8, TB_EDGE);
     return tb_plan(k, TB_EDGE);
+}
+
+int tb_remainder_plan(const cs_lattice* l, int plan)
+{
+    if (tb_plan_steps(plan) != 8 || l->params.collision != CS_COLL_BGK) return 0;
+    return (plan & ~0xFF) | 4;
 }
 
 void launch_tb(cs_lattice* l, int plan)
