@@ -163,7 +163,8 @@ int cs_lattice_init_nonequilibrium(cs_lattice* lat, const float* rho, const floa
 /* ---- the step ------------------------------------------------------------ */
 /* n x State::step (lbm.rs:694-714): stream, bounce-back and BGK collide fused
  * into one pull kernel per step, or (cs_lattice_temporal_steps) k steps per
- * launch with the remainder one step at a time.  Asynchronous. */
+ * launch, a remainder of 4 or more steps in one 4-step launch (BGK), the
+ * rest one step at a time.  Asynchronous. */
 int cs_lattice_step(cs_lattice* lat, int n);
 /* As cs_lattice_step, bracketed by HIP events on the lattice's stream;
  * synchronises and returns the elapsed milliseconds. */
