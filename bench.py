@@ -51,13 +51,21 @@ def hbm_frac(bytes_per_cell: float, cells: int, kernel_ms: float) -> float:
 TB_EDGE, TB_STEPS = 64, 8  # region side and steps per launch of the temporally blocked kernel
 
 
-def tb_bytes_per_step(w: int, h: int, esize: int = 4, k: int = TB_STEPS) -> float:
+def tb_region_edge(w: int, h: int) -> int:
+    """Region side the library takes for a w x h FP32 lattice: 32 while it has
+    at most 1024 regions of 32 x 32 (interior 16 x 16 at 8 steps), else 64."""
+    return 32 if -(-w // 16) * -(-h // 16) <= 1024 else TB_EDGE
+
+
+def tb_bytes_per_step(w: int, h: int, esize: int = 4, k: int = TB_STEPS, edge: int | None = None) -> float:
     """Bytes one launch of the temporally blocked kernel must move, per step:
-    every 64 x 64 region (interior of side 64 - 2k plus halo) read once, every
-    lattice cell written once, 9 populations of `esize` bytes."""
-    inner = TB_EDGE - 2 * k
+    every region (interior of side edge - 2k plus halo) read once, every
+    lattice cell written once, 9 populations of `esize` bytes.  `edge`: the
+    region side (default: the single-lattice choice; batches use 64)."""
+    edge = tb_region_edge(w, h) if edge is None else edge
+    inner = edge - 2 * k
     regions = -(-w // inner) * -(-h // inner)
-    return (regions * TB_EDGE * TB_EDGE + w * h) * 9 * esize / k
+    return (regions * edge * edge + w * h) * 9 * esize / k
 
 
 def step_bytes_per_cell(storage: str, n: int, k: int) -> float:
@@ -66,6 +74,20 @@ def step_bytes_per_cell(storage: str, n: int, k: int) -> float:
     if k > 1:
         return tb_bytes_per_step(n, n, k=k) / (n * n)
     return BYTES_PER_CELL[storage]
+
+
+def run_bytes_per_cell(storage: str, n: int, k: int, steps: int) -> float:
+    """Algorithmic bytes per cell update of a run of `steps` BGK steps on an
+    n x n lattice: the k-step launches, then (k = 8) one 4-step launch for a
+    remainder of 4 or more steps, then one-step launches."""
+    if k <= 1:
+        return BYTES_PER_CELL[storage]
+    nk, r = divmod(steps, k)
+    n4 = 1 if k == 8 and r >= 4 else 0
+    n1 = r - 4 * n4
+    total = nk * k * tb_bytes_per_step(n, n, k=k) + n4 * 4 * tb_bytes_per_step(n, n, k=4) \
+        + n1 * n * n * BYTES_PER_CELL[storage]
+    return total / (n * n * steps)
 
 
 def warm_kernels(ctx, n, storage, tau, u0):
@@ -288,7 +310,7 @@ def batch_cavities_run(ctx, steps=5000, per_lattice_steps=20, other_steps=1000):
     # temporal kind every 64 x 64 region read and every cell written once per 8 steps
     bpc = 72.0
     if best.get("launch", 0) in (capi.CS_LAUNCH_TEMPORAL, capi.CS_LAUNCH_TEMPORAL_PERSISTENT):
-        bpc = sum(tb_bytes_per_step(s.side, s.side) for s in specs) / cells
+        bpc = sum(tb_bytes_per_step(s.side, s.side, edge=TB_EDGE) for s in specs) / cells
     out = {"lattices": len(b), "cells": cells, "steps": steps,
            "GLUPS_batch": round(cells * steps / ms / 1e6, 3), "hbm_frac": hbm_frac(bpc, cells, ms / steps),
            "bytes_per_cell_update": round(bpc, 3), "launch": kinds[best.get("launch", 0)], **others,
@@ -346,7 +368,7 @@ def main():
     kernel_ms = ms / args.steps
     k_launch = lat.temporal_steps()
     temporal = k_launch > 1
-    bpc = step_bytes_per_cell(storage, n, k_launch)
+    bpc = run_bytes_per_cell(storage, n, k_launch, args.steps)
     achieved = bpc * cells / (kernel_ms * 1e-3) / 1e9
     log(f"[bench] {glups:.2f} GLUPS wall, {kernel_ms * 1e3:.1f} us/step (events), {achieved:.0f} GB/s algorithmic")
     # four more timed regions of the same length, for the spread (value stays the first)
@@ -379,7 +401,7 @@ def main():
                                      "kernel_us": round(ms1 / args.steps * 1e3, 2),
                                      "hbm_frac": hbm_frac(BYTES_PER_CELL[storage], cells, ms1 / args.steps),
                                      "bytes_per_cell": BYTES_PER_CELL[storage]}
-        extras["temporal_blocking"] = {"steps_per_launch": k_launch, "region": [TB_EDGE, TB_EDGE],
+        extras["temporal_blocking"] = {"steps_per_launch": k_launch, "region": [tb_region_edge(n, n)] * 2,
                                        "bytes_per_cell_update": round(bpc, 3),
                                        "speedup_vs_one_step": round(extras["one_step_kernel"]["kernel_us"]
                                                                     / (kernel_ms * 1e3), 3)}
@@ -527,7 +549,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "bytes_per_cell": round(bpc, 3), "kernel_us": round(kernel_ms * 1e3, 2),
-                     "kernel": f"temporally blocked, {k_launch} steps per launch" if temporal else "k_step / k_step_vec",
+                     "kernel": (f"temporally blocked, {k_launch} steps per launch (a remainder of 4 or more steps: "
+                                "one 4-step launch)") if temporal else "k_step / k_step_vec",
                      "per": "launch" if temporal else "step"},
         "cpu_baseline": cpu,
         "extras": extras,
