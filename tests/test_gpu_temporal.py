@@ -66,7 +66,9 @@ def test_bgk_f32_matches_oracle(ctx, edge, shape, steps_per_launch):
     with Lattice(ctx, w, h, "f32", gpu_params(op)) as lat:
         lat.upload(f)
         ref = f
-        for n in (19, 8):  # K-step launches, then the remainder one step at a time
+        # 8-step launches, then a remainder of 3 one step at a time (19), of 5 as a
+        # 4-step launch and one step (13)
+        for n in (19, 8, 13):
             lat.step(n)
             ref = O.run(op, ref, n)
             np.testing.assert_array_equal(lat.download(), ref)
