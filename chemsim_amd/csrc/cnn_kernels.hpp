@@ -73,7 +73,8 @@ struct GemmArgs {
     const uint16_t* mask;   // EPI_BF16_MASK: [M][ldc] bf16 ReLU output whose zeros mask C
 };
 
-constexpr int BK = 32;
+constexpr int BK = 64;
+constexpr int KV = BK / 8;         // 16-byte vectors per BK-wide row
 constexpr int LDS_PITCH = BK + 8;  // bf16 elements per LDS row (padding against bank conflicts)
 
 // Channels c..c+7 of pixel q = (img, yy, xx) of the gathered tensor `in`,
@@ -94,7 +95,7 @@ __device__ __forceinline__ u32x4 gathered8(const uint16_t* __restrict__ in, cons
 template <int L, int R>
 struct TileLoader {
     static_assert(L == KMAJOR, "K-contiguous operand");
-    static constexpr int VECS = 4 * R;                  // R rows x BK/8 vectors
+    static constexpr int VECS = KV * R;                 // R rows x BK/8 vectors
     static constexpr int NV = (VECS + 255) / 256;
     u32x4 v[NV];
 
@@ -107,7 +108,7 @@ struct TileLoader {
             const int e = t + 256 * i;
             v[i] = u32x4{0u, 0u, 0u, 0u};
             if (VECS % 256 != 0 && e >= VECS) continue;
-            const int r = e >> 2, kg = e & 3;
+            const int r = e / KV, kg = e % KV;
             if (r0 + r < rows && k0 + 8 * kg < kend)
                 v[i] = *(const u32x4*)(src + (int64_t)(r0 + r) * ld + k0 + 8 * kg);
         }
@@ -119,7 +120,7 @@ struct TileLoader {
         for (int i = 0; i < NV; ++i) {
             const int e = t + 256 * i;
             if (VECS % 256 != 0 && e >= VECS) continue;
-            *(u32x4*)(&lds[e >> 2][8 * (e & 3)]) = v[i];
+            *(u32x4*)(&lds[e / KV][8 * (e % KV)]) = v[i];
         }
     }
 };
@@ -128,7 +129,7 @@ struct TileLoader {
 // so its coordinates are computed once; a K slice then only decodes the tap.
 template <int R>
 struct GatherLoader {
-    static constexpr int VECS = 4 * R;
+    static constexpr int VECS = KV * R;
     static constexpr int NV = (VECS + 255) / 256;
     u32x4 v[NV];
     int y[NV], x[NV], img[NV];
@@ -140,7 +141,7 @@ struct GatherLoader {
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             const int e = t + 256 * i;
-            const int p = r0 + (e >> 2);
+            const int p = r0 + e / KV;
             y[i] = -(1 << 28);  // marks an unused slot: every tap falls outside
             x[i] = 0;
             img[i] = 0;
@@ -160,7 +161,7 @@ struct GatherLoader {
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             const int e = t + 256 * i;
-            const int k = k0 + 8 * (e & 3);
+            const int k = k0 + 8 * (e % KV);
             v[i] = u32x4{0u, 0u, 0u, 0u};
             if (k < kend && k < 9 * g.gC) {
                 const int tap = k / g.gC, c = k - tap * g.gC;
@@ -182,7 +183,7 @@ struct GatherLoader {
         for (int i = 0; i < NV; ++i) {
             const int e = t + 256 * i;
             if (VECS % 256 != 0 && e >= VECS) continue;
-            *(u32x4*)(&lds[e >> 2][8 * (e & 3)]) = v[i];
+            *(u32x4*)(&lds[e / KV][8 * (e % KV)]) = v[i];
         }
     }
 };
@@ -334,15 +335,19 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
             la.load(g, m0, k0 + BK, ke);
             lb.load(g, n0, k0 + BK, ke);
         }
-        bf16x8 a[TM], b[TN];
 #pragma unroll
-        for (int i = 0; i < TM; ++i) a[i] = *(const bf16x8*)(&As[cur][wm + 16 * i + fr][fk]);
+        for (int ks = 0; ks < BK; ks += 32) {
+            bf16x8 a[TM], b[TN];
 #pragma unroll
-        for (int j = 0; j < TN; ++j) b[j] = *(const bf16x8*)(&Bs[cur][wn + 16 * j + fr][fk]);
+            for (int i = 0; i < TM; ++i) a[i] = *(const bf16x8*)(&As[cur][wm + 16 * i + fr][ks + fk]);
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
+            for (int j = 0; j < TN; ++j) b[j] = *(const bf16x8*)(&Bs[cur][wn + 16 * j + fr][ks + fk]);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
         if (more) {
             la.store(As[cur ^ 1]);
             lb.store(Bs[cur ^ 1]);
