@@ -72,7 +72,7 @@ def test_bgk_f32_matches_oracle(ctx, edge, shape, steps_per_launch):
             lat.step(n)
             ref = O.run(op, ref, n)
             np.testing.assert_array_equal(lat.download(), ref)
-        assert lat.time == 27.0
+        assert lat.time == 40.0
 
 
 @pytest.mark.parametrize("edge", [O.EDGE_ZERO, O.EDGE_PERIODIC])
