@@ -166,10 +166,13 @@ def steady_states(ctx: Context, chans, tol: float = 1e-6, interval: int = 100, c
     `interval` steps is <= tol.  Returns (fields, steps, residuals)."""
     n = len(chans)
     b = Batch(ctx, [(c.w, c.h) for c in chans], storage)
+    t_set = time.perf_counter()
     for i, c in enumerate(chans):
         b.set_params(i, lbm_params(c))
         b.set_flags(i, c.flags())
         b.init_equilibrium(i, *cold_state(c))
+        if (i + 1) % 500 == 0:
+            log(f"[steady] {i + 1}/{n} lattices set up ({time.perf_counter() - t_set:.0f}s)")
     done = np.zeros(n, bool)
     steps = np.full(n, -1, np.int64)
     res = np.full(n, np.inf)
@@ -187,9 +190,8 @@ def steady_states(ctx: Context, chans, tol: float = 1e-6, interval: int = 100, c
             b.set_active(int(i), False)
             steps[i] = taken
         done |= newly
-        if (taken // chunk) % 10 == 0:
-            log(f"[steady] {taken} steps: {done.sum()}/{n} converged, max residual "
-                f"{np.max(np.where(done, 0, r)):.2e}, {time.perf_counter() - t0:.0f}s")
+        log(f"[steady] {taken} steps: {done.sum()}/{n} converged, max residual "
+            f"{np.max(np.where(done, 0, r)):.2e}, {time.perf_counter() - t0:.0f}s")
     fields = [b.moments(i) for i in range(n)]
     b.close()
     return fields, steps, res
