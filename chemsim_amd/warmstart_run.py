@@ -149,8 +149,16 @@ def time_step(ctx: Context, base: int = 32, batch: int = 8, iters: int = 10) -> 
 
 
 def dataset(ctx: Context, chans, args):
+    """Training arrays from the steady states of `chans`; channels that did not
+    reach the residual within --data-max-steps are left out (their fields are
+    not steady states and would teach the network transients)."""
     fields, steps, res = WS.steady_states(ctx, chans, tol=args.data_tol, interval=100, chunk=args.chunk,
                                           max_steps=args.data_max_steps, storage=args.storage, log=log)
+    keep = [i for i in range(len(chans)) if steps[i] >= 0]
+    if len(keep) < len(chans):
+        log(f"[data] {len(chans) - len(keep)} unconverged channels left out")
+    chans = [chans[i] for i in keep]
+    fields = [fields[i] for i in keep]
     n = len(chans) * (2 if args.mirror else 1)
     h, w = chans[0].h, chans[0].w
     x = np.empty((n, h, w, 8), np.float32)
