@@ -280,12 +280,13 @@ enum cs_batch_order { CS_ORDER_LARGEST_FIRST = 0, CS_ORDER_GIVEN = 1, CS_ORDER_S
  * cells_per_thread, order apply).  CS_LAUNCH_TILES: one launch per step over
  * the (block_threads / 64) x 64 tiles of every lattice.  CS_LAUNCH_TEMPORAL
  * (FP32 and FP32-deviation storage): one launch per 8 steps over the 64 x 64
- * regions of every lattice, each run 8 steps out of LDS (the temporally
- * blocked kernel); the remaining steps of a call (steps mod 8) as tile
- * launches (block_threads applies).  CS_LAUNCH_TEMPORAL_PERSISTENT: the same
- * rounds of 8 steps in one cooperative launch whose resident blocks meet at a
- * grid-wide barrier after every round (a batch whose lattices fall into more
- * than one boundary-mode group runs as CS_LAUNCH_TEMPORAL). */
+ * regions of every lattice, each run 8 steps in registers or LDS (the
+ * temporally blocked kernels); the remaining steps of a call (steps mod 8) as
+ * tile launches (block_threads applies).  CS_LAUNCH_TEMPORAL_PERSISTENT: the
+ * same rounds of 8 steps in one cooperative launch (per 4096 rounds) whose
+ * resident blocks draw regions one at a time and meet at a grid-wide barrier
+ * after every round (a batch whose lattices fall into more than one
+ * boundary-mode group runs as CS_LAUNCH_TEMPORAL). */
 enum cs_batch_launch {
     CS_LAUNCH_PERSISTENT = 0,
     CS_LAUNCH_TILES = 1,
