@@ -91,9 +91,11 @@ def test_record_argument_checks(ctx):
         assert rec.shape[0] == 0 and stats["frames"] == 0
 
 
-def test_record_large_frames_overlap(ctx):
+def test_record_large_frames_depths_agree(ctx):
     """A 2048 x 2048 Taylor-Green recording: depth 3 equals depth 1 byte for
-    byte and does not take longer."""
+    byte.  The wall times are printed, not asserted: they are dominated by
+    this test's host callback (summing each 16 MB frame), whose speed varies
+    from box to box; bench.py's record_4096 leg measures the overlap."""
     n = 2048
     rho, ux, uy = W.taylor_green(n, 0.05, np.float32)
     p = Params(tau=0.8, edge=capi.CS_EDGE_PERIODIC)
@@ -107,8 +109,8 @@ def test_record_large_frames_overlap(ctx):
                              lambda k, rgba: acc.append(int(rgba.astype(np.int64).sum())))
             sums[depth] = acc
             walls[depth] = st["wall_ms"]
+    print(f"[record] 2048^2 wall ms by ring depth: {walls}", flush=True)
     assert sums[1] == sums[3]
-    assert walls[3] <= walls[1] * 1.2, walls
 
 
 @pytest.mark.parametrize("w,h", [(5, 3), (1, 7), (33, 1)])
