@@ -150,8 +150,38 @@ def warmstart_leg(ctx, count=16):
     return out
 
 
+_wake_lattices = {}
+
+
+def wake_gpu(ctx, seconds=0.05):
+    """About `seconds` of FP32 steps on a scratch TGV 2048^2 lattice (one per
+    context, kept for the run), right before a timed region.  A GPU left idle
+    runs its first milliseconds of work slower: 20 steps of TGV 4096^2 FP32
+    timed after one second of idle host-side set-up (building a lattice)
+    measure 190-202 GLUPS, 212-215 after 10 ms idle, 227-232 right after
+    0.05-1 s of other lattices' steps (1,000 steps: 244).  0.05 s of 256 MiB
+    copies instead brings the wall figure to 212-215 but leaves the kernels'
+    time where it was (77 against 78 us per step; after steps: 71.5)."""
+    scratch = _wake_lattices.get(id(ctx))
+    if scratch is None:
+        scratch = _wake_lattices[id(ctx)] = tgv_lattice(ctx, 2048, "f32", 0.8, 0.05)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        scratch.step(512)
+        scratch.sync()
+
+
+def close_wake_lattices():
+    for scratch in _wake_lattices.values():
+        scratch.close()
+    _wake_lattices.clear()
+
+
 def time_steps(lat, steps):
+    """Wall seconds and HIP-event milliseconds of `steps` steps of `lat`, on a
+    GPU kept busy up to the start (wake_gpu)."""
     lat.sync()
+    wake_gpu(lat.ctx)
     t0 = time.perf_counter()
     ms = lat.step_timed(steps)          # HIP events on the lattice's stream; synchronises
     lat.sync()
@@ -556,6 +586,7 @@ def main():
         "extras": extras,
     }
     print(json.dumps(line), flush=True)
+    close_wake_lattices()
     ctx.close()
 
 
