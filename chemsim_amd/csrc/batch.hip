@@ -326,9 +326,9 @@ __global__ __launch_bounds__(TBR_THREADS, 4) void k_batch_tbr(const BatchLat<typ
 // Grid-wide barrier of the persistent temporal kind: a monotonic arrival
 // counter (barrier r waits for (r + 1) x blocks arrivals) with agent-scope
 // release/acquire ordering, so every block's stores of a round are visible to
-// every block (on any XCD) after it.  The wait is bounded: past about a
-// second it raises `abort`, every block leaves at its next barrier and the
-// host reports the timeout -- the grid always drains.  `next` hands out the
+// every block (on any XCD) after it.  The wait is bounded: past 2^24 polls
+// (some seconds) it raises `abort`, every block leaves at its next barrier
+// and the host reports the timeout -- the grid always drains.  `next` hands out the
 // work: draw i is region i % nregions of round i / nregions.  The host clears
 // `next` and `count` before each launch and `abort` once per call.
 struct GridBarrier {
