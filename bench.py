@@ -9,7 +9,9 @@ of 4 or more steps one 4-step launch, the rest one step per launch); the
 one-step kernel of the same workload is timed beside it in `extras`.  Before
 the warm-up, each kernel the timed steps use is launched once on a scratch
 lattice of the same shape, so first-launch costs stay out of the timed region
-(the measured lattice takes exactly W + K steps).
+(the measured lattice takes exactly W + K steps), and right before every timed
+region a scratch lattice's steps keep the GPU busy for 0.05 s (wake_gpu);
+`extras.idle_start` times the same region after one second of idle GPU.
 
 Prints ONE JSON line.  `value` is lattice updates per second (GLUPS) over the
 timed steps (wall clock around a synchronised region); the roofline object
@@ -177,11 +179,12 @@ def close_wake_lattices():
     _wake_lattices.clear()
 
 
-def time_steps(lat, steps):
+def time_steps(lat, steps, wake=True):
     """Wall seconds and HIP-event milliseconds of `steps` steps of `lat`, on a
-    GPU kept busy up to the start (wake_gpu)."""
+    GPU kept busy up to the start (wake_gpu) unless `wake` is false."""
     lat.sync()
-    wake_gpu(lat.ctx)
+    if wake:
+        wake_gpu(lat.ctx)
     t0 = time.perf_counter()
     ms = lat.step_timed(steps)          # HIP events on the lattice's stream; synchronises
     lat.sync()
@@ -406,6 +409,9 @@ def main():
     for _ in range(4):
         w_r, _ = time_steps(lat, args.steps)
         reps.append(cells * args.steps / w_r / 1e9)
+    # the same region once more, started after one second of idle GPU (no wake-up)
+    time.sleep(1.0)
+    w_i, ms_i = time_steps(lat, args.steps, wake=False)
 
     copy_bytes = 1 << 30
     copy_ms = ctx.bench_copy(copy_bytes, 20)
@@ -418,7 +424,10 @@ def main():
               if storage == "f32" else None,
               "mass_final": float(s_final[0]),
               "repeats_GLUPS": [round(r, 3) for r in reps],
-              "median_GLUPS": round(float(np.median(reps)), 3)}
+              "median_GLUPS": round(float(np.median(reps)), 3),
+              "idle_start": {"GLUPS": round(cells * args.steps / w_i / 1e9, 3),
+                             "kernel_us": round(ms_i / args.steps * 1e3, 2),
+                             "note": "the same timed region after 1 s of idle GPU, without the wake-up"}}
     if temporal:
         # the one-step kernel (vector lanes, 72 B per cell update) on the same lattice and steps
         from chemsim_amd import capi as _cap
